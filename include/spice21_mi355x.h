@@ -1,0 +1,221 @@
+/*
+ * spice21_mi355x.h -- C ABI of the MI355X-native Spice21 Newton engine.
+ *
+ * The engine replaces the bodies of Spice21's Newton inner loop: component loading and matrix
+ * stamping (spice21/src/analysis.rs:153-168), the residual (spice21/src/sparse21/mod.rs:298-327),
+ * and LU factorisation plus forward/backward solves (spice21/src/sparse21/mod.rs:647-991), for
+ * batches of circuit instances resident on one MI355X.
+ *
+ * Every entry point is synchronous, never unwinds across the ABI, returns an S21_* status
+ * and leaves the reference's error text (SpError.desc, spice21/src/spresult.rs:9-48) in
+ * s21_last_error().  No torch or HIP types appear in any signature.
+ */
+#ifndef SPICE21_MI355X_H
+#define SPICE21_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S21_ABI_VERSION 1
+#define S21_GND (-1)
+
+/* ---- status codes ------------------------------------------------------------------------ */
+enum {
+    S21_OK = 0,
+    S21_ERR = 1,          /* any other SpError; text in s21_last_error()                        */
+    S21_CONVERGENCE = 2,  /* "Convergence Failed"        analysis.rs:209 (real), :302 (complex)  */
+    S21_SINGULAR = 3,     /* "Singular Matrix"           sparse21/mod.rs:651,656,868,957,971     */
+    S21_PIVOT = 4,        /* "Pivot Search Fail" / zero pivot  sparse21/mod.rs:665,872          */
+    S21_DECODE = 5,       /* "Decode Error: ..."         proto.rs:15-19                          */
+    S21_INVALID = 6,      /* malformed argument (null pointer, bad index, size beyond limits)    */
+    S21_DEVICE = 7,       /* HIP runtime failure or no gfx950 device                             */
+    S21_NONFINITE = 8     /* a NaN/Inf reached the Newton update (per-instance status only)      */
+};
+
+/* ---- flattened netlist (the output of elaboration, elab.rs:244-265) ----------------------- */
+enum { S21_VAR_V = 0, S21_VAR_I = 1 };                       /* analysis.rs:33-38 */
+
+enum {                                                      /* comps/mod.rs:60-72 */
+    S21_COMP_R = 0,      /* Resistor, value = conductance g          comps/mod.rs:271-323 */
+    S21_COMP_C = 1,      /* Capacitor, value = c                      comps/mod.rs:152-239 */
+    S21_COMP_I = 2,      /* Isrc, value = dc                          comps/mod.rs:326-346 */
+    S21_COMP_V = 3,      /* Vsrc, value = dc, acm = AC magnitude      comps/mod.rs:95-150  */
+    S21_COMP_MOS1 = 4,   /* Mos level 1                               comps/mos.rs:625-969 */
+    S21_COMP_MOS0 = 5,   /* Mos level 0 (model = 0 NMOS / 1 PMOS)     comps/mos.rs:1007-1100 */
+    S21_COMP_DIODE = 6   /* Diode                                     comps/diode.rs:215-355 */
+};
+
+/* Optional-double parameter set keyed by proto field number (protos/mos.proto:22-76,
+ * protos/spice21.proto:43-68): v[f] is used iff bit f of `set` is 1, otherwise the reference
+ * default applies.  Enum and integer fields (Mos1Model.mos_type = 2, tpg = 28) are stored as
+ * doubles. */
+#define S21_PARAM_SLOTS 40
+typedef struct {
+    double v[S21_PARAM_SLOTS];
+    uint64_t set;
+} s21_params;
+
+typedef struct {
+    int32_t type;     /* S21_COMP_* */
+    int32_t node[6];  /* variable indices, S21_GND for ground:
+                         R, C, I: {p, n}; V: {p, n, ivar}; MOS1: {d, g, s, b, dp, sp};
+                         MOS0: {d, g, s, b}; DIODE: {p, n, r}                          */
+    int32_t model;    /* MOS1: index into mos1_models; DIODE: into diode_models; MOS0: 0|1 */
+    int32_t inst;     /* MOS1: index into mos1_insts; DIODE: into diode_insts             */
+    double value;     /* R: g; C: c; I: dc; V: dc                                         */
+    double acm;       /* V: AC magnitude                                                  */
+} s21_comp;
+
+typedef struct {
+    int32_t n_vars;
+    const int32_t* var_kind;          /* [n_vars] S21_VAR_*                              */
+    const char* const* var_name;      /* [n_vars] signal names, may be NULL              */
+    int32_t n_comps;
+    const s21_comp* comps;            /* [n_comps] in elaboration order                  */
+    int32_t n_mos1_models;  const s21_params* mos1_models;
+    int32_t n_mos1_insts;   const s21_params* mos1_insts;
+    int32_t n_diode_models; const s21_params* diode_models;
+    int32_t n_diode_insts;  const s21_params* diode_insts;
+} s21_netlist;
+
+/* Simulation options, defaults as analysis.rs:665-693:
+ * temp = tnom = 300.15 K, gmin = 1e-12 S, iabstol = 1e-12 A, reltol = 1e-3. */
+typedef struct {
+    double temp, tnom, gmin, iabstol, reltol;
+} s21_options;
+
+/* Transient options (analysis.rs:451-456).  Initial conditions force node ic_var[k] to ic_val[k]
+ * through a 1 S resistor and a Vsrc during the operating point, then 1e-9 S / 0 V for the rest
+ * of the run (analysis.rs:510-549); ic_name[k] names the appended ".{name}.vic" / ".{name}.iic"
+ * variables. */
+typedef struct {
+    double tstep, tstop;
+    int32_t n_ic;
+    const int32_t* ic_var;
+    const double* ic_val;
+    const char* const* ic_name;
+} s21_tran_options;
+
+/* AC options (analysis.rs:700-714): npts is the total count, not per decade. */
+typedef struct {
+    uint64_t fstart, fstop, npts;
+} s21_ac_options;
+
+/* ---- library / errors --------------------------------------------------------------------- */
+int32_t s21_abi_version(void);
+/* Copies the text of the most recent error of the calling thread; returns its full length. */
+size_t s21_last_error(char* buf, size_t len);
+void s21_default_options(s21_options* opts);
+/* Number of time points the reference's fixed-step loop produces, including the OP point at
+ * t = 0 (analysis.rs:551-570: t = tstep; while t < tstop { ...; t += tstep }). */
+int64_t s21_tran_point_count(double tstep, double tstop);
+/* Number of AC frequencies (analysis.rs:791-819, including both ends). */
+int64_t s21_ac_point_count(const s21_ac_options* ac);
+
+/* ---- circuits and elaboration (circuit.rs, elab.rs) ---------------------------------------- */
+typedef struct s21_ckt s21_ckt;
+typedef struct s21_elab s21_elab;
+
+/* Decode a protobuf `spice21.Circuit` (protos/spice21.proto:128-133) as Ckt::decode /
+ * Ckt::from_proto do (circuit.rs:248-256, 281-330). */
+int s21_ckt_decode(const uint8_t* buf, size_t len, s21_ckt** out);
+/* Register a level-0 MOS model name, as the reference's test helper does by inserting into
+ * `defs.mos0` (tests.rs:1443-1447); proto carries no Mos0 definition. */
+int s21_ckt_add_mos0(s21_ckt* ckt, const char* name, int32_t mos_type);
+void s21_ckt_free(s21_ckt* ckt);
+/* Flatten the hierarchy into variables and components (elab.rs:244-265). */
+int s21_elaborate(const s21_ckt* ckt, s21_elab** out);
+/* View of the flattened netlist; valid until s21_elab_free. */
+int s21_elab_netlist(const s21_elab* e, s21_netlist* out);
+void s21_elab_free(s21_elab* e);
+
+/* ---- batched engine -------------------------------------------------------------------------
+ * A batch holds one or more groups.  A group is one topology (a flattened netlist) times n_inst
+ * instances that differ only in parameter values.  All instances of all groups are solved by one
+ * persistent kernel launch per analysis call. */
+typedef struct s21_batch s21_batch;
+
+enum { S21_OVR_VALUE = 0, S21_OVR_MODEL = 1, S21_OVR_INST = 2 };
+typedef struct {
+    int32_t comp;         /* component index in the group's netlist                       */
+    int32_t target;       /* S21_OVR_VALUE: comp.value; S21_OVR_MODEL / S21_OVR_INST: a
+                             proto field of that component's model / instance params      */
+    int32_t field;        /* proto field number for MODEL / INST targets                  */
+    const double* vals;   /* [n_inst] per-instance values                                 */
+} s21_override;
+
+typedef struct {
+    const s21_netlist* net;
+    int32_t n_inst;
+    int32_t n_over;
+    const s21_override* over;
+} s21_group;
+
+enum { S21_FP64 = 0, S21_MIXED = 1 };
+
+/* Build per-instance derived parameters (comps/mos.rs:320-476, comps/diode.rs:146-212) and keep
+ * the groups resident on the current HIP device.  precision: S21_FP64 factors and solves in
+ * double; S21_MIXED factors and solves in float while device evaluation, the KCL residual, the
+ * Newton update and the convergence test stay in double. */
+int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_options* opts,
+                     int32_t precision, s21_batch** out);
+void s21_batch_free(s21_batch* b);
+
+/* DC operating point of every instance (analysis.rs:383-388 and Solver::solve :169-210). */
+int s21_batch_op(s21_batch* b);
+/* Fixed-step Backward-Euler transient with initial-condition forcing (analysis.rs:489-574). */
+int s21_batch_tran(s21_batch* b, const s21_tran_options* t);
+/* Operating point, then complex small-signal sweep (analysis.rs:761-832). */
+int s21_batch_ac(s21_batch* b, const s21_ac_options* a);
+
+int32_t s21_batch_n_groups(const s21_batch* b);
+int32_t s21_batch_n_inst(const s21_batch* b, int32_t group);
+/* Unknowns of the group after the last analysis (includes IC variables after a transient). */
+int32_t s21_batch_n_vars(const s21_batch* b, int32_t group);
+/* Name of unknown k of the group after the last analysis (stable until the next analysis). */
+const char* s21_batch_var_name(const s21_batch* b, int32_t group, int32_t k);
+/* Final solution of each instance: out[i * n_vars + k]. */
+int s21_batch_get_x(const s21_batch* b, int32_t group, double* out);
+/* Per-instance S21_* status and Newton iteration totals of the last analysis. */
+int s21_batch_get_status(const s21_batch* b, int32_t group, int32_t* status, int64_t* iters);
+/* Number of stored points of the last transient / AC sweep. */
+int64_t s21_batch_n_points(const s21_batch* b);
+/* Time (or frequency) axis of the last transient (AC) analysis: out[npts]. */
+int s21_batch_get_axis(const s21_batch* b, double* out);
+/* Waveform of one instance: out[p * n_vars + k] (transient); AC writes interleaved re/im,
+ * out[(p * n_vars + k) * 2 + {0,1}]. */
+int s21_batch_get_wave(const s21_batch* b, int32_t group, int32_t inst, double* out);
+/* Device time of the last analysis's Newton kernel launches (HIP events, milliseconds), the
+ * number of those launches, and the average duration of the dominant kernel. */
+int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, double* kernel_avg_ms);
+
+/* ---- introspection used by the parity tests ------------------------------------------------ */
+/* Pivot order chosen by the symbolic phase of `group` for its last analysis: internal-to-
+ * external row and column maps (AxisMapping.i2e, sparse21/mod.rs:126-162), each [n_vars]. */
+int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e, int32_t* col_i2e);
+/* Sizes of the frozen structure: nnz of A, nnz of L+U (incl. fill), LU levels, contributions. */
+int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_a, int64_t* nnz_lu,
+                        int64_t* levels, int64_t* n_contrib);
+/* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the
+ * voltages x[n_vars], in OP mode: returns matrix values per created element, in the order
+ * Matrix::make created them (row_out/col_out give their external positions), and the RHS. */
+int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const double* x,
+                       int32_t* row_out, int32_t* col_out, double* val_out, double* rhs_out,
+                       int64_t* n_elem);
+
+/* ---- byte ABI mirroring CallableProto::call_bytes (proto.rs:32-145) ------------------------ */
+/* Op -> OpResult, Tran -> TranResult, Ac -> AcResult, all protobuf-encoded.  *out is allocated
+ * by the library and released with s21_free_bytes. */
+int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+void s21_free_bytes(uint8_t* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPICE21_MI355X_H */

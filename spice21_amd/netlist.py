@@ -1,0 +1,160 @@
+"""Flattened-netlist builder.
+
+Builds the ``s21_netlist`` that the engine's C ABI takes, creating variables in exactly the order the
+reference's top-level elaboration does (spice21/src/elab.rs:35-173): terminals are looked up or created
+in port order, a Vsrc appends its branch current after its terminals, a Mos1 appends ``{name}.dp`` /
+``{name}.sp`` when its model has rd/rs/rsh (comps/mos.rs:592-620), and a Diode appends ``{name}.r``
+when its model has a nonzero rs (comps/diode.rs:100-110).  Hierarchical circuits go through the
+library's own elaborator instead (``spice21_amd.Circuit``).
+"""
+import ctypes as C
+
+from . import _abi as A
+
+
+class Netlist:
+    def __init__(self, signals=()):
+        self.names = []
+        self.kinds = []
+        self.comps = []  # tuples (type, nodes[6], model, inst, value, acm, name)
+        self.mos1_models = []
+        self.mos1_insts = []
+        self.diode_models = []
+        self.diode_insts = []
+        self._mos1_model_r = []  # (has_rd_or_rsh, has_rs_or_rsh)
+        self._diode_has_rs = []
+        for s in signals:
+            self.addv(s)
+
+    # ---- variables (analysis.rs:43-123) ----
+    def add(self, name, kind):
+        self.names.append(name)
+        self.kinds.append(kind)
+        return len(self.names) - 1
+
+    def addv(self, name):
+        return self.add(name, A.S21_VAR_V)
+
+    def addi(self, name):
+        return self.add(name, A.S21_VAR_I)
+
+    def find(self, name):
+        try:
+            return self.names.index(name)
+        except ValueError:
+            return None
+
+    def node(self, name):
+        """Variables::find_or_create for a top-level node; '' / None is ground."""
+        if name is None or name == "":
+            return A.S21_GND
+        name = str(name)
+        k = self.find(name)
+        return k if k is not None else self.addv(name)
+
+    @property
+    def n_vars(self):
+        return len(self.names)
+
+    # ---- definitions ----
+    def mos1_model(self, **kw):
+        self.mos1_models.append(A.s21_params.make(A.MOS1_MODEL_FIELDS, **kw))
+        rd = kw.get("rd") is not None or kw.get("rsh") is not None
+        rs = kw.get("rs") is not None or kw.get("rsh") is not None
+        self._mos1_model_r.append((rd, rs))
+        return len(self.mos1_models) - 1
+
+    def mos1_inst(self, **kw):
+        self.mos1_insts.append(A.s21_params.make(A.MOS1_INST_FIELDS, **kw))
+        return len(self.mos1_insts) - 1
+
+    def diode_model(self, **kw):
+        self.diode_models.append(A.s21_params.make(A.DIODE_MODEL_FIELDS, **kw))
+        self._diode_has_rs.append((kw.get("rs") or 0.0) != 0.0)
+        return len(self.diode_models) - 1
+
+    def diode_inst(self, **kw):
+        self.diode_insts.append(A.s21_params.make(A.DIODE_INST_FIELDS, **kw))
+        return len(self.diode_insts) - 1
+
+    # ---- components (elab.rs:55-173) ----
+    def _push(self, typ, nodes, model=0, inst=0, value=0.0, acm=0.0, name=""):
+        nodes = list(nodes) + [A.S21_GND] * (6 - len(nodes))
+        self.comps.append((typ, nodes, model, inst, float(value), float(acm), name))
+        return len(self.comps) - 1
+
+    def r(self, g, p, n="", name="r"):
+        return self._push(A.S21_COMP_R, [self.node(p), self.node(n)], value=g, name=name)
+
+    def c(self, c, p, n="", name="c"):
+        return self._push(A.S21_COMP_C, [self.node(p), self.node(n)], value=c, name=name)
+
+    def i(self, dc, p, n="", name="i"):
+        return self._push(A.S21_COMP_I, [self.node(p), self.node(n)], value=dc, name=name)
+
+    def v(self, name, dc, p, n="", acm=0.0):
+        pv, nv = self.node(p), self.node(n)
+        iv = self.addi(name)
+        return self._push(A.S21_COMP_V, [pv, nv, iv], value=dc, acm=acm, name=name)
+
+    def mos1(self, name, d, g, s, b, model, inst):
+        nd = [self.node(d), self.node(g), self.node(s), self.node(b)]
+        has_rd, has_rs = self._mos1_model_r[model]
+        dp = self.addv(f"{name}.dp") if has_rd else nd[0]
+        sp = self.addv(f"{name}.sp") if has_rs else nd[2]
+        return self._push(A.S21_COMP_MOS1, nd + [dp, sp], model=model, inst=inst, name=name)
+
+    def mos0(self, name, d, g, s, b, mos_type=0):
+        nd = [self.node(d), self.node(g), self.node(s), self.node(b)]
+        return self._push(A.S21_COMP_MOS0, nd, model=int(mos_type), name=name)
+
+    def diode(self, name, p, n, model, inst):
+        pv = A.S21_GND if p in ("", None) else self.find(p)
+        nv = A.S21_GND if n in ("", None) else self.find(n)
+        if pv is None or nv is None:  # elab.rs:91-92 unwraps a namespace lookup
+            raise KeyError(f"diode {name}: nodes must already exist")
+        r = self.addv(f"{name}.r") if self._diode_has_rs[model] else pv
+        return self._push(A.S21_COMP_DIODE, [pv, nv, r], model=model, inst=inst, name=name)
+
+    # ---- C view ----
+    def to_c(self):
+        """Returns an ``s21_netlist`` whose arrays are owned by the returned object (keep it alive)."""
+        return CNetlist(self)
+
+
+class CNetlist:
+    """Owns the ctypes arrays behind an ``s21_netlist``."""
+
+    def __init__(self, nl: Netlist):
+        n = len(nl.names)
+        self.kinds = (C.c_int32 * max(n, 1))(*nl.kinds)
+        self._names_b = [s.encode() for s in nl.names]
+        self.names = (C.c_char_p * max(n, 1))(*self._names_b)
+        m = len(nl.comps)
+        self.comps = (A.s21_comp * max(m, 1))()
+        for k, (typ, nodes, model, inst, value, acm, _name) in enumerate(nl.comps):
+            c = self.comps[k]
+            c.type = typ
+            for j in range(6):
+                c.node[j] = nodes[j]
+            c.model, c.inst, c.value, c.acm = model, inst, value, acm
+
+        def arr(lst):
+            a = (A.s21_params * max(len(lst), 1))()
+            for k, p in enumerate(lst):
+                a[k] = p
+            return a
+
+        self.mm, self.mi = arr(nl.mos1_models), arr(nl.mos1_insts)
+        self.dm, self.di = arr(nl.diode_models), arr(nl.diode_insts)
+        self.net = A.s21_netlist(
+            n, self.kinds, self.names, m, self.comps,
+            len(nl.mos1_models), self.mm, len(nl.mos1_insts), self.mi,
+            len(nl.diode_models), self.dm, len(nl.diode_insts), self.di,
+        )
+        self.n_vars = n
+        self.var_names = list(nl.names)
+
+    @property
+    def ptr(self):
+        return C.byref(self.net)

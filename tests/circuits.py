@@ -1,0 +1,51 @@
+"""Circuits of the reference's own tests (spice21/src/tests.rs), as flattened netlists."""
+from spice21_amd.netlist import Netlist
+
+
+def mos1_defaults(n):
+    """add_mos1_defaults (tests.rs:1449-1461): default NMOS, PMOS and instance parameters."""
+    nmos = n.mos1_model()
+    pmos = n.mos1_model(mos_type=1)
+    inst = n.mos1_inst()
+    return nmos, pmos, inst
+
+
+def ro3(kind):
+    """cmos_ro3 / nmos_ro3 / pmos_ro3 (tests.rs:889-912, 966-990, 1009-1033) flattened by hand: module
+    instances x1..x3 map inp/out onto "1"->"2", "2"->"3", "3"->"1"; vss is ground."""
+    n = Netlist(signals=["1", "2", "3", "vdd"])
+    stages = [("1", "2"), ("2", "3"), ("3", "1")]
+    if kind == "mos0":
+        n.v("v1", 1.0, "vdd", "")
+        for x, (a, b) in enumerate(stages):
+            n.mos0(f"x{x+1}.p", b, a, "vdd", "vdd", 1)
+            n.mos0(f"x{x+1}.n", b, a, "", "", 0)
+            n.c(1e-15, b, "")
+        return n
+    nm, pm, ip = mos1_defaults(n)
+    n.v("v1", 1.0, "vdd", "")
+    for x, (a, b) in enumerate(stages):
+        if kind == "cmos":
+            n.mos1(f"x{x+1}.p", b, a, "vdd", "vdd", pm, ip)
+            n.mos1(f"x{x+1}.n", b, a, "", "", nm, ip)
+            n.c(1e-15, b, "")
+        elif kind == "nmos":
+            n.mos1(f"x{x+1}.m", b, a, "", "", nm, ip)
+            n.r(1e-6, b, "vdd")
+            n.c(0.5e-15, b, "vdd")
+        elif kind == "pmos":
+            n.c(1e-16, b, "")
+            n.r(1e-6, b, "")
+            n.mos1(f"x{x+1}.m", b, a, "vdd", "vdd", pm, ip)
+        else:
+            raise ValueError(kind)
+    return n
+
+
+GOLDEN_RUNS = {
+    # fixture -> (circuit kind, tstep, tstop), tests.rs:773-790, 931-947, 992-1008, 1035-1051
+    "test_mos1_cmos_ro_tran.json": ("cmos", 1e-11, 1e-8),
+    "test_mos1_nmos_ro_tran.json": ("nmos", 1e-11, 1e-8),
+    "test_mos1_pmos_ro_tran.json": ("pmos", 1e-11, 1e-8),
+    "test_mos0_cmos_ro_tran.json": ("mos0", 1e-15, 1e-12),
+}
