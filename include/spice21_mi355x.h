@@ -22,6 +22,7 @@ extern "C" {
 
 #define S21_ABI_VERSION 1
 #define S21_GND (-1)
+#define S21_API __attribute__((visibility("default")))
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -106,15 +107,15 @@ typedef struct {
 } s21_ac_options;
 
 /* ---- library / errors --------------------------------------------------------------------- */
-int32_t s21_abi_version(void);
+S21_API int32_t s21_abi_version(void);
 /* Copies the text of the most recent error of the calling thread; returns its full length. */
-size_t s21_last_error(char* buf, size_t len);
-void s21_default_options(s21_options* opts);
+S21_API size_t s21_last_error(char* buf, size_t len);
+S21_API void s21_default_options(s21_options* opts);
 /* Number of time points the reference's fixed-step loop produces, including the OP point at
  * t = 0 (analysis.rs:551-570: t = tstep; while t < tstop { ...; t += tstep }). */
-int64_t s21_tran_point_count(double tstep, double tstop);
+S21_API int64_t s21_tran_point_count(double tstep, double tstop);
 /* Number of AC frequencies (analysis.rs:791-819, including both ends). */
-int64_t s21_ac_point_count(const s21_ac_options* ac);
+S21_API int64_t s21_ac_point_count(const s21_ac_options* ac);
 
 /* ---- circuits and elaboration (circuit.rs, elab.rs) ---------------------------------------- */
 typedef struct s21_ckt s21_ckt;
@@ -122,16 +123,16 @@ typedef struct s21_elab s21_elab;
 
 /* Decode a protobuf `spice21.Circuit` (protos/spice21.proto:128-133) as Ckt::decode /
  * Ckt::from_proto do (circuit.rs:248-256, 281-330). */
-int s21_ckt_decode(const uint8_t* buf, size_t len, s21_ckt** out);
+S21_API int s21_ckt_decode(const uint8_t* buf, size_t len, s21_ckt** out);
 /* Register a level-0 MOS model name, as the reference's test helper does by inserting into
  * `defs.mos0` (tests.rs:1443-1447); proto carries no Mos0 definition. */
-int s21_ckt_add_mos0(s21_ckt* ckt, const char* name, int32_t mos_type);
-void s21_ckt_free(s21_ckt* ckt);
+S21_API int s21_ckt_add_mos0(s21_ckt* ckt, const char* name, int32_t mos_type);
+S21_API void s21_ckt_free(s21_ckt* ckt);
 /* Flatten the hierarchy into variables and components (elab.rs:244-265). */
-int s21_elaborate(const s21_ckt* ckt, s21_elab** out);
+S21_API int s21_elaborate(const s21_ckt* ckt, s21_elab** out);
 /* View of the flattened netlist; valid until s21_elab_free. */
-int s21_elab_netlist(const s21_elab* e, s21_netlist* out);
-void s21_elab_free(s21_elab* e);
+S21_API int s21_elab_netlist(const s21_elab* e, s21_netlist* out);
+S21_API void s21_elab_free(s21_elab* e);
 
 /* ---- batched engine -------------------------------------------------------------------------
  * A batch holds one or more groups.  A group is one topology (a flattened netlist) times n_inst
@@ -161,59 +162,66 @@ enum { S21_FP64 = 0, S21_MIXED = 1 };
  * the groups resident on the current HIP device.  precision: S21_FP64 factors and solves in
  * double; S21_MIXED factors and solves in float while device evaluation, the KCL residual, the
  * Newton update and the convergence test stay in double. */
-int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_options* opts,
+S21_API int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_options* opts,
                      int32_t precision, s21_batch** out);
-void s21_batch_free(s21_batch* b);
+S21_API void s21_batch_free(s21_batch* b);
 
 /* DC operating point of every instance (analysis.rs:383-388 and Solver::solve :169-210). */
-int s21_batch_op(s21_batch* b);
+S21_API int s21_batch_op(s21_batch* b);
 /* Fixed-step Backward-Euler transient with initial-condition forcing (analysis.rs:489-574). */
-int s21_batch_tran(s21_batch* b, const s21_tran_options* t);
+S21_API int s21_batch_tran(s21_batch* b, const s21_tran_options* t);
 /* Operating point, then complex small-signal sweep (analysis.rs:761-832). */
-int s21_batch_ac(s21_batch* b, const s21_ac_options* a);
+S21_API int s21_batch_ac(s21_batch* b, const s21_ac_options* a);
 
-int32_t s21_batch_n_groups(const s21_batch* b);
-int32_t s21_batch_n_inst(const s21_batch* b, int32_t group);
+S21_API int32_t s21_batch_n_groups(const s21_batch* b);
+S21_API int32_t s21_batch_n_inst(const s21_batch* b, int32_t group);
 /* Unknowns of the group after the last analysis (includes IC variables after a transient). */
-int32_t s21_batch_n_vars(const s21_batch* b, int32_t group);
+S21_API int32_t s21_batch_n_vars(const s21_batch* b, int32_t group);
 /* Name of unknown k of the group after the last analysis (stable until the next analysis). */
-const char* s21_batch_var_name(const s21_batch* b, int32_t group, int32_t k);
+S21_API const char* s21_batch_var_name(const s21_batch* b, int32_t group, int32_t k);
 /* Final solution of each instance: out[i * n_vars + k]. */
-int s21_batch_get_x(const s21_batch* b, int32_t group, double* out);
+S21_API int s21_batch_get_x(const s21_batch* b, int32_t group, double* out);
 /* Per-instance S21_* status and Newton iteration totals of the last analysis. */
-int s21_batch_get_status(const s21_batch* b, int32_t group, int32_t* status, int64_t* iters);
+S21_API int s21_batch_get_status(const s21_batch* b, int32_t group, int32_t* status, int64_t* iters);
 /* Number of stored points of the last transient / AC sweep. */
-int64_t s21_batch_n_points(const s21_batch* b);
+S21_API int64_t s21_batch_n_points(const s21_batch* b);
 /* Time (or frequency) axis of the last transient (AC) analysis: out[npts]. */
-int s21_batch_get_axis(const s21_batch* b, double* out);
+S21_API int s21_batch_get_axis(const s21_batch* b, double* out);
 /* Waveform of one instance: out[p * n_vars + k] (transient); AC writes interleaved re/im,
  * out[(p * n_vars + k) * 2 + {0,1}]. */
-int s21_batch_get_wave(const s21_batch* b, int32_t group, int32_t inst, double* out);
+S21_API int s21_batch_get_wave(const s21_batch* b, int32_t group, int32_t inst, double* out);
 /* Device time of the last analysis's Newton kernel launches (HIP events, milliseconds), the
  * number of those launches, and the average duration of the dominant kernel. */
-int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, double* kernel_avg_ms);
+S21_API int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, double* kernel_avg_ms);
 
 /* ---- introspection used by the parity tests ------------------------------------------------ */
+/* Host symbolic phase only (no GPU): pivot order (row/col i2e, each [n_vars + 2 * n_ic]) that the
+ * engine freezes for this netlist -- the reference's first factorisation of the operating-point
+ * Jacobian at x = 0 (sparse21/mod.rs:647-919) -- and the sizes of the resulting structure.  `tran`
+ * may be NULL (operating point) or carry the initial conditions of a transient. */
+S21_API int s21_symbolic(const s21_netlist* net, const s21_options* opts, const s21_tran_options* tran,
+                         int32_t* row_i2e, int32_t* col_i2e, int64_t* nnz_a, int64_t* nnz_lu, int64_t* levels,
+                         int64_t* flops_lu);
 /* Pivot order chosen by the symbolic phase of `group` for its last analysis: internal-to-
  * external row and column maps (AxisMapping.i2e, sparse21/mod.rs:126-162), each [n_vars]. */
-int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e, int32_t* col_i2e);
+S21_API int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e, int32_t* col_i2e);
 /* Sizes of the frozen structure: nnz of A, nnz of L+U (incl. fill), LU levels, contributions. */
-int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_a, int64_t* nnz_lu,
+S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_a, int64_t* nnz_lu,
                         int64_t* levels, int64_t* n_contrib);
 /* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the
  * voltages x[n_vars], in OP mode: returns matrix values per created element, in the order
  * Matrix::make created them (row_out/col_out give their external positions), and the RHS. */
-int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const double* x,
+S21_API int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const double* x,
                        int32_t* row_out, int32_t* col_out, double* val_out, double* rhs_out,
                        int64_t* n_elem);
 
 /* ---- byte ABI mirroring CallableProto::call_bytes (proto.rs:32-145) ------------------------ */
 /* Op -> OpResult, Tran -> TranResult, Ac -> AcResult, all protobuf-encoded.  *out is allocated
  * by the library and released with s21_free_bytes. */
-int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
-int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
-int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
-void s21_free_bytes(uint8_t* p);
+S21_API int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+S21_API int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+S21_API int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
+S21_API void s21_free_bytes(uint8_t* p);
 
 #ifdef __cplusplus
 }

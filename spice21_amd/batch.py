@@ -1,0 +1,180 @@
+"""Batched analyses on the MI355X engine (Python view of the engine C ABI).
+
+A :class:`Batch` holds groups; a group is one flattened netlist (one topology) times ``n_inst`` instances
+that differ only in parameter values given as per-instance overrides.  Each analysis runs all instances
+of all groups in persistent kernel launches on the current GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import check, lib
+
+FP64, MIXED = A.S21_FP64, A.S21_MIXED
+
+
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _field(target, comp_type, name):
+    if isinstance(name, int):
+        return name
+    if target == A.S21_OVR_VALUE:
+        return 0
+    table = {
+        (A.S21_OVR_MODEL, A.S21_COMP_MOS1): A.MOS1_MODEL_FIELDS,
+        (A.S21_OVR_INST, A.S21_COMP_MOS1): A.MOS1_INST_FIELDS,
+        (A.S21_OVR_MODEL, A.S21_COMP_DIODE): A.DIODE_MODEL_FIELDS,
+        (A.S21_OVR_INST, A.S21_COMP_DIODE): A.DIODE_INST_FIELDS,
+    }[(target, comp_type)]
+    return A._field(table, name)
+
+
+def make_tran_options(tstep, tstop, ics=(), var_names=()):
+    """ics: (node name, value) pairs resolved against ``var_names``."""
+    n = len(ics)
+    var = (C.c_int32 * max(n, 1))()
+    val = (C.c_double * max(n, 1))()
+    names_b = [str(nm).encode() for nm, _ in ics]
+    names = (C.c_char_p * max(n, 1))(*names_b)
+    for k, (nm, v) in enumerate(ics):
+        var[k] = list(var_names).index(str(nm))
+        val[k] = v
+    t = A.s21_tran_options(tstep, tstop, n, var, val, names)
+    t._keep = (var, val, names, names_b)
+    return t
+
+
+def symbolic(cnet, opts=None, tran=None):
+    """Host symbolic phase only: (row_i2e, col_i2e, nnz_a, nnz_lu, levels, lu_flops)."""
+    n = cnet.n_vars + (2 * tran.n_ic if tran is not None else 0)
+    ri = np.zeros(max(n, 1), dtype=np.int32)
+    ci = np.zeros(max(n, 1), dtype=np.int32)
+    na, nl, lv, fl = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    check(lib().s21_symbolic(cnet.ptr, C.byref(opts) if opts is not None else None,
+                             C.byref(tran) if tran is not None else None, _i32p(ri), _i32p(ci), C.byref(na),
+                             C.byref(nl), C.byref(lv), C.byref(fl)))
+    return ri[:n], ci[:n], na.value, nl.value, lv.value, fl.value
+
+
+class Group:
+    def __init__(self, cnet, n_inst=1, overrides=()):
+        """overrides: iterable of (comp_index, target, field, values[n_inst]) with target one of
+        'value' | 'model' | 'inst' and field a parameter name (e.g. 'vt0', 'w') or proto field number."""
+        self.cnet = cnet
+        self.n_inst = int(n_inst)
+        tmap = {"value": A.S21_OVR_VALUE, "model": A.S21_OVR_MODEL, "inst": A.S21_OVR_INST}
+        self._vals = []
+        self.over = (A.s21_override * max(len(overrides), 1))()
+        for k, (comp, target, field, vals) in enumerate(overrides):
+            tg = tmap[target] if isinstance(target, str) else int(target)
+            v = np.ascontiguousarray(vals, dtype=np.float64)
+            assert v.shape == (self.n_inst,)
+            self._vals.append(v)
+            ctype = cnet.comps[comp].type
+            self.over[k] = A.s21_override(comp, tg, _field(tg, ctype, field), _dp(v))
+        self.n_over = len(overrides)
+
+
+class Batch:
+    def __init__(self, groups, opts=None, precision=FP64):
+        if isinstance(groups, Group):
+            groups = [groups]
+        self.groups = list(groups)
+        arr = (A.s21_group * len(self.groups))()
+        for k, g in enumerate(self.groups):
+            arr[k] = A.s21_group(C.pointer(g.cnet.net), g.n_inst, g.n_over, g.over)
+        self._arr = arr
+        self.opts = opts
+        self.h = C.c_void_p()
+        check(lib().s21_batch_create(len(self.groups), arr, C.byref(opts) if opts is not None else None,
+                                     int(precision), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().s21_batch_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- analyses ----
+    def op(self):
+        check(lib().s21_batch_op(self.h))
+        return self
+
+    def tran(self, tstep, tstop, ics=()):
+        names = self.groups[0].cnet.var_names
+        t = make_tran_options(tstep, tstop, ics, names)
+        check(lib().s21_batch_tran(self.h, C.byref(t)))
+        return self
+
+    # ---- results ----
+    def n_vars(self, g=0):
+        return lib().s21_batch_n_vars(self.h, g)
+
+    def var_names(self, g=0):
+        return [lib().s21_batch_var_name(self.h, g, k).decode() for k in range(self.n_vars(g))]
+
+    def x(self, g=0):
+        n, ni = self.n_vars(g), self.groups[g].n_inst
+        out = np.zeros((ni, n))
+        check(lib().s21_batch_get_x(self.h, g, _dp(out)))
+        return out
+
+    def status(self, g=0):
+        ni = self.groups[g].n_inst
+        st = np.zeros(ni, dtype=np.int32)
+        it = np.zeros(ni, dtype=np.int64)
+        check(lib().s21_batch_get_status(self.h, g, _i32p(st), it.ctypes.data_as(C.POINTER(C.c_int64))))
+        return st, it
+
+    def axis(self):
+        n = lib().s21_batch_n_points(self.h)
+        out = np.zeros(n)
+        check(lib().s21_batch_get_axis(self.h, _dp(out)))
+        return out
+
+    def wave(self, g=0, inst=0):
+        n = lib().s21_batch_n_points(self.h)
+        out = np.zeros((n, self.n_vars(g)))
+        check(lib().s21_batch_get_wave(self.h, g, inst, _dp(out)))
+        return out
+
+    def timing(self):
+        ms, nl, avg = C.c_double(), C.c_int64(), C.c_double()
+        check(lib().s21_batch_timing(self.h, C.byref(ms), C.byref(nl), C.byref(avg)))
+        return ms.value, nl.value, avg.value
+
+    def pivots(self, g=0):
+        n = self.n_vars(g)
+        ri = np.zeros(n, dtype=np.int32)
+        ci = np.zeros(n, dtype=np.int32)
+        check(lib().s21_batch_pivots(self.h, g, _i32p(ri), _i32p(ci)))
+        return ri, ci
+
+    def structure(self, g=0):
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().s21_batch_structure(self.h, g, *[C.byref(x) for x in v]))
+        return dict(nnz_a=v[0].value, nnz_lu=v[1].value, levels=v[2].value, n_contrib=v[3].value)
+
+    def assemble(self, x, g=0, inst=0, max_elems=1 << 20):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        rows = np.zeros(max_elems, dtype=np.int32)
+        cols = np.zeros(max_elems, dtype=np.int32)
+        vals = np.zeros(max_elems)
+        rhs = np.zeros(max(len(x), 1))
+        ne = C.c_int64()
+        check(lib().s21_batch_assemble(self.h, g, inst, _dp(x), _i32p(rows), _i32p(cols), _dp(vals), _dp(rhs),
+                                       C.byref(ne)))
+        n = ne.value
+        return rows[:n], cols[:n], vals[:n], rhs[: len(x)]

@@ -1,0 +1,345 @@
+// Device physics shared by the HIP kernels and the host symbolic pass.
+//
+// Each model writes its stamps in "entry order": entry e is a fixed (row-terminal, col-terminal)
+// pair of the device, named as if source and drain were not swapped.  A MOS device whose source and
+// drain are swapped at this bias writes reference stamp k into entry sigma(k), which is where the
+// reference's `matps[(dr, ..)]` indexing sends it (comps/mos.rs:833-868, 1085-1097).  Entry order
+// therefore maps statically onto matrix elements, and the per-element order of a device's own
+// contributions is the reference's stamp order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace s21 {
+
+// ---- Mos1 ------------------------------------------------------------------------------------
+// Terminal roles, comps/mos.rs:569-577.
+enum { RD = 0, RG = 1, RS = 2, RB = 3, RDP = 4, RSP = 5 };
+constexpr int MOS1_NG = 22, MOS1_NB = 4, MOS1_NE = 26;
+
+// Entry e -> (row role, col role); entries 22..25 are RHS entries of one role (col = -1).
+__host__ __device__ constexpr int mos1_entry_row(int e) {
+    constexpr int r[MOS1_NE] = {RDP, RSP, RDP, RSP, RDP, RSP, RG, RB, RG, RG, RG, RB, RB,
+                                RB,  RDP, RSP, RD,  RDP, RD,  RS, RSP, RS, RDP, RSP, RG, RB};
+    return r[e];
+}
+__host__ __device__ constexpr int mos1_entry_col(int e) {
+    constexpr int c[MOS1_NE] = {RDP, RSP, RSP, RDP, RG, RG, RG, RB, RB, RDP, RSP, RG, RDP,
+                                RSP, RB,  RB,  RDP, RD, RD, RSP, RS, RS, -1,  -1,  -1, -1};
+    return c[e];
+}
+// Source/drain swap of the entry list (an involution).
+__host__ __device__ constexpr int mos1_sigma(int e) {
+    constexpr int s[MOS1_NE] = {1,  0,  3,  2,  5,  4,  6,  7,  8,  10, 9,  11, 13,
+                                12, 15, 14, 19, 20, 21, 16, 17, 18, 23, 22, 24, 25};
+    return s[e];
+}
+// Entries that only carry charge-integration terms (exactly zero for a device without capacitance
+// and in the operating point), and entries that only carry the terminal resistances.
+__host__ __device__ constexpr bool mos1_entry_tr_only(int e) {
+    return e == 6 || e == 8 || e == 9 || e == 10 || e == 11 || e == 24;
+}
+__host__ __device__ constexpr bool mos1_entry_r_only(int e) { return e >= 16 && e <= 21; }
+
+// Per-device derived parameters used by the DC load (Mos1InternalParams + Mos1Model fields,
+// comps/mos.rs:300-317, 107-138).
+struct Mos1Core {
+    double p, vt0_t, beta, phi_t, gamma, lambda, vtherm, isat_d, isat_s, grd, grs;
+};
+// Gate capacitances (Meyer + overlap): only these reach the stamps.
+struct Mos1Caps {
+    double cox, cgs_ov, cgd_ov, cgb_ov;
+};
+// Committed operating point fields the next transient load reads (comps/mos.rs:757-806).
+struct Mos1State {
+    double vgs, vgd, vgb, cgs, cgd, cgb;
+    double reversed;  // 0 or 1
+};
+
+// Backward-Euler companion (analysis.rs:420-428).
+struct Integ {
+    double g, i, rhs;
+};
+__host__ __device__ inline Integ be_integ(double dq, double dq_dv, double vguess, double dt) {
+    double g = dq_dv / dt;
+    double i = dq / dt;
+    double rhs = i - g * vguess;
+    return Integ{g, i, rhs};
+}
+
+// Mos1::op_stamp (comps/mos.rs:649-893).  Terminal voltages vd, vg, vs, vb; writes entry-order values
+// v[0..25].  TRAN selects the transient branch; with caps == nullptr the device has no capacitance
+// and the integration terms are zero.  guess (optional) receives the new operating point.
+template <bool TRAN>
+__host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* caps, const Mos1State* op, double vd_,
+                                          double vg_, double vs_, double vb_, double gmin, double dt, double* v,
+                                          Mos1State* guess) {
+    const double p = P.p;
+    const bool reversed = p * (vd_ - vs_) < 0.0;
+    const double vd = reversed ? vs_ : vd_;
+    const double vs = reversed ? vd_ : vs_;
+    const double vgs = p * (vg_ - vs);
+    const double vgd = p * (vg_ - vd);
+    const double vds = p * (vd - vs);
+    const double vgb = p * (vg_ - vb_);
+    const double vsb = p * (vs - vb_);
+    const double vdb = p * (vd - vb_);
+    const double von = vsb > 0.0 ? P.vt0_t + P.gamma * (sqrt(P.phi_t + vsb) - sqrt(P.phi_t)) : P.vt0_t;
+    const double vov = vgs - von;
+    const double vdsat = fmax(vov, 0.0);
+    double ids = 0.0, gm = 0.0, gds = 0.0, gmbs = 0.0;
+    if (vov > 0.0) {
+        if (vds >= vov) {
+            ids = P.beta / 2.0 * (vov * vov) * (1.0 + P.lambda * vds);
+            gm = P.beta * vov * (1.0 + P.lambda * vds);
+            gds = P.lambda * P.beta / 2.0 * (vov * vov);
+        } else {
+            ids = P.beta * (vov * vds - (vds * vds) / 2.0) * (1.0 + P.lambda * vds);
+            gm = P.beta * vds * (1.0 + P.lambda * vds);
+            gds = P.beta * ((vov - vds) * (1.0 + P.lambda * vds) + P.lambda * ((vov * vds) - (vds * vds) / 2.0));
+        }
+        gmbs = (P.phi_t + vsb > 0.0) ? gm * P.gamma / 2.0 / sqrt(P.phi_t + vsb) : 0.0;
+    }
+    const double vt = P.vtherm;
+    const double isat_bs = reversed ? P.isat_d : P.isat_s;
+    const double isat_bd = reversed ? P.isat_s : P.isat_d;
+    const double ebs = exp(-vsb / vt);
+    const double ibs = isat_bs * (ebs - 1.0);
+    const double gbs = (isat_bs / vt) * ebs + gmin;
+    const double ibs_rhs = ibs + vsb * gbs;
+    const double ebd = exp(-vdb / vt);
+    const double ibd = isat_bd * (ebd - 1.0);
+    const double gbd = (isat_bd / vt) * ebd + gmin;
+    const double ibd_rhs = ibd + vdb * gbd;
+
+    // Charge integration (only tr.gs -- holding the gate-drain integration, mos.rs:801 -- and tr.gb
+    // reach the stamps; tr.gd stays zero).
+    double trgs_g = 0.0, trgs_rhs = 0.0, trgb_g = 0.0, trgb_rhs = 0.0;
+    const double trgd_g = 0.0, trgd_rhs = 0.0;
+    if (caps) {
+        const double cox = caps->cox;
+        double cgs1, cgd1, cgb1;
+        if (vov <= -P.phi_t) {
+            cgb1 = cox / 2.0;
+            cgs1 = 0.0;
+            cgd1 = 0.0;
+        } else if (vov <= -P.phi_t / 2.0) {
+            cgb1 = -vov * cox / (2.0 * P.phi_t);
+            cgs1 = 0.0;
+            cgd1 = 0.0;
+        } else if (vov <= 0.0) {
+            cgb1 = -vov * cox / (2.0 * P.phi_t);
+            cgs1 = vov * cox / (1.5 * P.phi_t) + cox / 3.0;
+            cgd1 = 0.0;
+        } else if (vdsat <= vds) {
+            cgs1 = cox / 3.0;
+            cgd1 = 0.0;
+            cgb1 = 0.0;
+        } else {
+            const double vddif = 2.0 * vdsat - vds;
+            const double vddif1 = vdsat - vds;
+            const double vddif2 = vddif * vddif;
+            cgd1 = cox * (1.0 - vdsat * vdsat / vddif2) / 3.0;
+            cgs1 = cox * (1.0 - vddif1 * vddif1 / vddif2) / 3.0;
+            cgb1 = 0.0;
+        }
+        if (TRAN) {
+            const bool same = reversed == (op->reversed != 0.0);
+            const double cgd = cgd1 + caps->cgd_ov + (same ? op->cgd : op->cgs);
+            const double cgb = cgb1 + caps->cgb_ov + op->cgb;
+            const double dqgd = same ? (vgd - op->vgd) * cgd : (vgd - op->vgs) * cgd;
+            const Integ a = be_integ(dqgd, cgd, vgd, dt);
+            trgs_g = a.g;
+            trgs_rhs = a.rhs;
+            const Integ b = be_integ((vgb - op->vgb) * cgb, cgb, vgb, dt);
+            trgb_g = b.g;
+            trgb_rhs = b.rhs;
+        }
+        if (guess) {
+            guess->vgs = vgs;
+            guess->vgd = vgd;
+            guess->vgb = vgb;
+            guess->cgs = cgs1;
+            guess->cgd = cgd1;
+            guess->cgb = cgb1;
+            guess->reversed = reversed ? 1.0 : 0.0;
+        }
+    }
+    const double irhs = ids - gm * vgs - gds * vds;
+    const double grd = P.grd, grs = P.grs;
+    double s[MOS1_NE];
+    s[0] = gds + grd + gbd + trgd_g;
+    s[1] = gm + gds + grs + gbs + gmbs + trgs_g;
+    s[2] = -gm - gds - gmbs;
+    s[3] = -gds;
+    s[4] = gm - trgd_g;
+    s[5] = -gm - trgs_g;
+    s[6] = (trgd_g + trgs_g + trgb_g);
+    s[7] = (gbd + gbs + trgb_g);
+    s[8] = -trgb_g;
+    s[9] = -trgd_g;
+    s[10] = -trgs_g;
+    s[11] = -trgb_g;
+    s[12] = -gbd;
+    s[13] = -gbs;
+    s[14] = -gbd + gmbs;
+    s[15] = -gbs - gmbs;
+    s[16] = -grd;
+    s[17] = -grd;
+    s[18] = grd;
+    s[19] = -grs;
+    s[20] = -grs;
+    s[21] = grs;
+    s[22] = p * (-irhs + ibd_rhs + trgd_rhs);
+    s[23] = p * (irhs + ibs_rhs + trgs_rhs);
+    s[24] = -p * (trgs_rhs + trgb_rhs + trgd_rhs);
+    s[25] = -p * (ibd_rhs + ibs_rhs - trgb_rhs);
+#pragma unroll
+    for (int e = 0; e < MOS1_NE; e++) v[e] = reversed ? s[mos1_sigma(e)] : s[e];
+}
+
+// ---- Mos0 (comps/mos.rs:1051-1099) -------------------------------------------------------------
+constexpr int MOS0_NE = 8;  // 6 matrix entries + 2 RHS
+__host__ __device__ constexpr int mos0_entry_row(int e) {
+    constexpr int r[MOS0_NE] = {RD, RS, RD, RS, RD, RS, RD, RS};
+    return r[e];
+}
+__host__ __device__ constexpr int mos0_entry_col(int e) {
+    constexpr int c[MOS0_NE] = {RD, RS, RS, RD, RG, RG, -1, -1};
+    return c[e];
+}
+__host__ __device__ constexpr int mos0_sigma(int e) {
+    constexpr int s[MOS0_NE] = {1, 0, 3, 2, 5, 4, 7, 6};
+    return s[e];
+}
+__host__ __device__ inline void mos0_load(double p, double vd, double vg, double vs, double gmin, double* v) {
+    const double vth = 0.25, beta = 50e-3, lam = 3e-3;
+    const double vds1 = p * (vd - vs);
+    const bool reversed = vds1 < 0.0;
+    const double vgs = reversed ? p * (vg - vd) : p * (vg - vs);
+    const double vds = reversed ? -vds1 : vds1;
+    const double vov = vgs - vth;
+    double ids = 0.0, gm = 0.0, gds = 0.0;
+    if (vov > 0.0) {
+        if (vds >= vov) {
+            ids = beta / 2.0 * (vov * vov) * (1.0 + lam * vds);
+            gm = beta * vov * (1.0 + lam * vds);
+            gds = lam * beta / 2.0 * (vov * vov);
+        } else {
+            ids = beta * (vov * vds - (vds * vds) / 2.0) * (1.0 + lam * vds);
+            gm = beta * vds * (1.0 + lam * vds);
+            gds = beta * ((vov - vds) * (1.0 + lam * vds) + lam * ((vov * vds) - (vds * vds) / 2.0));
+        }
+    }
+    const double irhs = ids - gm * vgs - gds * vds;
+    double s[MOS0_NE];
+    s[0] = gds + gmin;
+    s[1] = (gm + gds + gmin);
+    s[2] = -(gm + gds + gmin);
+    s[3] = -gds - gmin;
+    s[4] = gm;
+    s[5] = -gm;
+    s[6] = -p * irhs;
+    s[7] = p * irhs;
+#pragma unroll
+    for (int e = 0; e < MOS0_NE; e++) v[e] = reversed ? s[mos0_sigma(e)] : s[e];
+}
+
+// ---- Diode (comps/diode.rs:228-354) -------------------------------------------------------------
+enum { DP_ = 0, DN_ = 1, DR_ = 2 };  // terminals p, n, internal r
+constexpr int DIODE_NE = 9;
+__host__ __device__ constexpr int diode_entry_row(int e) {
+    constexpr int r[DIODE_NE] = {DN_, DR_, DN_, DR_, DP_, DP_, DR_, DR_, DN_};
+    return r[e];
+}
+__host__ __device__ constexpr int diode_entry_col(int e) {
+    constexpr int c[DIODE_NE] = {DN_, DN_, DR_, DR_, DP_, DR_, DP_, -1, -1};
+    return c[e];
+}
+struct DiodeP {
+    double isat, vte, vcrit, gspr, bv, has_bv, cz, cz2, dep, f1, f3, tt, vj, m;
+};
+struct DiodeGuess {
+    double vd, charge;
+};
+__host__ __device__ inline double diode_limit(const DiodeP& P, double vnew, double vold) {
+    if (vnew <= P.vcrit || fabs(vnew - vold) <= 2.0 * P.vte) return vnew;
+    if (vold > 0.0) {
+        const double arg = 1.0 + (vnew - vold) / P.vte;
+        if (arg > 0.0) return vold + P.vte * log(arg);
+        return P.vcrit;
+    }
+    return P.vte * log(vnew / P.vte);
+}
+template <bool TRAN>
+__host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn, double gmin, double dt,
+                                           double prev_guess_vd, double op_charge, double* v, DiodeGuess* guess) {
+    double vd = vr - vn;
+    const bool has_bv = P.has_bv != 0.0;
+    if (has_bv && vd < fmin(10.0 * P.vte - P.bv, 0.0)) {
+        const double vtemp = diode_limit(P, -P.bv, P.bv - prev_guess_vd);
+        vd = vtemp - P.bv;
+    } else {
+        vd = diode_limit(P, vd, prev_guess_vd);
+    }
+    double id, gd;
+    if (!has_bv || vd >= -P.bv) {
+        const double e = exp(vd / P.vte);
+        id = P.isat * (e - 1.0) + gmin * vd;
+        gd = P.isat * e / P.vte + gmin;
+    } else {
+        const double e = exp((vd - P.bv) / P.vte);
+        id = -P.isat * e + gmin * vd;
+        gd = P.isat * e / P.vte + gmin;
+    }
+    double qd, cd;
+    if (vd < P.dep) {
+        const double a = 1.0 - vd / P.vj;
+        const double s = -P.m * log(a);
+        qd = P.tt * P.vj * P.cz * (1.0 - a * s) / (1.0 - P.m);
+        cd = P.tt * gd + P.cz * s;
+    } else {
+        qd = P.tt * id + P.cz * P.f1 + P.cz2 * (P.f3 * (vd - P.dep) + P.m / 2.0 / P.vj * (vd * vd - P.dep * P.dep));
+        cd = P.tt + P.cz2 * P.f3 + P.m * vd / P.vj;
+    }
+    if (TRAN) {
+        const Integ c = be_integ(qd - op_charge, cd, vd, dt);
+        id += c.i;
+        gd += c.g;
+    }
+    if (guess) {
+        guess->vd = vd;
+        guess->charge = qd;
+    }
+    const double irhs = id - vd * gd;
+    v[0] = gd;
+    v[1] = -gd;
+    v[2] = -gd;
+    v[3] = gd + P.gspr;
+    v[4] = P.gspr;
+    v[5] = -P.gspr;
+    v[6] = -P.gspr;
+    v[7] = -irhs;
+    v[8] = irhs;
+}
+
+// ---- linear elements (comps/mod.rs:95-346) -------------------------------------------------------
+// Resistor entries: (p,p) (n,n) (p,n) (n,p).  Capacitor: the same four plus RHS p, n.
+// Vsrc: (p,i) (i,p) (n,i) (i,n) plus RHS i.  Isrc: RHS p, n.
+constexpr int R_NE = 4, C_NE = 6, V_NE = 5, I_NE = 2;
+
+// Capacitor transient load: returns the companion (g, rhs) of the BE integration.
+__host__ __device__ inline void cap_load_tran(double c, double vd, double op_q, double dt, double* v) {
+    const double q = c * vd;
+    const Integ a = be_integ(q - op_q, c, vd, dt);
+    v[0] = a.g;
+    v[1] = a.g;
+    v[2] = -a.g;
+    v[3] = -a.g;
+    v[4] = -a.rhs;
+    v[5] = a.rhs;
+}
+
+}  // namespace s21

@@ -1,0 +1,635 @@
+// Host symbolic phase: the frozen structure the GPU Newton kernel runs on.
+//
+//  * topo_elements   -- matrix elements in the order Component::create_matrix_elems makes them
+//                       (comps/mod.rs:127-132, 190-195, 291-298; comps/mos.rs:896-903, 1044-1050;
+//                       comps/diode.rs:248-256), deduplicated as Matrix::make does (sparse21/mod.rs:265).
+//  * topo_pivot      -- the reference's pivot choice on the first Newton iteration's matrix: Markowitz
+//                       diagonal search with relative threshold 1e-3 and the x5 tie cut-off, the
+//                       column-n submatrix search, then the global maximum; the last index is never
+//                       searched (sparse21/mod.rs:647-919).  Restated over permutation arrays instead of
+//                       orthogonal linked lists, visiting candidates in the same order.
+//  * topo_schedule   -- L+U pattern in internal order, IKJ update lists, row levels for the
+//                       factorisation and both sweeps, contribution slots and gather lists.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <unordered_map>
+
+#include "internal.h"
+
+namespace s21 {
+namespace {
+
+inline int64_t key(int r, int c, int n) { return (int64_t)r * (int64_t)(n + 1) + c; }
+
+// Terminal-role -> variable of one component (ground = S21_GND).
+inline int mos1_role_node(const Comp& c, int role) { return c.node[role]; }
+
+}  // namespace
+
+void topo_elements(Topology& t) {
+    std::unordered_map<int64_t, int> made;
+    t.erow.clear();
+    t.ecol.clear();
+    auto make = [&](int r, int c) {
+        if (r < 0 || c < 0) return;
+        int64_t k = key(r, c, t.n);
+        if (made.count(k)) return;
+        made[k] = (int)t.erow.size();
+        t.erow.push_back(r);
+        t.ecol.push_back(c);
+    };
+    for (const Comp& c : t.comps) {
+        const int* nd = c.node;
+        switch (c.type) {
+        case S21_COMP_V:
+            make(nd[0], nd[2]);
+            make(nd[2], nd[0]);
+            make(nd[1], nd[2]);
+            make(nd[2], nd[1]);
+            break;
+        case S21_COMP_C:
+            make(nd[0], nd[0]);
+            make(nd[0], nd[1]);
+            make(nd[1], nd[0]);
+            make(nd[1], nd[1]);
+            break;
+        case S21_COMP_R:
+            make(nd[0], nd[0]);
+            make(nd[0], nd[1]);
+            make(nd[1], nd[0]);
+            make(nd[1], nd[1]);
+            break;
+        case S21_COMP_I: break;
+        case S21_COMP_MOS1: {
+            static const int order[6] = {RG, RD, RS, RB, RDP, RSP};
+            for (int a : order)
+                for (int b : order) make(mos1_role_node(c, a), mos1_role_node(c, b));
+            break;
+        }
+        case S21_COMP_MOS0: {
+            static const int pr[6][2] = {{RD, RD}, {RS, RS}, {RD, RS}, {RS, RD}, {RD, RG}, {RS, RG}};
+            for (auto& q : pr) make(nd[q[0]], nd[q[1]]);
+            break;
+        }
+        case S21_COMP_DIODE:
+            make(nd[0], nd[0]);
+            make(nd[0], nd[2]);
+            make(nd[2], nd[0]);
+            make(nd[2], nd[2]);
+            make(nd[1], nd[2]);
+            make(nd[2], nd[1]);
+            make(nd[1], nd[1]);
+            break;
+        default: fail(S21_INVALID, "Invalid Component");
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host assembly in the reference's order (analysis.rs:153-168): components in order, stamps in order.
+// ------------------------------------------------------------------------------------------------
+void host_assemble_op(const Topology& t, const HostInstance& h, const s21_options& o, const std::vector<double>& x,
+                      std::vector<double>& vals, std::vector<double>& rhs) {
+    std::unordered_map<int64_t, int> elem;
+    for (size_t k = 0; k < t.erow.size(); k++) elem[key(t.erow[k], t.ecol[k], t.n)] = (int)k;
+    vals.assign(t.erow.size(), 0.0);
+    rhs.assign(t.n, 0.0);
+    auto X = [&](int v) { return v < 0 ? 0.0 : x[v]; };
+    auto addm = [&](int r, int c, double v) {
+        if (r < 0 || c < 0) return;
+        vals[elem.at(key(r, c, t.n))] += v;
+    };
+    auto addb = [&](int r, double v) {
+        if (r >= 0) rhs[r] += v;
+    };
+    int im1 = 0, id = 0, il = 0;
+    for (const Comp& c : t.comps) {
+        const int* nd = c.node;
+        switch (c.type) {
+        case S21_COMP_MOS1: {
+            const Mos1Derived& D = h.m1[im1++];
+            double v[MOS1_NE];
+            mos1_load<false>(D.core, nullptr, nullptr, X(nd[RD]), X(nd[RG]), X(nd[RS]), X(nd[RB]), o.gmin, 0.0, v,
+                             nullptr);
+            const bool rev = D.core.p * (X(nd[RD]) - X(nd[RS])) < 0.0;
+            for (int k = 0; k < MOS1_NE; k++) {  // stamp order; stamp k sits in entry sigma(k) when reversed
+                int e = rev ? mos1_sigma(k) : k;
+                int r = nd[mos1_entry_row(e)];
+                if (mos1_entry_col(e) < 0)
+                    addb(r, v[e]);
+                else
+                    addm(r, nd[mos1_entry_col(e)], v[e]);
+            }
+            break;
+        }
+        case S21_COMP_MOS0: {
+            double v[MOS0_NE];
+            const double p = c.model == 1 ? -1.0 : 1.0;
+            mos0_load(p, X(nd[RD]), X(nd[RG]), X(nd[RS]), o.gmin, v);
+            const bool rev = p * (X(nd[RD]) - X(nd[RS])) < 0.0;
+            for (int k = 0; k < MOS0_NE; k++) {
+                int e = rev ? mos0_sigma(k) : k;
+                int r = nd[mos0_entry_row(e)];
+                if (mos0_entry_col(e) < 0)
+                    addb(r, v[e]);
+                else
+                    addm(r, nd[mos0_entry_col(e)], v[e]);
+            }
+            break;
+        }
+        case S21_COMP_DIODE: {
+            double v[DIODE_NE];
+            diode_load<false>(h.dio[id++], X(nd[DR_]), X(nd[DN_]), o.gmin, 0.0, 0.0, 0.0, v, nullptr);
+            for (int e = 0; e < DIODE_NE; e++) {
+                int r = nd[diode_entry_row(e)];
+                if (diode_entry_col(e) < 0)
+                    addb(r, v[e]);
+                else
+                    addm(r, nd[diode_entry_col(e)], v[e]);
+            }
+            break;
+        }
+        case S21_COMP_R: {
+            const double g = h.lin_value[il++];
+            addm(nd[0], nd[0], g);
+            addm(nd[1], nd[1], g);
+            addm(nd[0], nd[1], -g);
+            addm(nd[1], nd[0], -g);
+            break;
+        }
+        case S21_COMP_C: il++; break;  // nothing in the operating point (comps/mod.rs:205-209)
+        case S21_COMP_V: {
+            const double vv = h.lin_value[il++];
+            addm(nd[0], nd[2], 1.0);
+            addm(nd[2], nd[0], 1.0);
+            addm(nd[1], nd[2], -1.0);
+            addm(nd[2], nd[1], -1.0);
+            addb(nd[2], vv);
+            break;
+        }
+        case S21_COMP_I: {
+            const double i = h.lin_value[il++];
+            addb(nd[0], i);
+            addb(nd[1], -i);
+            break;
+        }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pivot search (restated from sparse21/mod.rs:647-919).
+// ------------------------------------------------------------------------------------------------
+namespace {
+struct PivotState {
+    int n;
+    std::vector<int> orow, ocol;     // element -> external (original) row / col
+    std::vector<double> val;
+    std::vector<std::vector<int>> rows, cols;  // external row / col -> element ids
+    std::unordered_map<int64_t, int> at;       // (external row, external col) -> element
+    std::vector<int> r_i2e, r_e2i, c_i2e, c_e2i;
+    std::vector<long> mr, mc;  // Markowitz counts by internal index
+
+    int lookup(int orr, int occ) const {
+        auto it = at.find(key(orr, occ, n));
+        return it == at.end() ? -1 : it->second;
+    }
+    int irow(int e) const { return r_e2i[orow[e]]; }
+    int icol(int e) const { return c_e2i[ocol[e]]; }
+    int add(int orr, int occ, double v) {
+        int e = (int)orow.size();
+        orow.push_back(orr);
+        ocol.push_back(occ);
+        val.push_back(v);
+        rows[orr].push_back(e);
+        cols[occ].push_back(e);
+        at[key(orr, occ, n)] = e;
+        return e;
+    }
+    // Elements of internal column k with internal row >= from, in increasing internal row order.
+    void col_from(int k, int from, std::vector<int>& out) const {
+        out.clear();
+        for (int e : cols[c_i2e[k]])
+            if (irow(e) >= from) out.push_back(e);
+        std::sort(out.begin(), out.end(), [&](int a, int b) { return irow(a) < irow(b); });
+    }
+    void row_from(int k, int from, std::vector<int>& out) const {
+        out.clear();
+        for (int e : rows[r_i2e[k]])
+            if (icol(e) >= from) out.push_back(e);
+        std::sort(out.begin(), out.end(), [&](int a, int b) { return icol(a) < icol(b); });
+    }
+    long mark(int e) const {
+        long a = mr[irow(e)], b = mc[icol(e)];
+        if (a <= 0 || b <= 0) fail(S21_PIVOT, "Pivot Search Fail");
+        return (a - 1) * (b - 1);
+    }
+    int max_of(const std::vector<int>& lst) const {  // first element with the largest |value|
+        int best = lst[0];
+        double bv = fabs(val[best]);
+        for (size_t i = 1; i < lst.size(); i++)
+            if (fabs(val[lst[i]]) > bv) {
+                best = lst[i];
+                bv = fabs(val[lst[i]]);
+            }
+        return best;
+    }
+    int search_diagonal(int n0, std::vector<int>& tmp) const {  // sparse21/mod.rs:735-783
+        int best = -1;
+        long best_mark = -1;  // "usize::MAX"
+        double best_ratio = 0.0;
+        long ties = 0;
+        for (int k = n0; k < n; k++) {
+            int d = lookup(r_i2e[k], c_i2e[k]);
+            if (d < 0) continue;
+            col_from(k, n0, tmp);
+            if (tmp.empty()) continue;
+            int mx = max_of(tmp);
+            double threshold = 1e-3 * fabs(val[mx]) + 0.0;
+            if (fabs(val[d]) < threshold) continue;
+            long m = mark(d);
+            if (best_mark < 0 || m < best_mark) {
+                ties = 0;
+                best = d;
+                best_mark = m;
+                best_ratio = fabs(val[d] / val[mx]);
+            } else if (m == best_mark) {
+                ties += 1;
+                double ratio = fabs(val[d] / val[mx]);
+                if (ratio > best_ratio) {
+                    best = d;
+                    best_mark = m;
+                    best_ratio = ratio;
+                }
+                if (ties >= best_mark * 5) return best;
+            }
+        }
+        return best;
+    }
+    int search_submatrix(int n0, std::vector<int>& tmp) const {  // sparse21/mod.rs:785-834 (column n0 only)
+        col_from(n0, n0, tmp);
+        if (tmp.empty()) return -1;
+        int mx = max_of(tmp);
+        int best = -1;
+        long best_mark = -1;
+        double best_ratio = 0.0;
+        for (int e : tmp) {
+            long m = mark(e);
+            if (best_mark < 0 || m < best_mark) {
+                best = e;
+                best_mark = m;
+                best_ratio = fabs(val[e] / val[mx]);
+            } else if (m == best_mark) {
+                double ratio = fabs(val[e] / val[mx]);
+                if (ratio > best_ratio) {
+                    best = e;
+                    best_mark = m;
+                    best_ratio = ratio;
+                }
+            }
+        }
+        return best;
+    }
+    int find_max(int n0, std::vector<int>& tmp) const {  // sparse21/mod.rs:837-863
+        int best = -1;
+        double bv = 0.0;
+        for (int k = n0; k < n; k++) {
+            col_from(k, n0, tmp);
+            for (int e : tmp)
+                if (fabs(val[e]) > bv) {
+                    best = e;
+                    bv = fabs(val[e]);
+                }
+        }
+        return best;
+    }
+    void swap_rows(int a, int b) {
+        if (a == b) return;
+        std::swap(r_i2e[a], r_i2e[b]);
+        r_e2i[r_i2e[a]] = a;
+        r_e2i[r_i2e[b]] = b;
+        std::swap(mr[a], mr[b]);
+    }
+    void swap_cols(int a, int b) {
+        if (a == b) return;
+        std::swap(c_i2e[a], c_i2e[b]);
+        c_e2i[c_i2e[a]] = a;
+        c_e2i[c_i2e[b]] = b;
+        std::swap(mc[a], mc[b]);
+    }
+    void eliminate(int pivot, int n0, std::vector<int>& L, std::vector<int>& U) {  // sparse21/mod.rs:865-919
+        const double pv = val[pivot];
+        if (pv == 0.0) fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
+        col_from(n0, n0 + 1, L);
+        row_from(n0, n0 + 1, U);
+        for (int l : L) val[l] = val[l] / pv;
+        for (int u : U) {
+            const int uc = ocol[u];
+            for (int l : L) {
+                int t = lookup(orow[l], uc);
+                if (t < 0) {
+                    t = add(orow[l], uc, 0.0);
+                    mr[irow(t)] += 1;  // fill-in while factoring bumps both counts (sparse21/mod.rs:350-353)
+                    mc[icol(t)] += 1;
+                }
+                const double v = val[u] * val[l];
+                val[t] = val[t] - v;
+            }
+            mc[icol(u)] -= 1;
+        }
+        mr[n0] -= 1;
+        mc[n0] -= 1;
+        for (int l : L) mr[irow(l)] -= 1;
+    }
+};
+}  // namespace
+
+// Unpivoted pattern used when the pivot search fails: the residual and convergence test still run (the
+// reference may converge before it ever factors), and the failure is reported when a solve is needed.
+static void topo_identity_pattern(Topology& t) {
+    const int n = t.n;
+    t.row_i2e.resize(n);
+    std::iota(t.row_i2e.begin(), t.row_i2e.end(), 0);
+    t.row_e2i = t.col_i2e = t.col_e2i = t.row_i2e;
+    std::vector<std::vector<std::pair<int, int>>> byrow(n);
+    for (size_t e = 0; e < t.erow.size(); e++) byrow[t.erow[e]].push_back({t.ecol[e], (int)e});
+    t.rp.assign(n + 1, 0);
+    t.ci.clear();
+    t.diag.assign(n, -1);
+    t.elem_pos.assign(t.erow.size(), -1);
+    t.is_a.clear();
+    for (int i = 0; i < n; i++) {
+        std::sort(byrow[i].begin(), byrow[i].end());
+        for (auto& pr : byrow[i]) {
+            int pos = (int)t.ci.size();
+            if (pr.first == i) t.diag[i] = pos;
+            t.ci.push_back(pr.first);
+            t.is_a.push_back(1);
+            t.elem_pos[pr.second] = pos;
+        }
+        t.rp[i + 1] = (int)t.ci.size();
+    }
+    t.nnz = (int)t.ci.size();
+}
+
+void topo_pivot(Topology& t, const std::vector<double>& first_values) {
+    t.sym_status = S21_OK;
+    t.sym_error.clear();
+    try {
+        topo_pivot_search(t, first_values);
+    } catch (Error& e) {
+        t.sym_status = e.code;
+        t.sym_error = e.what();
+        topo_identity_pattern(t);
+    }
+}
+
+void topo_pivot_search(Topology& t, const std::vector<double>& first_values) {
+    const int n = t.n;
+    PivotState s;
+    s.n = n;
+    s.rows.resize(n);
+    s.cols.resize(n);
+    s.r_i2e.resize(n);
+    s.c_i2e.resize(n);
+    std::iota(s.r_i2e.begin(), s.r_i2e.end(), 0);
+    std::iota(s.c_i2e.begin(), s.c_i2e.end(), 0);
+    s.r_e2i = s.r_i2e;
+    s.c_e2i = s.c_i2e;
+    s.mr.assign(n, 0);
+    s.mc.assign(n, 0);
+    for (size_t k = 0; k < t.erow.size(); k++) {
+        s.add(t.erow[k], t.ecol[k], first_values[k]);
+        s.mr[t.erow[k]] += 1;
+        s.mc[t.ecol[k]] += 1;
+    }
+    if (n == 0) fail(S21_ERR, "Assert Gt Failed: 0 <= 0");
+    for (int k = 0; k < n; k++)
+        if (s.rows[k].empty() || s.cols[k].empty()) fail(S21_SINGULAR, "Singular Matrix");
+    std::vector<int> tmp, L, U;
+    for (int n0 = 0; n0 < n - 1; n0++) {
+        int pv = s.search_diagonal(n0, tmp);
+        if (pv < 0) pv = s.search_submatrix(n0, tmp);
+        if (pv < 0) pv = s.find_max(n0, tmp);
+        if (pv < 0) fail(S21_PIVOT, "Pivot Search Fail");
+        s.swap_rows(s.irow(pv), n0);
+        s.swap_cols(s.icol(pv), n0);
+        s.eliminate(pv, n0, L, U);
+    }
+    t.row_i2e = s.r_i2e;
+    t.row_e2i = s.r_e2i;
+    t.col_i2e = s.c_i2e;
+    t.col_e2i = s.c_e2i;
+    if (s.lookup(s.r_i2e[n - 1], s.c_i2e[n - 1]) < 0) fail(S21_SINGULAR, "Singular Matrix");
+
+    // Pattern in internal coordinates; the first erow.size() entries are the matrix elements.
+    const int ne = (int)s.orow.size();
+    std::vector<std::vector<std::pair<int, int>>> byrow(n);  // internal col, element id
+    for (int e = 0; e < ne; e++) byrow[s.irow(e)].push_back({s.icol(e), e});
+    t.rp.assign(n + 1, 0);
+    t.ci.clear();
+    t.diag.assign(n, -1);
+    t.elem_pos.assign(t.erow.size(), -1);
+    t.is_a.clear();
+    for (int i = 0; i < n; i++) {
+        std::sort(byrow[i].begin(), byrow[i].end());
+        for (auto& pr : byrow[i]) {
+            int pos = (int)t.ci.size();
+            if (pr.first == i) t.diag[i] = pos;
+            t.ci.push_back(pr.first);
+            bool is_elem = pr.second < (int)t.erow.size();
+            t.is_a.push_back(is_elem ? 1 : 0);
+            if (is_elem) t.elem_pos[pr.second] = pos;
+        }
+        t.rp[i + 1] = (int)t.ci.size();
+        if (t.diag[i] < 0) fail(S21_SINGULAR, "Singular Matrix");
+    }
+    t.nnz = (int)t.ci.size();
+}
+
+void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
+    const int n = t.n;
+    // IKJ update lists: for L entry (i,k), subtract l_ik * u_kj from (i,j) for each U entry (k,j).
+    // upd_ptr is indexed by LU position; only L positions own updates.
+    std::vector<int> posmap(n, -1);
+    t.upd_ptr.assign(t.nnz + 1, 0);
+    t.upd_src.clear();
+    t.upd_dst.clear();
+    t.flops_lu = 0;
+    for (int i = 0; i < n; i++) {
+        for (int q = t.rp[i]; q < t.rp[i + 1]; q++) posmap[t.ci[q]] = q;
+        for (int q = t.rp[i]; q < t.rp[i + 1]; q++) {
+            t.upd_ptr[q] = (int)t.upd_src.size();
+            if (q >= t.diag[i] || t.diag[i] < 0) continue;
+            const int k = t.ci[q];
+            for (int u = t.diag[k] + 1; u < t.rp[k + 1]; u++) {
+                int dst = posmap[t.ci[u]];
+                if (dst < 0) fail(S21_ERR, "internal: fill pattern not closed");
+                t.upd_src.push_back(u);
+                t.upd_dst.push_back(dst);
+            }
+            t.flops_lu += 1 + 2 * (t.rp[k + 1] - t.diag[k] - 1);
+        }
+        for (int q = t.rp[i]; q < t.rp[i + 1]; q++) posmap[t.ci[q]] = -1;
+    }
+    t.upd_ptr[t.nnz] = (int)t.upd_src.size();
+    // Row levels: factorisation / forward sweep depend on L rows, backward on U rows.
+    std::vector<int> lf(n, 0), lb(n, 0);
+    int maxf = 0, maxb = 0;
+    for (int i = 0; i < n; i++) {
+        int l = 0;
+        for (int q = t.rp[i]; q < t.diag[i]; q++) l = std::max(l, lf[t.ci[q]] + 1);
+        lf[i] = l;
+        maxf = std::max(maxf, l);
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        int l = 0;
+        for (int q = t.diag[i] + 1; q < t.rp[i + 1]; q++) l = std::max(l, lb[t.ci[q]] + 1);
+        lb[i] = l;
+        maxb = std::max(maxb, l);
+    }
+    auto bucket = [&](const std::vector<int>& lev, int mx, std::vector<int>& ptr, std::vector<int>& rows) {
+        ptr.assign(mx + 2, 0);
+        for (int i = 0; i < n; i++) ptr[lev[i] + 1]++;
+        for (int l = 0; l <= mx; l++) ptr[l + 1] += ptr[l];
+        rows.assign(n, 0);
+        std::vector<int> fillp(ptr.begin(), ptr.end() - 1);
+        for (int i = 0; i < n; i++) rows[fillp[lev[i]]++] = i;
+    };
+    bucket(lf, maxf, t.lvl_ptr, t.lvl_rows);
+    bucket(lb, maxb, t.blvl_ptr, t.blvl_rows);
+
+    // Residual rows over A entries (internal row i = external row row_i2e[i]).
+    t.res_ptr.assign(n + 1, 0);
+    t.res_pos.clear();
+    t.res_xcol.clear();
+    for (int i = 0; i < n; i++) {
+        for (int q = t.rp[i]; q < t.rp[i + 1]; q++)
+            if (t.is_a[q]) {
+                t.res_pos.push_back(q);
+                t.res_xcol.push_back(t.col_i2e[t.ci[q]]);
+            }
+        t.res_ptr[i + 1] = (int)t.res_pos.size();
+    }
+
+    // Contributions: every live (component, entry) gets one slot, in component then entry order, which
+    // is the order the reference adds stamps into an element.
+    std::unordered_map<int64_t, int> elem;
+    for (size_t k = 0; k < t.erow.size(); k++) elem[key(t.erow[k], t.ecol[k], t.n)] = (int)k;
+    std::vector<std::vector<int>> slot(t.nnz + n);
+    t.m1_comp.clear(); t.m0_comp.clear(); t.d_comp.clear(); t.lin_comp.clear();
+    t.m1_nodes.clear(); t.m1_mask.clear(); t.m1_base.clear(); t.m1_caps.clear();
+    t.m0_nodes.clear(); t.m0_mask.clear(); t.m0_base.clear(); t.m0_p.clear();
+    t.d_nodes.clear(); t.d_mask.clear(); t.d_base.clear();
+    t.lin_type.clear(); t.lin_nodes.clear(); t.lin_mask.clear(); t.lin_base.clear();
+    t.contrib_comp.clear();
+    int nc = 0;
+    auto target = [&](int r, int c) -> int {  // slot of a matrix entry or RHS entry, -1 if grounded
+        if (r < 0) return -1;
+        if (c == -2) return t.nnz + r;  // RHS
+        if (c < 0) return -1;
+        return t.elem_pos[elem.at(key(r, c, t.n))];
+    };
+    int im1 = 0;
+    for (size_t ci = 0; ci < t.comps.size(); ci++) {
+        const Comp& c = t.comps[ci];
+        const int* nd = c.node;
+        uint32_t mask = 0;
+        int base = nc;
+        auto live = [&](int e, int r, int col) {
+            int s = target(r, col);
+            if (s < 0) return;
+            mask |= 1u << e;
+            slot[s].push_back(nc);
+            t.contrib_comp.push_back((int)ci);
+            nc++;
+        };
+        switch (c.type) {
+        case S21_COMP_MOS1: {
+            const bool caps = m1_caps[im1++] != 0;
+            const bool rdrs = nd[RDP] != nd[RD] || nd[RSP] != nd[RS];
+            for (int e = 0; e < MOS1_NE; e++) {
+                if (mos1_entry_tr_only(e) && !(caps && t.tran)) continue;  // exactly zero
+                if (mos1_entry_r_only(e) && !rdrs) continue;             // grd = grs = 0
+                int r = nd[mos1_entry_row(e)];
+                int col = mos1_entry_col(e) < 0 ? -2 : nd[mos1_entry_col(e)];
+                if (mos1_entry_col(e) >= 0 && col < 0) continue;
+                live(e, r, col);
+            }
+            t.m1_comp.push_back((int)ci);
+            for (int k = 0; k < 6; k++) t.m1_nodes.push_back(nd[k]);
+            t.m1_mask.push_back(mask);
+            t.m1_base.push_back(base);
+            t.m1_caps.push_back(caps && t.tran ? 1 : 0);
+            break;
+        }
+        case S21_COMP_MOS0:
+            for (int e = 0; e < MOS0_NE; e++) {
+                int r = nd[mos0_entry_row(e)];
+                int col = mos0_entry_col(e) < 0 ? -2 : nd[mos0_entry_col(e)];
+                if (mos0_entry_col(e) >= 0 && col < 0) continue;
+                live(e, r, col);
+            }
+            t.m0_comp.push_back((int)ci);
+            for (int k = 0; k < 4; k++) t.m0_nodes.push_back(nd[k]);
+            t.m0_mask.push_back(mask);
+            t.m0_base.push_back(base);
+            t.m0_p.push_back(c.model == 1 ? -1.0 : 1.0);
+            break;
+        case S21_COMP_DIODE:
+            for (int e = 0; e < DIODE_NE; e++) {
+                int r = nd[diode_entry_row(e)];
+                int col = diode_entry_col(e) < 0 ? -2 : nd[diode_entry_col(e)];
+                if (diode_entry_col(e) >= 0 && col < 0) continue;
+                live(e, r, col);
+            }
+            t.d_comp.push_back((int)ci);
+            for (int k = 0; k < 3; k++) t.d_nodes.push_back(nd[k]);
+            t.d_mask.push_back(mask);
+            t.d_base.push_back(base);
+            break;
+        default: {
+            const int p = nd[0], q = nd[1], iv = nd[2];
+            if (c.type == S21_COMP_R) {
+                live(0, p, p < 0 ? -1 : p);
+                live(1, q, q < 0 ? -1 : q);
+                live(2, p, q < 0 ? -1 : q);
+                live(3, q, p < 0 ? -1 : p);
+            } else if (c.type == S21_COMP_C) {
+                if (t.tran) {
+                    live(0, p, p < 0 ? -1 : p);
+                    live(1, q, q < 0 ? -1 : q);
+                    live(2, p, q < 0 ? -1 : q);
+                    live(3, q, p < 0 ? -1 : p);
+                    live(4, p, -2);
+                    live(5, q, -2);
+                }
+            } else if (c.type == S21_COMP_V) {
+                live(0, p, iv);
+                live(1, iv, p < 0 ? -1 : p);
+                live(2, q, iv);
+                live(3, iv, q < 0 ? -1 : q);
+                live(4, iv, -2);
+            } else if (c.type == S21_COMP_I) {
+                live(0, p, -2);
+                live(1, q, -2);
+            }
+            t.lin_comp.push_back((int)ci);
+            t.lin_type.push_back(c.type);
+            for (int k = 0; k < 3; k++) t.lin_nodes.push_back(nd[k]);
+            t.lin_mask.push_back(mask);
+            t.lin_base.push_back(base);
+        }
+        }
+    }
+    t.n_contrib = nc;
+    t.g_ptr.assign(t.nnz + n + 1, 0);
+    t.g_idx.clear();
+    for (int s = 0; s < t.nnz + n; s++) {
+        for (int c : slot[s]) t.g_idx.push_back(c);
+        t.g_ptr[s + 1] = (int)t.g_idx.size();
+    }
+}
+
+}  // namespace s21

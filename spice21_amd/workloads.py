@@ -1,0 +1,110 @@
+"""Synthetic benchmark circuits (flattened netlists plus per-instance parameter overrides).
+
+Random draws use splitmix64 -> 53-bit uniform -> Box-Muller, seeded per workload, so every consumer
+(tests, bench, CPU baseline) sees the same instances.
+
+  ring5()      5-stage CMOS ring oscillator, 10 fF per stage, VDD = 1 V (single instance)
+  chain_mc()   N instances of a 64-stage CMOS inverter chain, per-device VTO ~ N(base, 15 mV) and
+               W ~ N(base, 3 %), input tied to 0.5 V
+  ring_batch() ring oscillators of uneven size: k = 3 + 2 U{0..49} stages, C ~ U[5, 50] fF, VDD ~ U[0.8, 1.2]
+"""
+import numpy as np
+
+from .netlist import Netlist
+
+MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed, n):
+    """n successive outputs of splitmix64 started at `seed`."""
+    with np.errstate(over="ignore"):
+        state = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = state
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n):
+    """n uniforms in [0, 1) with 53-bit resolution."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, n):
+    """n standard normals by Box-Muller over pairs of uniforms."""
+    m = (n + 1) // 2
+    u = uniform(seed, 2 * m)
+    u1 = 1.0 - u[0::2]  # (0, 1]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    return z[:n]
+
+
+# Mos1 parameters of the benchmark circuits
+NMOS = dict(vt0=0.4, kp=100e-6)
+PMOS = dict(mos_type=1, vt0=-0.4, kp=50e-6)
+NINST = dict(w=1e-6, l=0.1e-6)
+PINST = dict(w=2e-6, l=0.1e-6)
+
+
+def _models(n, pmos=None):
+    nm = n.mos1_model(**NMOS)
+    pm = n.mos1_model(**(pmos or PMOS))
+    ni = n.mos1_inst(**NINST)
+    pi = n.mos1_inst(**PINST)
+    return nm, pm, ni, pi
+
+
+def ring5(stages=5, cap=10e-15, vdd=1.0, pmos=None):
+    """CMOS ring oscillator: stage k drives node str(k+1) from node str(k); the last stage closes the ring
+    on node "1".  Returns the netlist; the transient kicks node "1" through an initial condition."""
+    n = Netlist(signals=[str(k + 1) for k in range(stages)] + ["vdd"])
+    nm, pm, ni, pi = _models(n, pmos)
+    n.v("vdd", vdd, "vdd", "")
+    for k in range(stages):
+        a, b = str(k + 1), str((k + 1) % stages + 1)
+        n.mos1(f"x{k}.p", b, a, "vdd", "vdd", pm, pi)
+        n.mos1(f"x{k}.n", b, a, "", "", nm, ni)
+        n.c(cap, b, "", name=f"x{k}.c")
+    return n
+
+
+def chain(stages=64, vin=0.5, vdd=1.0):
+    """Inverter chain: input Vsrc on "in", stage k drives "s{k}" from "s{k-1}" ("s0" = "in")."""
+    n = Netlist()
+    nm, pm, ni, pi = _models(n)
+    n.v("vin", vin, "in", "")
+    n.v("vdd", vdd, "vdd", "")
+    mos = []
+    for k in range(1, stages + 1):
+        a = "in" if k == 1 else f"s{k-1}"
+        b = f"s{k}"
+        mos.append((n.mos1(f"p{k}", b, a, "vdd", "vdd", pm, pi), "p"))
+        mos.append((n.mos1(f"n{k}", b, a, "", "", nm, ni), "n"))
+    return n, mos
+
+
+def chain_mc(n_inst=4096, stages=64, seed=0xC2, sigma_vt=0.015, sigma_w=0.03):
+    """Monte-Carlo inverter chains: returns (netlist, overrides) where overrides set every device's own
+    vt0 (model field) and w (instance field) per instance."""
+    n, mos = chain(stages)
+    nd = len(mos)
+    z = normal(seed, 2 * nd * n_inst).reshape(n_inst, nd, 2)
+    over = []
+    for j, (ci, kind) in enumerate(mos):
+        base_vt = NMOS["vt0"] if kind == "n" else PMOS["vt0"]
+        base_w = NINST["w"] if kind == "n" else PINST["w"]
+        over.append((ci, "model", "vt0", base_vt + sigma_vt * z[:, j, 0]))
+        over.append((ci, "inst", "w", base_w * (1.0 + sigma_w * z[:, j, 1])))
+    return n, over
+
+
+def ring_batch(n_inst=1024, seed=0xC3):
+    """Uneven ring oscillators: per instance (stages, cap, vdd)."""
+    u = uniform(seed, 3 * n_inst).reshape(n_inst, 3)
+    stages = 3 + 2 * np.minimum((u[:, 0] * 50).astype(int), 49)
+    cap = 5e-15 + u[:, 1] * 45e-15
+    vdd = 0.8 + u[:, 2] * 0.4
+    return stages, cap, vdd

@@ -1,0 +1,123 @@
+"""MI355X engine against the oracle (and the reference's golden waveforms).  Needs the GPU.
+
+Tolerances: device evaluation uses the GPU's exp/sqrt (within an ulp or so of glibc's), so assembled
+values agree to 1e-12 relative; FP64 Newton solutions agree with the oracle to 1e-9 relative (the
+north-star bar for the plain FP64 path); mixed precision agrees with FP64 to 1e-6 V."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from spice21_amd import workloads as W
+from spice21_amd.batch import FP64, MIXED, Batch, Group
+from tests.circuits import GOLDEN_RUNS, materialize, ro3
+from tests.test_symbolic import random_circuit
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-9))) if len(a) else 0.0
+
+
+@pytest.mark.parametrize("which", ["cmos", "mos0", "chain", "rand3", "rand8"])
+def test_assembly_matches_oracle(which):
+    if which == "chain":
+        nl = W.chain(64)[0]
+    elif which.startswith("rand"):
+        nl = random_circuit(int(which[4:]))
+    else:
+        nl = ro3(which)
+    cn = nl.to_c()
+    b = Batch(Group(cn))
+    rng = np.random.default_rng(1)
+    for trial in range(3):
+        x = rng.uniform(-0.2, 1.2, size=cn.n_vars)
+        log("assemble", which, trial)
+        r, c, v, rhs = b.assemble(x)
+        orr, occ, ov, orhs = O.assemble_op(cn, x)
+        assert list(r) == list(orr) and list(c) == list(occ)
+        scale = np.maximum(np.abs(ov), 1e-15)
+        assert np.max(np.abs(v - ov) / scale) < 1e-12, (which, trial)
+        assert np.max(np.abs(rhs - orhs) / np.maximum(np.abs(orhs), 1e-15)) < 1e-12
+
+
+@pytest.mark.parametrize("which", ["ring5", "chain", "cmos", "mos0", "rand1", "rand5", "rand11"])
+def test_dcop_matches_oracle(which):
+    if which == "ring5":
+        nl = W.ring5()
+    elif which == "chain":
+        nl = W.chain(64)[0]
+    elif which.startswith("rand"):
+        nl = random_circuit(int(which[4:]))
+    else:
+        nl = ro3(which)
+    cn = nl.to_c()
+    try:
+        ox, oit, _ = O.dcop(cn)
+    except O.OracleError:
+        pytest.skip("reference does not converge on this draw")
+    log("dcop", which)
+    b = Batch(Group(cn)).op()
+    st, it = b.status()
+    assert st[0] == 0
+    x = b.x()[0]
+    assert rel_err(x, ox) < 1e-9, (which, x, ox)
+
+
+@pytest.mark.parametrize("fixture", sorted(GOLDEN_RUNS))
+def test_tran_matches_golden(fixture):
+    kind, tstep, tstop = GOLDEN_RUNS[fixture]
+    g = json.load(open(os.path.join(GOLDEN, fixture)))
+    cn = ro3(kind).to_c()
+    log("tran", fixture)
+    b = Batch(Group(cn)).tran(tstep, tstop, [("1", 0.0)])
+    st, it = b.status()
+    assert st[0] == 0
+    t = b.axis()
+    X = b.wave()
+    names = b.var_names()
+    assert np.array_equal(t, np.array(g["time"]))
+    worst = 0.0
+    for k, nm in enumerate(names):
+        worst = max(worst, float(np.max(np.abs(X[:, k] - np.array(g[nm])))))
+    log("  max |dev| vs golden", worst)
+    assert worst <= 1e-6  # the reference's own tolerance (assert.rs:124-143)
+
+
+def test_chain_mc_batch_matches_oracle_per_instance():
+    n_inst = 96
+    nl, over = W.chain_mc(n_inst=n_inst)
+    cn = nl.to_c()
+    log("chain_mc fp64", n_inst)
+    b = Batch(Group(cn, n_inst, over)).op()
+    st, it = b.status()
+    assert np.all(st == 0)
+    X = b.x()
+    for i in [0, 1, 17, 63, 95]:
+        ox, _, _ = O.dcop(materialize(nl, over, i).to_c())
+        assert rel_err(X[i], ox) < 1e-9, i
+    log("chain_mc mixed", n_inst)
+    bm = Batch(Group(cn, n_inst, over), precision=MIXED).op()
+    stm, itm = bm.status()
+    assert np.all(stm == 0)
+    Xm = bm.x()
+    nv = [k for k, kind in enumerate(nl.kinds) if kind == 0]
+    assert np.max(np.abs(Xm[:, nv] - X[:, nv])) <= 1e-6
+
+
+def test_mixed_tran_close_to_fp64():
+    cn = ro3("cmos").to_c()
+    a = Batch(Group(cn)).tran(1e-11, 1e-8, [("1", 0.0)]).wave()
+    bm = Batch(Group(cn), precision=MIXED).tran(1e-11, 1e-8, [("1", 0.0)])
+    assert bm.status()[0][0] == 0
+    m = bm.wave()
+    log("mixed vs fp64 tran", float(np.max(np.abs(m[:, :4] - a[:, :4]))))
+    assert np.max(np.abs(m[:, :4] - a[:, :4])) <= 1e-6
