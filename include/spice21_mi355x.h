@@ -208,6 +208,10 @@ S21_API int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e
 /* Sizes of the frozen structure: nnz of A, nnz of L+U (incl. fill), LU levels, contributions. */
 S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_a, int64_t* nnz_lu,
                         int64_t* levels, int64_t* n_contrib);
+/* Per-phase device cycle counters (s_memtime) of the Newton kernel, for profiling: while on, each
+ * instance accumulates [eval, gather, residual, factor, sweeps, update, commit, total] cycles. */
+S21_API int s21_batch_set_profiling(s21_batch* b, int32_t on);
+S21_API int s21_batch_profile(const s21_batch* b, int32_t group, int64_t* out /* [n_inst][8] */);
 /* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the
  * voltages x[n_vars], in OP mode: returns matrix values per created element, in the order
  * Matrix::make created them (row_out/col_out give their external positions), and the RHS. */

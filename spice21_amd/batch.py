@@ -155,6 +155,18 @@ class Batch:
         check(lib().s21_batch_timing(self.h, C.byref(ms), C.byref(nl), C.byref(avg)))
         return ms.value, nl.value, avg.value
 
+    def set_profiling(self, on=True):
+        check(lib().s21_batch_set_profiling(self.h, 1 if on else 0))
+        return self
+
+    PHASES = ("eval", "gather", "residual", "factor", "sweeps", "update", "commit", "total")
+
+    def profile(self, g=0):
+        """Per-instance device cycles of the last analysis by phase: array [n_inst, 8]."""
+        out = np.zeros((self.groups[g].n_inst, 8), dtype=np.int64)
+        check(lib().s21_batch_profile(self.h, g, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
     def pivots(self, g=0):
         n = self.n_vars(g)
         ri = np.zeros(n, dtype=np.int32)

@@ -36,10 +36,15 @@ namespace s21 {
 // Device-side descriptors
 // ============================================================================================
 struct TopoGPU {
-    int n, nnz, n_contrib;
+    int n, nnz, n_contrib, n_part;
     int n_m1, n_m0, n_d, n_lin;
-    int n_lvl, n_blvl;
+    int n_gt, n_gc, n_rt, n_rc, n_clvl, n_blvl;
     int sym_status;
+    int blob_vec;       // 16-byte vectors of the record blob copied into LDS at launch
+    const uint4* blob;  // the blob in global memory
+    // offsets (uint16 units, 8-byte aligned) of the record tables inside the blob
+    int o_gt, o_gidx, o_gc, o_rt, o_re, o_rc, o_ri2e, o_ce2i, o_diag, o_clvl, o_fch, o_fl, o_fu, o_blvl, o_bro, o_bu;
+    // per-device tables (global; Mos1 tables are read once into registers)
     const int *m1_nodes, *m1_base;
     const uint32_t* m1_mask;
     const int* m1_caps;
@@ -50,12 +55,6 @@ struct TopoGPU {
     const uint32_t* d_mask;
     const int *lin_type, *lin_nodes, *lin_base;
     const uint32_t* lin_mask;
-    const int *g_ptr, *g_idx;
-    const int *res_ptr, *res_pos, *res_xcol;
-    const int *row_i2e, *col_e2i;
-    const int *rp, *ci, *diag;
-    const int *upd_ptr, *upd_src, *upd_dst;
-    const int *lvl_ptr, *lvl_rows, *blvl_ptr, *blvl_rows;
 };
 
 struct GroupGPU {
@@ -74,6 +73,7 @@ struct GroupGPU {
     long long* iters;       // [n_inst]
     double* wave;           // [n_inst][npts][n] (transient) or nullptr
     double* dbg;            // assembly dump [nnz + n] (assemble mode)
+    long long* prof;        // [n_inst][8] phase cycle counters (profiling mode) or nullptr
 };
 
 enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2 };
@@ -84,10 +84,33 @@ struct Args {
     double tstep, tstop;
     long long npts;
     int dbg_inst;
+    int prof;
+    int waves;          // instances (wavefronts) per workgroup
+    int inst_bytes;     // LDS bytes per instance
 };
 
+// Address spaces: pointers read out of descriptors carry no address space, so they are re-qualified in
+// device code to keep LDS accesses on ds_* and global accesses on global_* instructions.
+#define LDS __attribute__((address_space(3)))
+#define GLB __attribute__((address_space(1)))
+template <typename T>
+__device__ inline GLB T* glb(T* p) {
+    return (GLB T*)p;
+}
+
 constexpr int WAVE = 64;
+constexpr int MAX_WAVES = 4;  // wavefronts (instances) per workgroup; the kernel's launch bound
 constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 3;
+constexpr int NPROF = 8;  // eval, gather, residual, factor (+forward sweep), backward sweep, update, commit, total
+using u16 = unsigned short;
+
+// Orders LDS accesses between the lanes of one wavefront (a wavefront executes its LDS operations in
+// order; the fences keep the compiler from moving accesses across this point).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ inline double wave_max(double v) {
 #pragma unroll
@@ -99,35 +122,57 @@ __device__ inline double wave_max(double v) {
 }
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
-template <int NE>
-__device__ inline void write_entries(const double* v, uint32_t mask, int base, double* contrib) {
-#pragma unroll
-    for (int e = 0; e < NE; e++)
-        if ((mask >> e) & 1u) contrib[base + __popc(mask & ((1u << e) - 1u))] = v[e];
-}
+// Writes a stamp into its contribution slot when the entry is live for this device.
+struct Slot {
+    LDS double* contrib;
+    uint32_t mask;
+    int base;
+    __device__ void operator()(int e, double v) const {
+        if ((mask >> e) & 1u) contrib[base + __popc(mask & ((1u << e) - 1u))] = v;
+    }
+};
 
 // Per-lane register copy of the lane's Mos1 devices (DPL > 0).
 template <int DPL>
 struct Mos1Regs {
     Mos1Core P[DPL];
-    int nd[DPL][4];
+    int nd01[DPL], nd23[DPL];  // node pairs packed as 16-bit halves (+1, so ground is 0)
     uint32_t mask[DPL];
-    int base[DPL];
-    int caps[DPL];
+    int base[DPL];  // contribution base; bit 30 flags a device with gate capacitance
 };
+__device__ inline int lo16(int v) { return (v & 0xFFFF) - 1; }
+__device__ inline int hi16(int v) { return ((v >> 16) & 0xFFFF) - 1; }
+
+// LDS views of the record tables.
+struct Idx {
+    LDS const uint64_t *gt, *gc, *rt, *rc, *fch, *fl, *bro;  // four 16-bit fields
+    LDS const uint32_t *re, *fu, *bu;                         // two 16-bit fields
+    LDS const u16 *gidx, *ri2e, *ce2i, *diag, *clvl, *blvl;
+};
+struct R4 {
+    int x, y, z, w;
+};
+__device__ inline R4 r4(uint64_t v) {
+    return R4{(int)(v & 0xFFFF), (int)((v >> 16) & 0xFFFF), (int)((v >> 32) & 0xFFFF), (int)(v >> 48)};
+}
+struct R2 {
+    int x, y;
+};
+__device__ inline R2 r2(uint32_t v) { return R2{(int)(v & 0xFFFF), (int)(v >> 16)}; }
 
 struct Ctx {
     const GroupGPU* G;
-    const TopoGPU* T;
+    TopoGPU T;
+    Idx I;
     int inst, lane;
-    double *x, *rhs, *c, *contrib, *a;
-    float* lu32;
+    LDS double *x, *rhs, *c, *contrib, *a, *part;
+    LDS float* lu32;
 };
 
-__device__ inline double xv(const double* x, int v) { return v < 0 ? 0.0 : x[v]; }
+__device__ inline double xv(LDS const double* x, int v) { return v < 0 ? 0.0 : x[v]; }
 
 __device__ inline Mos1Core load_mos1_core(const GroupGPU& G, int n_m1, int inst, int d) {
-    const double* p = G.m1_core;
+    GLB const double* p = glb(G.m1_core);
     const size_t s = (size_t)G.n_inst * n_m1, o = (size_t)inst * n_m1 + d;
     Mos1Core P;
     P.p = p[0 * s + o];
@@ -144,20 +189,21 @@ __device__ inline Mos1Core load_mos1_core(const GroupGPU& G, int n_m1, int inst,
     return P;
 }
 
-// Mos1 with gate capacitance: parameters and state come from global memory.
+// Mos1 with gate capacitance: capacitances and operating-point state come from global memory.
 template <bool TRAN>
 __device__ inline void eval_mos1_caps(const Ctx& C, const Mos1Core& P, int d, double vd, double vg, double vs,
-                                      double vb, double gmin, double dt, double* v) {
+                                            double vb, double gmin, double dt, Slot put) {
     const GroupGPU& G = *C.G;
-    const int n_m1 = C.T->n_m1;
+    const int n_m1 = C.T.n_m1;
     const size_t s = (size_t)G.n_inst * n_m1, o = (size_t)C.inst * n_m1 + d;
     Mos1Caps K;
-    K.cox = G.m1_caps[0 * s + o];
-    K.cgs_ov = G.m1_caps[1 * s + o];
-    K.cgd_ov = G.m1_caps[2 * s + o];
-    K.cgb_ov = G.m1_caps[3 * s + o];
+    GLB const double* kc = glb(G.m1_caps);
+    K.cox = kc[0 * s + o];
+    K.cgs_ov = kc[1 * s + o];
+    K.cgd_ov = kc[2 * s + o];
+    K.cgb_ov = kc[3 * s + o];
     Mos1State op, gs;
-    double* st = G.m1_state;
+    GLB double* st = glb(G.m1_state);
     op.vgs = st[0 * s + o];
     op.vgd = st[1 * s + o];
     op.vgb = st[2 * s + o];
@@ -165,7 +211,7 @@ __device__ inline void eval_mos1_caps(const Ctx& C, const Mos1Core& P, int d, do
     op.cgd = st[4 * s + o];
     op.cgb = st[5 * s + o];
     op.reversed = st[6 * s + o];
-    mos1_load<TRAN>(P, &K, &op, vd, vg, vs, vb, gmin, dt, v, &gs);
+    mos1_load<TRAN>(P, &K, &op, vd, vg, vs, vb, gmin, dt, put, &gs);
     st[7 * s + o] = gs.vgs;
     st[8 * s + o] = gs.vgd;
     st[9 * s + o] = gs.vgb;
@@ -175,60 +221,33 @@ __device__ inline void eval_mos1_caps(const Ctx& C, const Mos1Core& P, int d, do
     st[13 * s + o] = gs.reversed;
 }
 
-template <int DPL>
-__device__ void eval_devices(const Ctx& C, const Mos1Regs<(DPL > 0 ? DPL : 1)>& R, bool tran, const Args& A) {
-    const GroupGPU& G = *C.G;
-    const TopoGPU& T = *C.T;
-    const double* x = C.x;
-    const double gmin = A.gmin, dt = A.tstep;
-    // ---- Mos1 ----
-    if (DPL > 0) {
-#pragma unroll
-        for (int j = 0; j < (DPL > 0 ? DPL : 1); j++) {
-            const int d = C.lane + WAVE * j;
-            if (d < T.n_m1) {
-                const double vd = xv(x, R.nd[j][0]), vg = xv(x, R.nd[j][1]), vs = xv(x, R.nd[j][2]),
-                             vb = xv(x, R.nd[j][3]);
-                double v[MOS1_NE];
-                if (R.caps[j]) {
-                    if (tran)
-                        eval_mos1_caps<true>(C, R.P[j], d, vd, vg, vs, vb, gmin, dt, v);
-                    else
-                        eval_mos1_caps<false>(C, R.P[j], d, vd, vg, vs, vb, gmin, dt, v);
-                } else {
-                    mos1_load<false>(R.P[j], nullptr, nullptr, vd, vg, vs, vb, gmin, dt, v, nullptr);
-                }
-                write_entries<MOS1_NE>(v, R.mask[j], R.base[j], C.contrib);
-            }
-        }
+__device__ inline void eval_mos1(const Ctx& C, const Mos1Core& P, int d, int n0, int n1, int n2, int n3, bool caps,
+                                 Slot put, bool tran, const Args& A) {
+    const double vd = xv(C.x, n0), vg = xv(C.x, n1), vs = xv(C.x, n2), vb = xv(C.x, n3);
+    if (caps) {
+        if (tran)
+            eval_mos1_caps<true>(C, P, d, vd, vg, vs, vb, A.gmin, A.tstep, put);
+        else
+            eval_mos1_caps<false>(C, P, d, vd, vg, vs, vb, A.gmin, A.tstep, put);
     } else {
-        for (int d = C.lane; d < T.n_m1; d += WAVE) {
-            const Mos1Core P = load_mos1_core(G, T.n_m1, C.inst, d);
-            const int* nd = T.m1_nodes + 6 * d;
-            const double vd = xv(x, nd[0]), vg = xv(x, nd[1]), vs = xv(x, nd[2]), vb = xv(x, nd[3]);
-            double v[MOS1_NE];
-            if (T.m1_caps[d]) {
-                if (tran)
-                    eval_mos1_caps<true>(C, P, d, vd, vg, vs, vb, gmin, dt, v);
-                else
-                    eval_mos1_caps<false>(C, P, d, vd, vg, vs, vb, gmin, dt, v);
-            } else {
-                mos1_load<false>(P, nullptr, nullptr, vd, vg, vs, vb, gmin, dt, v, nullptr);
-            }
-            write_entries<MOS1_NE>(v, T.m1_mask[d], T.m1_base[d], C.contrib);
-        }
+        mos1_load<false>(P, nullptr, nullptr, vd, vg, vs, vb, A.gmin, A.tstep, put, nullptr);
     }
-    // ---- Mos0 ----
+}
+
+// Diodes, level-0 MOS and linear elements (few per instance): parameters from global memory.
+__device__ inline void eval_others(const Ctx& C, bool tran, const Args& A) {
+    const GroupGPU& G = *C.G;
+    const TopoGPU& T = C.T;
+    LDS const double* x = C.x;
+    const double gmin = A.gmin, dt = A.tstep;
     for (int d = C.lane; d < T.n_m0; d += WAVE) {
-        const int* nd = T.m0_nodes + 4 * d;
-        double v[MOS0_NE];
-        mos0_load(T.m0_p[d], xv(x, nd[0]), xv(x, nd[1]), xv(x, nd[2]), gmin, v);
-        write_entries<MOS0_NE>(v, T.m0_mask[d], T.m0_base[d], C.contrib);
+        GLB const int* nd = glb(T.m0_nodes) + 4 * d;
+        mos0_load(glb(T.m0_p)[d], xv(x, nd[0]), xv(x, nd[1]), xv(x, nd[2]), gmin,
+                  Slot{C.contrib, glb(T.m0_mask)[d], glb(T.m0_base)[d]});
     }
-    // ---- Diode ----
     for (int d = C.lane; d < T.n_d; d += WAVE) {
         const size_t s = (size_t)G.n_inst * T.n_d, o = (size_t)C.inst * T.n_d + d;
-        const double* q = G.d_par;
+        GLB const double* q = glb(G.d_par);
         DiodeP P;
         P.isat = q[0 * s + o];
         P.vte = q[1 * s + o];
@@ -244,151 +263,184 @@ __device__ void eval_devices(const Ctx& C, const Mos1Regs<(DPL > 0 ? DPL : 1)>& 
         P.tt = q[11 * s + o];
         P.vj = q[12 * s + o];
         P.m = q[13 * s + o];
-        double* st = G.d_state;
-        const int* nd = T.d_nodes + 3 * d;
-        double v[DIODE_NE];
+        GLB double* st = glb(G.d_state);
+        GLB const int* nd = glb(T.d_nodes) + 3 * d;
+        const Slot put{C.contrib, glb(T.d_mask)[d], glb(T.d_base)[d]};
         DiodeGuess gs;
         if (tran)
-            diode_load<true>(P, xv(x, nd[2]), xv(x, nd[1]), gmin, dt, st[0 * s + o], st[2 * s + o], v, &gs);
+            diode_load<true>(P, xv(x, nd[2]), xv(x, nd[1]), gmin, dt, st[0 * s + o], st[2 * s + o], put, &gs);
         else
-            diode_load<false>(P, xv(x, nd[2]), xv(x, nd[1]), gmin, dt, st[0 * s + o], st[2 * s + o], v, &gs);
+            diode_load<false>(P, xv(x, nd[2]), xv(x, nd[1]), gmin, dt, st[0 * s + o], st[2 * s + o], put, &gs);
         st[0 * s + o] = gs.vd;
         st[1 * s + o] = gs.charge;
-        write_entries<DIODE_NE>(v, T.d_mask[d], T.d_base[d], C.contrib);
     }
-    // ---- R, C, V, I ----
     for (int d = C.lane; d < T.n_lin; d += WAVE) {
-        const int typ = T.lin_type[d];
-        const int* nd = T.lin_nodes + 3 * d;
+        const int typ = glb(T.lin_type)[d];
+        GLB const int* nd = glb(T.lin_nodes) + 3 * d;
         const size_t o = (size_t)C.inst * T.n_lin + d;
-        const double val = tran ? G.lin_val_tran[o] : G.lin_val[o];
-        double v[6];
+        const double val = tran ? glb(G.lin_val_tran)[o] : glb(G.lin_val)[o];
+        const Slot put{C.contrib, glb(T.lin_mask)[d], glb(T.lin_base)[d]};
         if (typ == S21_COMP_R) {
-            v[0] = val;
-            v[1] = val;
-            v[2] = -val;
-            v[3] = -val;
-            write_entries<4>(v, T.lin_mask[d], T.lin_base[d], C.contrib);
+            put(0, val);
+            put(1, val);
+            put(2, -val);
+            put(3, -val);
         } else if (typ == S21_COMP_C) {
-            if (tran) {
-                cap_load_tran(val, xv(x, nd[0]) - xv(x, nd[1]), G.c_q[o], dt, v);
-            } else {
-                for (int k = 0; k < 6; k++) v[k] = 0.0;  // no stamps in the operating point
-            }
-            write_entries<6>(v, T.lin_mask[d], T.lin_base[d], C.contrib);
+            if (tran)
+                cap_load_tran(val, xv(x, nd[0]) - xv(x, nd[1]), glb(G.c_q)[o], dt, put);
+            else
+                for (int k = 0; k < 6; k++) put(k, 0.0);  // no stamps in the operating point
         } else if (typ == S21_COMP_V) {
-            v[0] = 1.0;
-            v[1] = 1.0;
-            v[2] = -1.0;
-            v[3] = -1.0;
-            v[4] = val;
-            write_entries<5>(v, T.lin_mask[d], T.lin_base[d], C.contrib);
+            put(0, 1.0);
+            put(1, 1.0);
+            put(2, -1.0);
+            put(3, -1.0);
+            put(4, val);
         } else {
-            v[0] = val;
-            v[1] = -val;
-            write_entries<2>(v, T.lin_mask[d], T.lin_base[d], C.contrib);
+            put(0, val);
+            put(1, -val);
         }
     }
 }
 
-// Ordered gather: slot s < nnz is LU position s, slot nnz + k is RHS entry k (external order).
+template <int DPL>
+__device__ inline void eval_devices(const Ctx& C, const Mos1Regs<(DPL > 0 ? DPL : 1)>& R, bool tran, const Args& A) {
+    const TopoGPU& T = C.T;
+    if (DPL > 0) {
+#pragma unroll
+        for (int j = 0; j < (DPL > 0 ? DPL : 1); j++) {
+            const int d = C.lane + WAVE * j;
+            if (d < T.n_m1)
+                eval_mos1(C, R.P[j], d, lo16(R.nd01[j]), hi16(R.nd01[j]), lo16(R.nd23[j]), hi16(R.nd23[j]),
+                          (R.base[j] >> 30) & 1, Slot{C.contrib, R.mask[j], R.base[j] & 0x3FFFFFFF}, tran, A);
+        }
+    } else {
+        for (int d = C.lane; d < T.n_m1; d += WAVE) {
+            const int* nd = T.m1_nodes + 6 * d;
+            eval_mos1(C, load_mos1_core(*C.G, T.n_m1, C.inst, d), d, nd[0], nd[1], nd[2], nd[3], T.m1_caps[d] != 0,
+                      Slot{C.contrib, T.m1_mask[d], T.m1_base[d]}, tran, A);
+        }
+    }
+    if (T.n_m0 + T.n_d + T.n_lin) eval_others(C, tran, A);
+}
+
+__device__ inline LDS double& slot_ref(const Ctx& C, int s) { return s < C.T.nnz ? C.a[s] : C.rhs[s - C.T.nnz]; }
+
+// Ordered gather of the contributions into A (LU positions) and the RHS.  Each short slot is summed in
+// component-then-stamp order -- the order Solver::update adds them (analysis.rs:153-168); a long slot is
+// summed in chunks whose partial sums are then added in order.
 __device__ inline void gather(const Ctx& C) {
-    const TopoGPU& T = *C.T;
-    const int ns = T.nnz + T.n;
-    for (int s = C.lane; s < ns; s += WAVE) {
+    const Idx& I = C.I;
+    for (int t = C.lane; t < C.T.n_gt; t += WAVE) {
+        const R4 r = r4(I.gt[t]);
         double acc = 0.0;
-        const int e = T.g_ptr[s + 1];
-        for (int k = T.g_ptr[s]; k < e; k++) acc = acc + C.contrib[T.g_idx[k]];
-        if (s < T.nnz)
-            C.a[s] = acc;
+        for (int k = r.y; k < r.z; k++) acc = acc + C.contrib[I.gidx[k]];
+        if (r.w == 0xFFFF)
+            slot_ref(C, r.x) = acc;
         else
-            C.rhs[s - T.nnz] = acc;
+            C.part[r.w] = acc;
+    }
+    wave_sync();
+    for (int t = C.lane; t < C.T.n_gc; t += WAVE) {
+        const R4 r = r4(I.gc[t]);
+        double acc = C.part[r.y];
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + C.part[p];
+        slot_ref(C, r.x) = acc;
     }
 }
 
-// c[i] = rhs[row_i2e[i]] - (A x)[i] over internal row i; returns whether this lane saw |res| > iabstol.
+// c[i] = rhs[row_i2e[i]] - (A x)[i] over internal rows (sparse21/mod.rs:298-327).  Returns whether this
+// lane saw |res| > iabstol.
 __device__ inline bool residual(const Ctx& C, double iabstol) {
-    const TopoGPU& T = *C.T;
+    const Idx& I = C.I;
     bool bad = false;
-    for (int i = C.lane; i < T.n; i += WAVE) {
+    for (int t = C.lane; t < C.T.n_rt; t += WAVE) {
+        const R4 r = r4(I.rt[t]);
         double acc = 0.0;
-        const int e = T.res_ptr[i + 1];
-        for (int k = T.res_ptr[i]; k < e; k++) acc = acc + C.a[T.res_pos[k]] * C.x[T.res_xcol[k]];
-        const double r = C.rhs[T.row_i2e[i]] - acc;
-        C.c[i] = r;
-        bad = bad || (fabs(r) > iabstol);
+        for (int e = r.y; e < r.z; e++) {
+            const R2 q = r2(I.re[e]);
+            acc = acc + C.a[q.x] * C.x[q.y];
+        }
+        if (r.w == 0xFFFF) {
+            const double v = C.rhs[I.ri2e[r.x]] - acc;
+            C.c[r.x] = v;
+            bad = bad || (fabs(v) > iabstol);
+        } else {
+            C.part[r.w] = acc;
+        }
+    }
+    wave_sync();
+    for (int t = C.lane; t < C.T.n_rc; t += WAVE) {
+        const R4 r = r4(I.rc[t]);
+        double acc = C.part[r.y];
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + C.part[p];
+        const double v = C.rhs[I.ri2e[r.x]] - acc;
+        C.c[r.x] = v;
+        bad = bad || (fabs(v) > iabstol);
     }
     return bad;
 }
 
-// Level-scheduled IKJ factorisation.  Returns a nonzero status when a pivot other than the last is zero
-// (the reference's assertion in row_col_elim, sparse21/mod.rs:872).
+// Factorisation with the forward sweep fused in.  A chunk of row i processes L entries (i, k) once row k
+// is final: l_ik = a_ik / u_kk, a_ij -= l_ik u_kj over row k's U entries, and c_i -= c_k l_ik -- for each
+// target in increasing k, the order of the reference's right-looking elimination and column-oriented
+// forward substitution (sparse21/mod.rs:865-919, 946-964).  Returns S21_PIVOT when a pivot other than
+// the last is zero (the assertion at sparse21/mod.rs:872).
 template <typename V>
-__device__ inline int factor(const Ctx& C, V* lu) {
-    const TopoGPU& T = *C.T;
-    bool zero_pivot = false;
-    for (int l = 0; l < T.n_lvl; l++) {
-        const int e = T.lvl_ptr[l + 1];
-        for (int r = T.lvl_ptr[l] + C.lane; r < e; r += WAVE) {
-            const int i = T.lvl_rows[r];
-            const int dg = T.diag[i];
-            for (int q = T.rp[i]; q < dg; q++) {
-                const V piv = lu[T.diag[T.ci[q]]];
-                const V lv = lu[q] / piv;
-                lu[q] = lv;
-                const int ue = T.upd_ptr[q + 1];
-                for (int u = T.upd_ptr[q]; u < ue; u++) {
-                    const int dst = T.upd_dst[u];
-                    const V prod = lu[T.upd_src[u]] * lv;
-                    lu[dst] = lu[dst] - prod;
+__device__ inline int factor_forward(const Ctx& C, LDS V* lu) {
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_clvl; l++) {
+        const int e = I.clvl[l + 1];
+        for (int ch = I.clvl[l] + C.lane; ch < e; ch += WAVE) {
+            const R4 h = r4(I.fch[ch]);
+            V y = (V)C.c[h.x];
+            int u = h.w;
+            for (int j = h.y; j < h.z; j++) {
+                const R4 L = r4(I.fl[j]);
+                const V lv = lu[L.x] / lu[L.y];
+                lu[L.x] = lv;
+                const V cp = (V)C.c[L.z] * lv;
+                y = y - cp;
+                for (int m = 0; m < L.w; m++, u++) {
+                    const R2 q = r2(I.fu[u]);
+                    const V prod = lu[q.x] * lv;
+                    lu[q.y] = lu[q.y] - prod;
                 }
             }
-            if (i != T.n - 1 && lu[dg] == V(0)) zero_pivot = true;
+            C.c[h.x] = (double)y;
         }
-        __syncthreads();
+        wave_sync();
     }
+    bool zero_pivot = false;
+    for (int i = C.lane; i < C.T.n - 1; i += WAVE)
+        if (lu[I.diag[i]] == V(0)) zero_pivot = true;
     return wave_any(zero_pivot) ? S21_PIVOT : S21_OK;
 }
 
-// Forward (unit L) and backward (U) sweeps over c in internal order.
+// Backward sweep over U by row levels (sparse21/mod.rs:966-979).
 template <typename V>
-__device__ inline void sweeps(const Ctx& C, const V* lu) {
-    const TopoGPU& T = *C.T;
-    for (int l = 0; l < T.n_lvl; l++) {
-        const int e = T.lvl_ptr[l + 1];
-        for (int r = T.lvl_ptr[l] + C.lane; r < e; r += WAVE) {
-            const int i = T.lvl_rows[r];
-            V y = (V)C.c[i];
-            const int dg = T.diag[i];
-            for (int q = T.rp[i]; q < dg; q++) {
-                const V prod = (V)C.c[T.ci[q]] * lu[q];
+__device__ inline void backward(const Ctx& C, LDS const V* lu) {
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_blvl; l++) {
+        const int e = I.blvl[l + 1];
+        for (int r = I.blvl[l] + C.lane; r < e; r += WAVE) {
+            const R4 h = r4(I.bro[r]);
+            V y = (V)C.c[h.x];
+            for (int k = h.z; k < h.w; k++) {
+                const R2 q = r2(I.bu[k]);
+                const V prod = (V)C.c[q.y] * lu[q.x];
                 y = y - prod;
             }
-            C.c[i] = (double)y;
+            C.c[h.x] = (double)(y / lu[h.y]);
         }
-        __syncthreads();
-    }
-    for (int l = 0; l < T.n_blvl; l++) {
-        const int e = T.blvl_ptr[l + 1];
-        for (int r = T.blvl_ptr[l] + C.lane; r < e; r += WAVE) {
-            const int i = T.blvl_rows[r];
-            V y = (V)C.c[i];
-            const int dg = T.diag[i];
-            const int re = T.rp[i + 1];
-            for (int q = dg + 1; q < re; q++) {
-                const V prod = (V)C.c[T.ci[q]] * lu[q];
-                y = y - prod;
-            }
-            C.c[i] = (double)(y / lu[dg]);
-        }
-        __syncthreads();
+        wave_sync();
     }
 }
 
 // Commit the accepted operating point (Component::commit) of every stateful device.
 __device__ inline void commit(const Ctx& C) {
     const GroupGPU& G = *C.G;
-    const TopoGPU& T = *C.T;
+    const TopoGPU& T = C.T;
     for (int d = C.lane; d < T.n_lin; d += WAVE) {
         if (T.lin_type[d] != S21_COMP_C) continue;
         const int* nd = T.lin_nodes + 3 * d;
@@ -407,78 +459,119 @@ __device__ inline void commit(const Ctx& C) {
     }
 }
 
-// One Newton solve.  `iters` accumulates loads.  Returns an S21_* status.
+__device__ inline long long cycles() { return (long long)__builtin_readcyclecounter(); }
+
+// One Newton solve (Solver::solve, analysis.rs:169-210).  `iters` accumulates loads.  Returns an S21_* status.
 template <int DPL, bool MIXED>
 __device__ int newton(const Ctx& C, const Mos1Regs<(DPL > 0 ? DPL : 1)>& R, bool tran, const Args& A,
-                      long long& iters) {
-    const TopoGPU& T = *C.T;
+                      long long& iters, long long* pc) {
+    const TopoGPU& T = C.T;
+    const bool prof = A.prof != 0;
+    long long t0 = 0, t1 = 0;
     bool prev_ok = true;  // dx starts at zero
     for (int it = 0; it < A.max_iter; it++) {
         iters++;
+        if (prof) t0 = cycles();
         eval_devices<DPL>(C, R, tran, A);
-        __syncthreads();
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[0] += t1 - t0; t0 = t1; }
         gather(C);
-        __syncthreads();
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[1] += t1 - t0; t0 = t1; }
         const bool bad = wave_any(residual(C, A.iabstol));
-        __syncthreads();
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
         if (prev_ok && !bad) {
             commit(C);
-            __syncthreads();
+            wave_sync();
+            if (prof) { t1 = cycles(); pc[6] += t1 - t0; }
             return S21_OK;
         }
         if (T.sym_status != S21_OK) return T.sym_status;
         int st;
         if (MIXED) {
             for (int p = C.lane; p < T.nnz; p += WAVE) C.lu32[p] = (float)C.a[p];
-            __syncthreads();
-            st = factor<float>(C, C.lu32);
-            if (st == S21_OK) sweeps<float>(C, C.lu32);
+            wave_sync();
+            st = factor_forward<float>(C, C.lu32);
+            if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+            if (st == S21_OK) backward<float>(C, C.lu32);
         } else {
-            st = factor<double>(C, C.a);
-            if (st == S21_OK) sweeps<double>(C, C.a);
+            st = factor_forward<double>(C, C.a);
+            if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+            if (st == S21_OK) backward<double>(C, C.a);
         }
+        if (prof) { t1 = cycles(); pc[4] += t1 - t0; t0 = t1; }
         if (st != S21_OK) return st;
         // clamp and update (analysis.rs:196-207)
         double mx = 0.0;
         for (int k = C.lane; k < T.n; k += WAVE) {
-            const double dx = fabs(C.c[T.col_e2i[k]]);
+            const double dx = fabs(C.c[C.I.ce2i[k]]);
             mx = (dx > mx) ? dx : mx;
         }
         mx = wave_max(mx);
         bool ok = true;
         for (int k = C.lane; k < T.n; k += WAVE) {
-            double dx = C.c[T.col_e2i[k]];
+            double dx = C.c[C.I.ce2i[k]];
             if (mx > 1.0) dx = dx * 1.0 / mx;
             ok = ok && !(fabs(dx) > A.reltol);
             C.x[k] = C.x[k] + dx;
         }
         prev_ok = !wave_any(!ok);
-        __syncthreads();
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[5] += t1 - t0; }
     }
     return S21_CONVERGENCE;
 }
 
+// Workgroup = A.waves wavefronts, one circuit instance each, sharing one LDS copy of the group's records.
 template <int DPL, bool MIXED>
-__global__ __launch_bounds__(WAVE) void newton_kernel(const GroupGPU* __restrict__ groups, int n_groups, Args A) {
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel(const GroupGPU* __restrict__ groups, int n_groups, Args A) {
     int g = 0;
     const int b = blockIdx.x;
     while (g + 1 < n_groups && groups[g + 1].blk0 <= b) g++;
     const GroupGPU& G = groups[g];
-    const TopoGPU& T = G.t;
-    const int inst = (A.mode == MODE_ASSEMBLE) ? A.dbg_inst : b - G.blk0;
-    if (inst < 0 || inst >= G.n_inst) return;
-    extern __shared__ double smem[];
     Ctx C;
     C.G = &G;
-    C.T = &T;
+    C.T = G.t;
+    const TopoGPU& T = C.T;
+    extern __shared__ uint4 smem_u4[];
+    // records -> LDS (whole workgroup)
+    {
+        GLB const uint64_t* src = (GLB const uint64_t*)T.blob;
+        LDS uint64_t* dst = (LDS uint64_t*)smem_u4;
+        for (int k = threadIdx.x; k < 2 * T.blob_vec; k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int inst = (A.mode == MODE_ASSEMBLE) ? A.dbg_inst : (b - G.blk0) * A.waves + w;
+    if (inst < 0 || inst >= G.n_inst || (A.mode == MODE_ASSEMBLE && w != 0)) return;
     C.inst = inst;
-    C.lane = threadIdx.x;
-    C.x = smem;
+    C.lane = threadIdx.x % WAVE;
+    LDS const u16* b16 = (LDS const u16*)smem_u4;
+    C.I.gt = (LDS const uint64_t*)(b16 + T.o_gt);
+    C.I.gidx = b16 + T.o_gidx;
+    C.I.gc = (LDS const uint64_t*)(b16 + T.o_gc);
+    C.I.rt = (LDS const uint64_t*)(b16 + T.o_rt);
+    C.I.re = (LDS const uint32_t*)(b16 + T.o_re);
+    C.I.rc = (LDS const uint64_t*)(b16 + T.o_rc);
+    C.I.ri2e = b16 + T.o_ri2e;
+    C.I.ce2i = b16 + T.o_ce2i;
+    C.I.diag = b16 + T.o_diag;
+    C.I.clvl = b16 + T.o_clvl;
+    C.I.fch = (LDS const uint64_t*)(b16 + T.o_fch);
+    C.I.fl = (LDS const uint64_t*)(b16 + T.o_fl);
+    C.I.fu = (LDS const uint32_t*)(b16 + T.o_fu);
+    C.I.blvl = b16 + T.o_blvl;
+    C.I.bro = (LDS const uint64_t*)(b16 + T.o_bro);
+    C.I.bu = (LDS const uint32_t*)(b16 + T.o_bu);
+    LDS double* dbase = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) + (size_t)w * A.inst_bytes);
+    C.x = dbase;
     C.rhs = C.x + T.n;
     C.c = C.rhs + T.n;
     C.contrib = C.c + T.n;
     C.a = C.contrib + T.n_contrib;
-    C.lu32 = (float*)(C.a + T.nnz);
+    C.part = C.a + T.nnz;
+    C.lu32 = (LDS float*)(C.part + T.n_part);
 
     constexpr int NR = DPL > 0 ? DPL : 1;
     Mos1Regs<NR> R;
@@ -487,38 +580,39 @@ __global__ __launch_bounds__(WAVE) void newton_kernel(const GroupGPU* __restrict
         const int d = C.lane + WAVE * j;
         if (DPL > 0 && d < T.n_m1) {
             R.P[j] = load_mos1_core(G, T.n_m1, inst, d);
-            for (int k = 0; k < 4; k++) R.nd[j][k] = T.m1_nodes[6 * d + k];
+            const int* nd = T.m1_nodes + 6 * d;
+            R.nd01[j] = (nd[0] + 1) | ((nd[1] + 1) << 16);
+            R.nd23[j] = (nd[2] + 1) | ((nd[3] + 1) << 16);
             R.mask[j] = T.m1_mask[d];
-            R.base[j] = T.m1_base[d];
-            R.caps[j] = T.m1_caps[d];
-        } else {
-            R.caps[j] = 0;
+            R.base[j] = T.m1_base[d] | (T.m1_caps[d] ? (1 << 30) : 0);
         }
     }
     for (int k = C.lane; k < T.n; k += WAVE) C.x[k] = G.x[(size_t)inst * T.n + k];
-    __syncthreads();
+    wave_sync();
 
     if (A.mode == MODE_ASSEMBLE) {
         eval_devices<DPL>(C, R, false, A);
-        __syncthreads();
+        wave_sync();
         gather(C);
-        __syncthreads();
+        wave_sync();
         for (int s = C.lane; s < T.nnz; s += WAVE) G.dbg[s] = C.a[s];
         for (int k = C.lane; k < T.n; k += WAVE) G.dbg[T.nnz + k] = C.rhs[k];
         return;
     }
 
+    long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const long long tstart = A.prof ? cycles() : 0;
     long long iters = 0;
-    int st = newton<DPL, MIXED>(C, R, false, A, iters);
+    int st = newton<DPL, MIXED>(C, R, false, A, iters, pc);
     if (A.mode == MODE_TRAN && st == S21_OK) {
-        double* w = G.wave + (size_t)inst * (size_t)A.npts * T.n;
-        for (int k = C.lane; k < T.n; k += WAVE) w[k] = C.x[k];
+        double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
+        for (int k = C.lane; k < T.n; k += WAVE) wv[k] = C.x[k];
         long long pt = 1;
         double t = A.tstep;  // analysis.rs:553-570
         while (t < A.tstop && pt < A.npts) {
-            st = newton<DPL, MIXED>(C, R, true, A, iters);
+            st = newton<DPL, MIXED>(C, R, true, A, iters, pc);
             if (st != S21_OK) break;
-            for (int k = C.lane; k < T.n; k += WAVE) w[(size_t)pt * T.n + k] = C.x[k];
+            for (int k = C.lane; k < T.n; k += WAVE) wv[(size_t)pt * T.n + k] = C.x[k];
             pt++;
             t += A.tstep;
         }
@@ -527,6 +621,10 @@ __global__ __launch_bounds__(WAVE) void newton_kernel(const GroupGPU* __restrict
     if (C.lane == 0) {
         G.status[inst] = st;
         G.iters[inst] = iters;
+        if (A.prof && G.prof) {
+            pc[7] = cycles() - tstart;
+            for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
+        }
     }
 }
 
@@ -580,7 +678,8 @@ struct GroupHost {
     std::vector<int> status;
     std::vector<long long> iters;
     int dpl = 0;
-    size_t smem = 0;
+    size_t blob_bytes = 0, inst_bytes = 0;
+    std::string prepared;  // analysis the uploaded structure was built for ("" = none)
 };
 
 }  // namespace s21
@@ -594,6 +693,8 @@ struct s21_batch {
     double last_ms = 0.0, kernel_avg_ms = 0.0;
     int64_t launches = 0;
     int64_t npts = 0;
+    int profile = 0;
+    int last_waves = 1;
     std::vector<double> axis;
     std::unique_ptr<s21::DevMem> d_wave, d_groups;
     int last_mode = -1;
@@ -813,57 +914,86 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     const std::vector<double>&m1_core = hp.m1_core, &m1_caps = hp.m1_caps, &d_par = hp.d_par,
                               &lin_val = hp.lin_val, &lin_tran = hp.lin_tran;
 
-    // ---- upload topology (one int buffer) ----
+    // ---- record tables: one uint16 blob, copied into LDS by every workgroup ----
+    topo_records(g.topo);
+    std::vector<uint16_t> blob;
+    auto sect = [&](const std::vector<uint16_t>& v) {
+        const int off = (int)blob.size();
+        blob.insert(blob.end(), v.begin(), v.end());
+        while (blob.size() % 4) blob.push_back(0);  // 8-byte alignment for ushort4 records
+        if (v.empty()) blob.insert(blob.end(), 4, 0);
+        return off;
+    };
+    auto to16 = [](const std::vector<int>& v) {
+        std::vector<uint16_t> o;
+        for (int x : v) {
+            if (x < 0 || x > 65535) fail(S21_INVALID, "circuit too large for the batched kernel (16-bit index)");
+            o.push_back((uint16_t)x);
+        }
+        return o;
+    };
+    std::vector<int> dg(t.diag);
+    for (int& v : dg) v = std::max(v, 0);  // the unpivoted fallback pattern may lack diagonals
+    TopoGPU& T = g.gg.t;
+    T.o_gt = sect(t.gt);
+    T.o_gidx = sect(to16(t.g_idx));
+    T.o_gc = sect(t.gc);
+    T.o_rt = sect(t.rt);
+    T.o_re = sect(t.re);
+    T.o_rc = sect(t.rc);
+    T.o_ri2e = sect(to16(t.row_i2e));
+    T.o_ce2i = sect(to16(t.col_e2i));
+    T.o_diag = sect(to16(dg));
+    T.o_clvl = sect(to16(t.clvl_ptr.empty() ? std::vector<int>{0} : t.clvl_ptr));
+    T.o_fch = sect(t.fch);
+    T.o_fl = sect(t.fl);
+    T.o_fu = sect(t.fu);
+    T.o_blvl = sect(to16(t.blvl_ptr));
+    T.o_bro = sect(t.bro);
+    T.o_bu = sect(t.bu);
+    while (blob.size() % 8) blob.push_back(0);
+    T.n_gt = (int)t.gt.size() / 4;
+    T.n_gc = (int)t.gc.size() / 4;
+    T.n_rt = (int)t.rt.size() / 4;
+    T.n_rc = (int)t.rc.size() / 4;
+    T.n_part = t.n_part;
+
+    // ---- per-device tables (global) ----
     std::vector<int> ib;
-    std::vector<size_t> offs;
+    std::vector<size_t> io;
     auto push = [&](const std::vector<int>& v) {
-        offs.push_back(ib.size());
+        io.push_back(ib.size());
         ib.insert(ib.end(), v.begin(), v.end());
         if (v.empty()) ib.push_back(0);
     };
     auto as_int = [](const std::vector<uint32_t>& v) { return std::vector<int>(v.begin(), v.end()); };
     auto u8_int = [](const std::vector<uint8_t>& v) { return std::vector<int>(v.begin(), v.end()); };
-    push(t.m1_nodes);            // 0
-    push(t.m1_base);             // 1
-    push(as_int(t.m1_mask));     // 2
-    push(u8_int(t.m1_caps));     // 3
-    push(t.m0_nodes);            // 4
-    push(t.m0_base);             // 5
-    push(as_int(t.m0_mask));     // 6
-    push(t.d_nodes);             // 7
-    push(t.d_base);              // 8
-    push(as_int(t.d_mask));      // 9
-    push(t.lin_type);            // 10
-    push(t.lin_nodes);           // 11
-    push(t.lin_base);            // 12
-    push(as_int(t.lin_mask));    // 13
-    push(t.g_ptr);               // 14
-    push(t.g_idx);               // 15
-    push(t.res_ptr);             // 16
-    push(t.res_pos);             // 17
-    push(t.res_xcol);            // 18
-    push(t.row_i2e);             // 19
-    push(t.col_e2i);             // 20
-    push(t.rp);                  // 21
-    push(t.ci);                  // 22
-    push(t.diag);                // 23
-    push(t.upd_ptr);             // 24
-    push(t.upd_src);             // 25
-    push(t.upd_dst);             // 26
-    push(t.lvl_ptr);             // 27
-    push(t.lvl_rows);            // 28
-    push(t.blvl_ptr);            // 29
-    push(t.blvl_rows);           // 30
-    const size_t dbl_off_bytes = ((ib.size() * sizeof(int) + 255) / 256) * 256;
+    push(t.m1_nodes);          // 0
+    push(t.m1_base);           // 1
+    push(as_int(t.m1_mask));   // 2
+    push(u8_int(t.m1_caps));   // 3
+    push(t.m0_nodes);          // 4
+    push(t.m0_base);           // 5
+    push(as_int(t.m0_mask));   // 6
+    push(t.d_nodes);           // 7
+    push(t.d_base);            // 8
+    push(as_int(t.d_mask));    // 9
+    push(t.lin_type);          // 10
+    push(t.lin_nodes);         // 11
+    push(t.lin_base);          // 12
+    push(as_int(t.lin_mask));  // 13
     const size_t n_m0 = t.m0_p.size();
+    while (blob.size() % 8) blob.push_back(0);  // whole 16-byte vectors
+    const size_t blob_bytes = blob.size() * sizeof(uint16_t);
+    const size_t ib_off = ((blob_bytes + 255) / 256) * 256;
+    const size_t dbl_off = ib_off + ((ib.size() * sizeof(int) + 255) / 256) * 256;
     g.d_topo.reset(new DevMem());
-    g.d_topo->alloc(dbl_off_bytes + std::max<size_t>(n_m0, 1) * sizeof(double));
-    HIPCHECK(hipMemcpy(g.d_topo->p, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (n_m0)
-        HIPCHECK(hipMemcpy((char*)g.d_topo->p + dbl_off_bytes, t.m0_p.data(), n_m0 * sizeof(double),
-                           hipMemcpyHostToDevice));
-    const int* base = (const int*)g.d_topo->p;
-    TopoGPU& T = g.gg.t;
+    g.d_topo->alloc(dbl_off + std::max<size_t>(n_m0, 1) * sizeof(double));
+    char* tb = (char*)g.d_topo->p;
+    HIPCHECK(hipMemcpy(tb, blob.data(), blob_bytes, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(tb + ib_off, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (n_m0) HIPCHECK(hipMemcpy(tb + dbl_off, t.m0_p.data(), n_m0 * sizeof(double), hipMemcpyHostToDevice));
+    const int* base = (const int*)(tb + ib_off);
     T.n = t.n;
     T.nnz = t.nnz;
     T.n_contrib = t.n_contrib;
@@ -871,41 +1001,28 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     T.n_m0 = (int)n_m0;
     T.n_d = n_d;
     T.n_lin = n_lin;
-    T.n_lvl = (int)t.lvl_ptr.size() - 1;
+    T.n_clvl = (int)t.clvl_ptr.size() - 1;
+    if (T.n_clvl < 0) T.n_clvl = 0;
+    T.n_part = t.n_part;
     T.n_blvl = (int)t.blvl_ptr.size() - 1;
     T.sym_status = t.sym_status;
-    T.m1_nodes = base + offs[0];
-    T.m1_base = base + offs[1];
-    T.m1_mask = (const uint32_t*)(base + offs[2]);
-    T.m1_caps = base + offs[3];
-    T.m0_nodes = base + offs[4];
-    T.m0_base = base + offs[5];
-    T.m0_mask = (const uint32_t*)(base + offs[6]);
-    T.m0_p = (const double*)((const char*)g.d_topo->p + dbl_off_bytes);
-    T.d_nodes = base + offs[7];
-    T.d_base = base + offs[8];
-    T.d_mask = (const uint32_t*)(base + offs[9]);
-    T.lin_type = base + offs[10];
-    T.lin_nodes = base + offs[11];
-    T.lin_base = base + offs[12];
-    T.lin_mask = (const uint32_t*)(base + offs[13]);
-    T.g_ptr = base + offs[14];
-    T.g_idx = base + offs[15];
-    T.res_ptr = base + offs[16];
-    T.res_pos = base + offs[17];
-    T.res_xcol = base + offs[18];
-    T.row_i2e = base + offs[19];
-    T.col_e2i = base + offs[20];
-    T.rp = base + offs[21];
-    T.ci = base + offs[22];
-    T.diag = base + offs[23];
-    T.upd_ptr = base + offs[24];
-    T.upd_src = base + offs[25];
-    T.upd_dst = base + offs[26];
-    T.lvl_ptr = base + offs[27];
-    T.lvl_rows = base + offs[28];
-    T.blvl_ptr = base + offs[29];
-    T.blvl_rows = base + offs[30];
+    T.blob_vec = (int)(blob_bytes / 16);
+    T.blob = (const uint4*)tb;
+    T.m1_nodes = base + io[0];
+    T.m1_base = base + io[1];
+    T.m1_mask = (const uint32_t*)(base + io[2]);
+    T.m1_caps = base + io[3];
+    T.m0_nodes = base + io[4];
+    T.m0_base = base + io[5];
+    T.m0_mask = (const uint32_t*)(base + io[6]);
+    T.m0_p = (const double*)(tb + dbl_off);
+    T.d_nodes = base + io[7];
+    T.d_base = base + io[8];
+    T.d_mask = (const uint32_t*)(base + io[9]);
+    T.lin_type = base + io[10];
+    T.lin_nodes = base + io[11];
+    T.lin_base = base + io[12];
+    T.lin_mask = (const uint32_t*)(base + io[13]);
 
     // ---- parameters (read-only) ----
     auto pad = [](size_t n) { return ((n * sizeof(double) + 255) / 256) * 256; };
@@ -937,22 +1054,49 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
 
     // ---- outputs ----
     const size_t s_x = pad((size_t)NI * t.n), s_st = ((NI * sizeof(int) + 255) / 256) * 256,
-                 s_it = ((NI * sizeof(long long) + 255) / 256) * 256, s_dbg = pad((size_t)t.nnz + t.n);
+                 s_it = ((NI * sizeof(long long) + 255) / 256) * 256, s_dbg = pad((size_t)t.nnz + t.n),
+                 s_prof = b->profile ? ((NI * NPROF * sizeof(long long) + 255) / 256) * 256 : 0;
     g.d_out.reset(new DevMem());
-    g.d_out->alloc(s_x + s_st + s_it + s_dbg + 256);
+    g.d_out->alloc(s_x + s_st + s_it + s_dbg + s_prof + 256);
     HIPCHECK(hipMemsetAsync(g.d_out->p, 0, g.d_out->bytes, b->stream));
     char* op = (char*)g.d_out->p;
     g.gg.x = (double*)op;
     g.gg.status = (int*)(op + s_x);
     g.gg.iters = (long long*)(op + s_x + s_st);
     g.gg.dbg = (double*)(op + s_x + s_st + s_it);
+    g.gg.prof = b->profile ? (long long*)(op + s_x + s_st + s_it + s_dbg) : nullptr;
     g.gg.n_inst = NI;
     g.gg.wave = nullptr;
 
     HIPCHECK(hipStreamSynchronize(b->stream));
     g.dpl = n_m1 == 0 ? 1 : n_m1 <= 64 ? 1 : n_m1 <= 128 ? 2 : n_m1 <= 256 ? 4 : 0;
-    g.smem = (size_t)(3 * t.n + t.n_contrib + t.nnz) * sizeof(double) +
-             (b->precision == S21_MIXED ? (size_t)t.nnz * sizeof(float) : 0);
+    g.blob_bytes = blob_bytes;
+    g.inst_bytes = (((size_t)(3 * t.n + t.n_contrib + t.nnz + t.n_part) * sizeof(double) +
+                     (b->precision == S21_MIXED ? (size_t)t.nnz * sizeof(float) : 0)) + 15) / 16 * 16;
+}
+
+// Key of the structure an analysis needs: the operating point, or a transient with its IC set.
+static std::string prep_key(const s21_tran_options* tr) {
+    if (!tr) return "op";
+    std::string k = "tran";
+    for (int i = 0; i < tr->n_ic; i++) k += ":" + std::to_string(tr->ic_var[i]);
+    return k;
+}
+
+// Prepare every group for an analysis, or only reset the Newton state when the structure is current.
+static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
+    const std::string key = prep_key(tr);
+    for (GroupHost& g : b->groups) {
+        if (g.prepared == key) {
+            HIPCHECK(hipMemsetAsync(g.d_state->p, 0, g.d_state->bytes, b->stream));
+            HIPCHECK(hipMemsetAsync(g.d_out->p, 0, g.d_out->bytes, b->stream));
+        } else {
+            g.prepared.clear();
+            prepare_group(b, g, tr);
+            g.prepared = key;
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(b->stream));
 }
 
 template <int DPL, bool MIXED>
@@ -960,7 +1104,7 @@ static void launch_one(s21_batch* b, const GroupGPU* dgroups, int n_groups, int 
                        const Args& A) {
     auto kern = newton_kernel<DPL, MIXED>;
     if (smem > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(WAVE), smem, b->stream, dgroups, n_groups, A);
+    hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(WAVE * A.waves), smem, b->stream, dgroups, n_groups, A);
     HIPCHECK(hipGetLastError());
 }
 
@@ -974,7 +1118,24 @@ static void launch_dpl(s21_batch* b, int dpl, const GroupGPU* dg, int ng, int nb
     }
 }
 
-constexpr size_t MAX_SMEM = 160 * 1024;
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
+// Instances per workgroup: the count that keeps the most instances resident per CU (every workgroup
+// carries one copy of the records plus per-instance state).
+static int pick_waves(size_t blob, size_t inst, int max_inst, size_t max_block) {
+    int best = 1;
+    size_t best_res = 0;
+    for (int k = 1; k <= MAX_WAVES && k <= std::max(max_inst, 1); k++) {
+        const size_t bytes = blob + (size_t)k * inst;
+        if (bytes > max_block) break;
+        const size_t res = (LDS_PER_CU / bytes) * k;
+        if (res > best_res) {
+            best_res = res;
+            best = k;
+        }
+    }
+    return best;
+}
 
 // Runs the kernel over every group (one launch per register-residency class).
 static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
@@ -987,8 +1148,6 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     size_t wave_off = 0;
     for (int gi : order) {
         GroupHost& g = b->groups[gi];
-        if (g.smem > MAX_SMEM)
-            fail(S21_INVALID, "circuit too large for the batched kernel: " + std::to_string(g.smem) + " bytes of LDS");
         GroupGPU gg = g.gg;
         gg.wave = d_wave ? d_wave + wave_off : nullptr;
         wave_off += (size_t)g.n_inst * (size_t)npts * g.topo.n;
@@ -1001,23 +1160,39 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     size_t i = 0;
     b->launches = 0;
     float total = 0.0f;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    int max_block = 0;
+    HIPCHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     while (i < hg.size()) {
         size_t j = i;
-        int blk = 0;
-        size_t smem = 0;
+        size_t blob = 0, inst = 0;
+        int max_inst = 0;
         while (j < hg.size() && cls[j] == cls[i]) {
-            hg[j].blk0 = blk;
-            blk += (A0.mode == MODE_ASSEMBLE) ? 1 : hg[j].n_inst;
-            smem = std::max(smem, b->groups[order[j]].smem);
+            blob = std::max(blob, b->groups[order[j]].blob_bytes);
+            inst = std::max(inst, b->groups[order[j]].inst_bytes);
+            max_inst = std::max(max_inst, hg[j].n_inst);
             j++;
         }
+        Args A = A0;
+        A.waves = A0.mode == MODE_ASSEMBLE ? 1 : pick_waves(blob, inst, max_inst, (size_t)max_block);
+        A.inst_bytes = (int)inst;
+        const size_t smem = blob + (size_t)A.waves * inst;
+        if (smem > (size_t)max_block)
+            fail(S21_INVALID, "circuit too large for the batched kernel: " + std::to_string(smem) + " bytes of LDS");
+        int blk = 0;
+        for (size_t q = i; q < j; q++) {
+            hg[q].blk0 = blk;
+            blk += (A.mode == MODE_ASSEMBLE) ? 1 : (hg[q].n_inst + A.waves - 1) / A.waves;
+        }
+        b->last_waves = A.waves;
         HIPCHECK(hipMemcpyAsync((GroupGPU*)b->d_groups->p + i, hg.data() + i, (j - i) * sizeof(GroupGPU),
                                 hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipEventRecord(b->ev0, b->stream));
         if (b->precision == S21_MIXED)
-            launch_dpl<true>(b, cls[i], (GroupGPU*)b->d_groups->p + i, (int)(j - i), blk, smem, A0);
+            launch_dpl<true>(b, cls[i], (GroupGPU*)b->d_groups->p + i, (int)(j - i), blk, smem, A);
         else
-            launch_dpl<false>(b, cls[i], (GroupGPU*)b->d_groups->p + i, (int)(j - i), blk, smem, A0);
+            launch_dpl<false>(b, cls[i], (GroupGPU*)b->d_groups->p + i, (int)(j - i), blk, smem, A);
         HIPCHECK(hipEventRecord(b->ev1, b->stream));
         HIPCHECK(hipEventSynchronize(b->ev1));
         float ms = 0.0f;
@@ -1053,6 +1228,9 @@ static Args base_args(const s21_batch* b, int mode) {
     A.tstop = 0.0;
     A.npts = 1;
     A.dbg_inst = 0;
+    A.prof = b->profile;
+    A.waves = 1;
+    A.inst_bytes = 0;
     return A;
 }
 
@@ -1158,7 +1336,7 @@ void s21_batch_free(s21_batch* b) { delete b; }
 int s21_batch_op(s21_batch* b) {
     return guard([&] {
         if (!b) fail(S21_INVALID, "null batch");
-        for (GroupHost& g : b->groups) prepare_group(b, g, nullptr);
+        prepare_all(b, nullptr);
         Args A = base_args(b, MODE_OP);
         run(b, A, nullptr, 1);
         fetch_results(b);
@@ -1171,7 +1349,10 @@ int s21_batch_op(s21_batch* b) {
 int s21_batch_tran(s21_batch* b, const s21_tran_options* tr) {
     return guard([&] {
         if (!b || !tr) fail(S21_INVALID, "null argument");
-        for (GroupHost& g : b->groups) prepare_group(b, g, tr);
+        if (!(tr->tstep > 0.0) && tr->tstop > 0.0) fail(S21_INVALID, "tstep must be positive");
+        // IC values live in the uploaded element values: rebuild when they are part of the key.
+        for (GroupHost& g : b->groups) g.prepared.clear();
+        prepare_all(b, tr);
         const int64_t npts = s21_tran_point_count(tr->tstep, tr->tstop);
         size_t total = 0;
         for (GroupHost& g : b->groups) total += (size_t)g.n_inst * (size_t)npts * g.topo.n;
@@ -1273,6 +1454,22 @@ int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, do
         if (kernel_avg_ms) *kernel_avg_ms = b->kernel_avg_ms;
     });
 }
+int s21_batch_set_profiling(s21_batch* b, int32_t on) {
+    return guard([&] {
+        if (!b) fail(S21_INVALID, "null batch");
+        if (b->profile != (on ? 1 : 0))
+            for (s21::GroupHost& g : b->groups) g.prepared.clear();
+        b->profile = on ? 1 : 0;
+    });
+}
+int s21_batch_profile(const s21_batch* b, int32_t g, int64_t* out) {
+    return guard([&] {
+        if (!b || g < 0 || g >= (int)b->groups.size() || !out) fail(S21_INVALID, "bad arguments");
+        const GroupHost& gh = b->groups[g];
+        if (!gh.gg.prof) fail(S21_INVALID, "profiling was off during the last analysis");
+        HIPCHECK(hipMemcpy(out, gh.gg.prof, (size_t)gh.n_inst * NPROF * sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
 int s21_batch_pivots(const s21_batch* b, int32_t g, int32_t* row_i2e, int32_t* col_i2e) {
     return guard([&] {
         if (!b || g < 0 || g >= (int)b->groups.size()) fail(S21_INVALID, "bad arguments");
@@ -1301,22 +1498,24 @@ int s21_batch_assemble(s21_batch* b, int32_t g, int32_t inst, const double* x, i
         if (!b || g < 0 || g >= (int)b->groups.size() || !x) fail(S21_INVALID, "bad arguments");
         GroupHost& gh = b->groups[g];
         if (inst < 0 || inst >= gh.n_inst) fail(S21_INVALID, "instance out of range");
-        for (GroupHost& o : b->groups) prepare_group(b, o, nullptr);
+        prepare_all(b, nullptr);
         const Topology& t = gh.topo;
         HIPCHECK(hipMemcpy(gh.gg.x + (size_t)inst * t.n, x, t.n * sizeof(double), hipMemcpyHostToDevice));
         // run only this group
         std::vector<GroupHost> saved;
         Args A = base_args(b, MODE_ASSEMBLE);
         A.dbg_inst = inst;
+        A.waves = 1;
+        A.inst_bytes = (int)gh.inst_bytes;
         GroupGPU gg = gh.gg;
         gg.blk0 = 0;
         if (!b->d_groups) b->d_groups.reset(new DevMem());
         b->d_groups->alloc(sizeof(GroupGPU));
         HIPCHECK(hipMemcpy(b->d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice));
         if (b->precision == S21_MIXED)
-            launch_dpl<true>(b, gh.dpl, (GroupGPU*)b->d_groups->p, 1, 1, gh.smem, A);
+            launch_dpl<true>(b, gh.dpl, (GroupGPU*)b->d_groups->p, 1, 1, gh.blob_bytes + gh.inst_bytes, A);
         else
-            launch_dpl<false>(b, gh.dpl, (GroupGPU*)b->d_groups->p, 1, 1, gh.smem, A);
+            launch_dpl<false>(b, gh.dpl, (GroupGPU*)b->d_groups->p, 1, 1, gh.blob_bytes + gh.inst_bytes, A);
         HIPCHECK(hipStreamSynchronize(b->stream));
         std::vector<double> dbg(t.nnz + t.n);
         HIPCHECK(hipMemcpy(dbg.data(), gh.gg.dbg, dbg.size() * sizeof(double), hipMemcpyDeviceToHost));

@@ -60,6 +60,10 @@ struct Topology {
     std::vector<int> upd_src, upd_dst;
     std::vector<int> lvl_ptr, lvl_rows;    // row levels of the factorisation / forward sweep
     std::vector<int> blvl_ptr, blvl_rows;  // row levels of the backward sweep
+    // Pipelined factorisation / forward schedule: level l holds chunks (row, first L position, end L
+    // position); a chunk runs once the pivot rows of its L entries are complete, so a long row advances
+    // together with the rows it depends on instead of waiting for all of them.
+    std::vector<int> clvl_ptr, ch_row, ch_q0, ch_q1;
     std::vector<int> res_ptr, res_pos, res_xcol;  // residual rows (internal) over A entries
 
     // devices by type, in component order within each type
@@ -79,6 +83,17 @@ struct Topology {
     std::vector<uint32_t> lin_mask;
     std::vector<int> lin_base;
 
+    // Kernel records (16-bit fields, packed in fours or pairs), built by topo_records:
+    //   gt {slot, k0, k1, part}   gather tasks over g_idx[k0, k1); part = 0xFFFF writes the slot itself
+    //   gc {slot, p0, p1, 0}      sums of the partials of a split slot
+    //   rt {row, e0, e1, part}    residual tasks over re[e0, e1); re {LU position, x column}
+    //   rc {row, p0, p1, 0}       sums of the partials of a split residual row
+    //   fch {row, l0, l1, u0}     factorisation chunks, level order (clvl_ptr)
+    //   fl {q, pivot position, k, n_updates}; fu {src, dst}
+    //   bro {row, diag, e0, e1}   backward-sweep rows, level order (blvl_ptr); bu {LU position, column}
+    std::vector<uint16_t> gt, gc, rt, re, rc, fch, fl, fu, bro, bu;
+    int n_part = 0;  // partial-sum slots needed by the split gather / residual tasks
+
     int n_contrib = 0;
     int sym_status = S21_OK;  // failure of the pivot search, reported when an instance needs a solve
     std::string sym_error;
@@ -93,6 +108,7 @@ void topo_elements(Topology& t);
 void topo_pivot(Topology& t, const std::vector<double>& first_values);
 void topo_pivot_search(Topology& t, const std::vector<double>& first_values);
 void topo_schedule(Topology& t, const std::vector<uint8_t>& mos1_has_caps);
+void topo_records(Topology& t);
 
 // Reference-order OP assembly on the host (analysis.rs:153-168) at voltages x for the pivot search.
 struct HostInstance {

@@ -1,11 +1,10 @@
 // Device physics shared by the HIP kernels and the host symbolic pass.
 //
-// Each model writes its stamps in "entry order": entry e is a fixed (row-terminal, col-terminal)
-// pair of the device, named as if source and drain were not swapped.  A MOS device whose source and
-// drain are swapped at this bias writes reference stamp k into entry sigma(k), which is where the
-// reference's `matps[(dr, ..)]` indexing sends it (comps/mos.rs:833-868, 1085-1097).  Entry order
-// therefore maps statically onto matrix elements, and the per-element order of a device's own
-// contributions is the reference's stamp order.
+// Each model hands its stamps, in the reference's stamp order, to a caller-supplied put(entry, value).
+// Entry e is a fixed (row-terminal, col-terminal) pair of the device, named as if source and drain were
+// not swapped; a MOS device whose source and drain are swapped at this bias sends reference stamp k to
+// entry sigma(k), which is where the reference's `matps[(dr, ..)]` indexing sends it (comps/mos.rs:
+// 833-868, 1085-1097).  Entries therefore map statically onto matrix elements.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -69,12 +68,14 @@ __host__ __device__ inline Integ be_integ(double dq, double dq_dv, double vguess
     return Integ{g, i, rhs};
 }
 
-// Mos1::op_stamp (comps/mos.rs:649-893).  Terminal voltages vd, vg, vs, vb; writes entry-order values
-// v[0..25].  TRAN selects the transient branch; with caps == nullptr the device has no capacitance
-// and the integration terms are zero.  guess (optional) receives the new operating point.
-template <bool TRAN>
+// Mos1::op_stamp (comps/mos.rs:649-893).  Terminal voltages vd, vg, vs, vb.  Calls put(entry, value)
+// for the 22 matrix stamps and 4 RHS stamps in the reference's stamp order, stamp k going to entry
+// sigma(k) when source and drain are swapped.  TRAN selects the transient branch; with caps == nullptr
+// the device has no capacitance and the integration terms are zero.  guess (optional) receives the new
+// operating point.
+template <bool TRAN, class Put>
 __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* caps, const Mos1State* op, double vd_,
-                                          double vg_, double vs_, double vb_, double gmin, double dt, double* v,
+                                          double vg_, double vs_, double vb_, double gmin, double dt, Put&& put,
                                           Mos1State* guess) {
     const double p = P.p;
     const bool reversed = p * (vd_ - vs_) < 0.0;
@@ -169,35 +170,33 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
     }
     const double irhs = ids - gm * vgs - gds * vds;
     const double grd = P.grd, grs = P.grs;
-    double s[MOS1_NE];
-    s[0] = gds + grd + gbd + trgd_g;
-    s[1] = gm + gds + grs + gbs + gmbs + trgs_g;
-    s[2] = -gm - gds - gmbs;
-    s[3] = -gds;
-    s[4] = gm - trgd_g;
-    s[5] = -gm - trgs_g;
-    s[6] = (trgd_g + trgs_g + trgb_g);
-    s[7] = (gbd + gbs + trgb_g);
-    s[8] = -trgb_g;
-    s[9] = -trgd_g;
-    s[10] = -trgs_g;
-    s[11] = -trgb_g;
-    s[12] = -gbd;
-    s[13] = -gbs;
-    s[14] = -gbd + gmbs;
-    s[15] = -gbs - gmbs;
-    s[16] = -grd;
-    s[17] = -grd;
-    s[18] = grd;
-    s[19] = -grs;
-    s[20] = -grs;
-    s[21] = grs;
-    s[22] = p * (-irhs + ibd_rhs + trgd_rhs);
-    s[23] = p * (irhs + ibs_rhs + trgs_rhs);
-    s[24] = -p * (trgs_rhs + trgb_rhs + trgd_rhs);
-    s[25] = -p * (ibd_rhs + ibs_rhs - trgb_rhs);
-#pragma unroll
-    for (int e = 0; e < MOS1_NE; e++) v[e] = reversed ? s[mos1_sigma(e)] : s[e];
+    auto E = [reversed](int k) { return reversed ? mos1_sigma(k) : k; };
+    put(E(0), gds + grd + gbd + trgd_g);
+    put(E(1), gm + gds + grs + gbs + gmbs + trgs_g);
+    put(E(2), -gm - gds - gmbs);
+    put(E(3), -gds);
+    put(E(4), gm - trgd_g);
+    put(E(5), -gm - trgs_g);
+    put(E(6), (trgd_g + trgs_g + trgb_g));
+    put(E(7), (gbd + gbs + trgb_g));
+    put(E(8), -trgb_g);
+    put(E(9), -trgd_g);
+    put(E(10), -trgs_g);
+    put(E(11), -trgb_g);
+    put(E(12), -gbd);
+    put(E(13), -gbs);
+    put(E(14), -gbd + gmbs);
+    put(E(15), -gbs - gmbs);
+    put(E(16), -grd);
+    put(E(17), -grd);
+    put(E(18), grd);
+    put(E(19), -grs);
+    put(E(20), -grs);
+    put(E(21), grs);
+    put(E(22), p * (-irhs + ibd_rhs + trgd_rhs));
+    put(E(23), p * (irhs + ibs_rhs + trgs_rhs));
+    put(E(24), -p * (trgs_rhs + trgb_rhs + trgd_rhs));
+    put(E(25), -p * (ibd_rhs + ibs_rhs - trgb_rhs));
 }
 
 // ---- Mos0 (comps/mos.rs:1051-1099) -------------------------------------------------------------
@@ -214,7 +213,8 @@ __host__ __device__ constexpr int mos0_sigma(int e) {
     constexpr int s[MOS0_NE] = {1, 0, 3, 2, 5, 4, 7, 6};
     return s[e];
 }
-__host__ __device__ inline void mos0_load(double p, double vd, double vg, double vs, double gmin, double* v) {
+template <class Put>
+__host__ __device__ inline void mos0_load(double p, double vd, double vg, double vs, double gmin, Put&& put) {
     const double vth = 0.25, beta = 50e-3, lam = 3e-3;
     const double vds1 = p * (vd - vs);
     const bool reversed = vds1 < 0.0;
@@ -234,17 +234,15 @@ __host__ __device__ inline void mos0_load(double p, double vd, double vg, double
         }
     }
     const double irhs = ids - gm * vgs - gds * vds;
-    double s[MOS0_NE];
-    s[0] = gds + gmin;
-    s[1] = (gm + gds + gmin);
-    s[2] = -(gm + gds + gmin);
-    s[3] = -gds - gmin;
-    s[4] = gm;
-    s[5] = -gm;
-    s[6] = -p * irhs;
-    s[7] = p * irhs;
-#pragma unroll
-    for (int e = 0; e < MOS0_NE; e++) v[e] = reversed ? s[mos0_sigma(e)] : s[e];
+    auto E = [reversed](int k) { return reversed ? mos0_sigma(k) : k; };
+    put(E(0), gds + gmin);
+    put(E(1), (gm + gds + gmin));
+    put(E(2), -(gm + gds + gmin));
+    put(E(3), -gds - gmin);
+    put(E(4), gm);
+    put(E(5), -gm);
+    put(E(6), -p * irhs);
+    put(E(7), p * irhs);
 }
 
 // ---- Diode (comps/diode.rs:228-354) -------------------------------------------------------------
@@ -273,9 +271,9 @@ __host__ __device__ inline double diode_limit(const DiodeP& P, double vnew, doub
     }
     return P.vte * log(vnew / P.vte);
 }
-template <bool TRAN>
+template <bool TRAN, class Put>
 __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn, double gmin, double dt,
-                                           double prev_guess_vd, double op_charge, double* v, DiodeGuess* guess) {
+                                           double prev_guess_vd, double op_charge, Put&& put, DiodeGuess* guess) {
     double vd = vr - vn;
     const bool has_bv = P.has_bv != 0.0;
     if (has_bv && vd < fmin(10.0 * P.vte - P.bv, 0.0)) {
@@ -314,15 +312,15 @@ __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn
         guess->charge = qd;
     }
     const double irhs = id - vd * gd;
-    v[0] = gd;
-    v[1] = -gd;
-    v[2] = -gd;
-    v[3] = gd + P.gspr;
-    v[4] = P.gspr;
-    v[5] = -P.gspr;
-    v[6] = -P.gspr;
-    v[7] = -irhs;
-    v[8] = irhs;
+    put(0, gd);
+    put(1, -gd);
+    put(2, -gd);
+    put(3, gd + P.gspr);
+    put(4, P.gspr);
+    put(5, -P.gspr);
+    put(6, -P.gspr);
+    put(7, -irhs);
+    put(8, irhs);
 }
 
 // ---- linear elements (comps/mod.rs:95-346) -------------------------------------------------------
@@ -331,15 +329,16 @@ __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn
 constexpr int R_NE = 4, C_NE = 6, V_NE = 5, I_NE = 2;
 
 // Capacitor transient load: returns the companion (g, rhs) of the BE integration.
-__host__ __device__ inline void cap_load_tran(double c, double vd, double op_q, double dt, double* v) {
+template <class Put>
+__host__ __device__ inline void cap_load_tran(double c, double vd, double op_q, double dt, Put&& put) {
     const double q = c * vd;
     const Integ a = be_integ(q - op_q, c, vd, dt);
-    v[0] = a.g;
-    v[1] = a.g;
-    v[2] = -a.g;
-    v[3] = -a.g;
-    v[4] = -a.rhs;
-    v[5] = a.rhs;
+    put(0, a.g);
+    put(1, a.g);
+    put(2, -a.g);
+    put(3, -a.g);
+    put(4, -a.rhs);
+    put(5, a.rhs);
 }
 
 }  // namespace s21

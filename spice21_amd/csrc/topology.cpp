@@ -11,6 +11,7 @@
 //  * topo_schedule   -- L+U pattern in internal order, IKJ update lists, row levels for the
 //                       factorisation and both sweeps, contribution slots and gather lists.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <numeric>
 #include <unordered_map>
@@ -109,45 +110,38 @@ void host_assemble_op(const Topology& t, const HostInstance& h, const s21_option
         switch (c.type) {
         case S21_COMP_MOS1: {
             const Mos1Derived& D = h.m1[im1++];
-            double v[MOS1_NE];
-            mos1_load<false>(D.core, nullptr, nullptr, X(nd[RD]), X(nd[RG]), X(nd[RS]), X(nd[RB]), o.gmin, 0.0, v,
+            mos1_load<false>(D.core, nullptr, nullptr, X(nd[RD]), X(nd[RG]), X(nd[RS]), X(nd[RB]), o.gmin, 0.0,
+                             [&](int e, double v) {
+                                 const int r = nd[mos1_entry_row(e)];
+                                 if (mos1_entry_col(e) < 0)
+                                     addb(r, v);
+                                 else
+                                     addm(r, nd[mos1_entry_col(e)], v);
+                             },
                              nullptr);
-            const bool rev = D.core.p * (X(nd[RD]) - X(nd[RS])) < 0.0;
-            for (int k = 0; k < MOS1_NE; k++) {  // stamp order; stamp k sits in entry sigma(k) when reversed
-                int e = rev ? mos1_sigma(k) : k;
-                int r = nd[mos1_entry_row(e)];
-                if (mos1_entry_col(e) < 0)
-                    addb(r, v[e]);
-                else
-                    addm(r, nd[mos1_entry_col(e)], v[e]);
-            }
             break;
         }
         case S21_COMP_MOS0: {
-            double v[MOS0_NE];
             const double p = c.model == 1 ? -1.0 : 1.0;
-            mos0_load(p, X(nd[RD]), X(nd[RG]), X(nd[RS]), o.gmin, v);
-            const bool rev = p * (X(nd[RD]) - X(nd[RS])) < 0.0;
-            for (int k = 0; k < MOS0_NE; k++) {
-                int e = rev ? mos0_sigma(k) : k;
-                int r = nd[mos0_entry_row(e)];
+            mos0_load(p, X(nd[RD]), X(nd[RG]), X(nd[RS]), o.gmin, [&](int e, double v) {
+                const int r = nd[mos0_entry_row(e)];
                 if (mos0_entry_col(e) < 0)
-                    addb(r, v[e]);
+                    addb(r, v);
                 else
-                    addm(r, nd[mos0_entry_col(e)], v[e]);
-            }
+                    addm(r, nd[mos0_entry_col(e)], v);
+            });
             break;
         }
         case S21_COMP_DIODE: {
-            double v[DIODE_NE];
-            diode_load<false>(h.dio[id++], X(nd[DR_]), X(nd[DN_]), o.gmin, 0.0, 0.0, 0.0, v, nullptr);
-            for (int e = 0; e < DIODE_NE; e++) {
-                int r = nd[diode_entry_row(e)];
-                if (diode_entry_col(e) < 0)
-                    addb(r, v[e]);
-                else
-                    addm(r, nd[diode_entry_col(e)], v[e]);
-            }
+            diode_load<false>(h.dio[id++], X(nd[DR_]), X(nd[DN_]), o.gmin, 0.0, 0.0, 0.0,
+                              [&](int e, double v) {
+                                  const int r = nd[diode_entry_row(e)];
+                                  if (diode_entry_col(e) < 0)
+                                      addb(r, v);
+                                  else
+                                      addm(r, nd[diode_entry_col(e)], v);
+                              },
+                              nullptr);
             break;
         }
         case S21_COMP_R: {
@@ -500,6 +494,42 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
     bucket(lf, maxf, t.lvl_ptr, t.lvl_rows);
     bucket(lb, maxb, t.blvl_ptr, t.blvl_rows);
 
+    // Chunked schedule over L entries, rows in internal order (pivot rows come first).
+    {
+        std::vector<int> complete(n, 0);  // first level at which the row's factor is final
+        std::vector<std::array<int, 4>> chunks;  // level, row, q0, q1
+        int maxl = -1;
+        for (int i = 0; i < n; i++) {
+            int cur = 0, last = -1;
+            for (int q = t.rp[i]; t.diag[i] >= 0 && q < t.diag[i]; q++) {
+                const int lvl = std::max(cur, complete[t.ci[q]]);
+                if (last >= 0 && chunks[last][0] == lvl)
+                    chunks[last][3] = q + 1;
+                else {
+                    chunks.push_back({lvl, i, q, q + 1});
+                    last = (int)chunks.size() - 1;
+                }
+                cur = lvl;
+            }
+            complete[i] = last >= 0 ? cur + 1 : 0;
+            maxl = std::max(maxl, last >= 0 ? cur : -1);
+        }
+        const int nl = maxl + 1;
+        t.clvl_ptr.assign(nl + 1, 0);
+        for (auto& c : chunks) t.clvl_ptr[c[0] + 1]++;
+        for (int l = 0; l < nl; l++) t.clvl_ptr[l + 1] += t.clvl_ptr[l];
+        std::vector<int> fillp(t.clvl_ptr.begin(), t.clvl_ptr.end() - 1);
+        t.ch_row.assign(chunks.size(), 0);
+        t.ch_q0.assign(chunks.size(), 0);
+        t.ch_q1.assign(chunks.size(), 0);
+        for (auto& c : chunks) {
+            const int at = fillp[c[0]]++;
+            t.ch_row[at] = c[1];
+            t.ch_q0[at] = c[2];
+            t.ch_q1[at] = c[3];
+        }
+    }
+
     // Residual rows over A entries (internal row i = external row row_i2e[i]).
     t.res_ptr.assign(n + 1, 0);
     t.res_pos.clear();
@@ -629,6 +659,86 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
     for (int s = 0; s < t.nnz + n; s++) {
         for (int c : slot[s]) t.g_idx.push_back(c);
         t.g_ptr[s + 1] = (int)t.g_idx.size();
+    }
+}
+
+}  // namespace s21
+
+namespace s21 {
+
+// Packs the schedule into the records the kernel walks (see Topology).  Long gather lists and residual
+// rows are split into tasks of at most CHUNK elements whose partial sums a second pass adds in order.
+void topo_records(Topology& t) {
+    constexpr int CHUNK = 8;
+    auto u = [](int v) -> uint16_t {
+        if (v < 0 || v > 65535) fail(S21_INVALID, "circuit too large for the batched kernel (16-bit index)");
+        return (uint16_t)v;
+    };
+    const int n = t.n;
+    int part = 0, part_r = 0;
+    t.gt.clear();
+    t.gc.clear();
+    for (int s = 0; s < t.nnz + n; s++) {
+        const int k0 = t.g_ptr[s], k1 = t.g_ptr[s + 1];
+        if (k1 - k0 <= CHUNK) {
+            for (int v : {s, k0, k1, 65535}) t.gt.push_back(u(v));
+            continue;
+        }
+        const int p0 = part;
+        for (int k = k0; k < k1; k += CHUNK)
+            for (int v : {s, k, std::min(k + CHUNK, k1), part++}) t.gt.push_back(u(v));
+        for (int v : {s, p0, part, 0}) t.gc.push_back(u(v));
+    }
+    t.rt.clear();
+    t.re.clear();
+    t.rc.clear();
+    for (int i = 0; i < n; i++) {
+        const int e0 = (int)t.re.size() / 2;
+        for (int k = t.res_ptr[i]; k < t.res_ptr[i + 1]; k++) {
+            t.re.push_back(u(t.res_pos[k]));
+            t.re.push_back(u(t.res_xcol[k]));
+        }
+        const int e1 = (int)t.re.size() / 2;
+        if (e1 - e0 <= CHUNK) {
+            for (int v : {i, e0, e1, 65535}) t.rt.push_back(u(v));
+            continue;
+        }
+        const int p0 = part_r;
+        for (int e = e0; e < e1; e += CHUNK)
+            for (int v : {i, e, std::min(e + CHUNK, e1), part_r++}) t.rt.push_back(u(v));
+        for (int v : {i, p0, part_r, 0}) t.rc.push_back(u(v));
+    }
+    t.n_part = std::max(part, part_r);
+    // factorisation chunks: L entries of each chunk as {q, pivot, k, n_updates}, updates as {src, dst}
+    t.fch.clear();
+    t.fl.clear();
+    t.fu.clear();
+    for (size_t c = 0; c < t.ch_row.size(); c++) {
+        const int l0 = (int)t.fl.size() / 4, u0 = (int)t.fu.size() / 2;
+        for (int q = t.ch_q0[c]; q < t.ch_q1[c]; q++) {
+            const int k = t.ci[q];
+            const int nu = t.upd_ptr[q + 1] - t.upd_ptr[q];
+            for (int v : {q, t.diag[k], k, nu}) t.fl.push_back(u(v));
+            for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++) {
+                t.fu.push_back(u(t.upd_src[x]));
+                t.fu.push_back(u(t.upd_dst[x]));
+            }
+        }
+        const int l1 = (int)t.fl.size() / 4;
+        for (int v : {t.ch_row[c], l0, l1, u0}) t.fch.push_back(u(v));
+    }
+    // backward rows in level order with their U entries {position, column}
+    t.bro.clear();
+    t.bu.clear();
+    for (int r = 0; r < (int)t.blvl_rows.size(); r++) {
+        const int i = t.blvl_rows[r];
+        const int e0 = (int)t.bu.size() / 2;
+        for (int q = t.diag[i] + 1; q < t.rp[i + 1]; q++) {
+            t.bu.push_back(u(q));
+            t.bu.push_back(u(t.ci[q]));
+        }
+        const int e1 = (int)t.bu.size() / 2;
+        for (int v : {i, std::max(t.diag[i], 0), e0, e1}) t.bro.push_back(u(v));
     }
 }
 

@@ -5,7 +5,7 @@ reference's top-level elaboration does (spice21/src/elab.rs:35-173): terminals a
 in port order, a Vsrc appends its branch current after its terminals, a Mos1 appends ``{name}.dp`` /
 ``{name}.sp`` when its model has rd/rs/rsh (comps/mos.rs:592-620), and a Diode appends ``{name}.r``
 when its model has a nonzero rs (comps/diode.rs:100-110).  Hierarchical circuits go through the
-library's own elaborator instead (``spice21_amd.Circuit``).
+library's own elaborator instead (``spice21_amd.elab``).
 """
 import ctypes as C
 
@@ -115,6 +115,43 @@ class Netlist:
             raise KeyError(f"diode {name}: nodes must already exist")
         r = self.addv(f"{name}.r") if self._diode_has_rs[model] else pv
         return self._push(A.S21_COMP_DIODE, [pv, nv, r], model=model, inst=inst, name=name)
+
+    def instance(self, overrides, i):
+        """Standalone netlist of instance i of a batch with these overrides: every overridden device gets its
+        own model / instance parameter set, as the reference needs one Mos1Model + Mos1InstParams per
+        varied device."""
+        import copy
+
+        m = copy.deepcopy(self)
+        for comp, target, field, vals in overrides:
+            typ, nodes, model, inst, value, acm, name = m.comps[comp]
+            if target == "value":
+                m.comps[comp] = (typ, nodes, model, inst, float(vals[i]), acm, name)
+                continue
+            mos = typ == A.S21_COMP_MOS1
+            if mos:
+                table = A.MOS1_MODEL_FIELDS if target == "model" else A.MOS1_INST_FIELDS
+                lst = m.mos1_models if target == "model" else m.mos1_insts
+            else:
+                table = A.DIODE_MODEL_FIELDS if target == "model" else A.DIODE_INST_FIELDS
+                lst = m.diode_models if target == "model" else m.diode_insts
+            f = A._field(table, field) if isinstance(field, str) else int(field)
+            idx = model if target == "model" else inst
+            p = A.s21_params()
+            C.memmove(C.byref(p), C.byref(lst[idx]), C.sizeof(A.s21_params))
+            p.v[f] = float(vals[i])
+            p.set |= 1 << f
+            lst.append(p)
+            if target == "model" and mos:
+                m._mos1_model_r.append(m._mos1_model_r[model])
+            if target == "model" and not mos:
+                m._diode_has_rs.append(m._diode_has_rs[model])
+            new = len(lst) - 1
+            if target == "model":
+                m.comps[comp] = (typ, nodes, new, inst, value, acm, name)
+            else:
+                m.comps[comp] = (typ, nodes, model, new, value, acm, name)
+        return m
 
     # ---- C view ----
     def to_c(self):

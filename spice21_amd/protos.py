@@ -1,0 +1,134 @@
+"""Protobuf message classes of the Spice21 wire schema, built at import time from descriptors.
+
+The schema restates the messages of spice21/protos/spice21.proto and spice21/protos/mos.proto that the
+analyses take and return, with the same package, message and field names and field numbers, so bytes
+produced here decode in the reference and vice versa.  (The image has no protoc; the descriptors are
+assembled directly.)  BSIM4 messages carry only their names here: BSIM4 devices are not simulated.
+"""
+from google.protobuf import descriptor_pb2, descriptor_pool, message_factory, wrappers_pb2
+
+F = descriptor_pb2.FieldDescriptorProto
+_D, _U64, _I64, _S, _M, _E = F.TYPE_DOUBLE, F.TYPE_UINT64, F.TYPE_INT64, F.TYPE_STRING, F.TYPE_MESSAGE, F.TYPE_ENUM
+_OPT, _REP = F.LABEL_OPTIONAL, F.LABEL_REPEATED
+_DV = ".google.protobuf.DoubleValue"
+_IV = ".google.protobuf.Int64Value"
+
+# message name -> list of (field name, number, type, label, type name | None, oneof | None)
+_SCHEMA = {
+    "Resistor": [("name", 1, _S), ("p", 2, _S), ("n", 3, _S), ("g", 4, _D)],
+    "Capacitor": [("name", 1, _S), ("p", 2, _S), ("n", 3, _S), ("c", 4, _D)],
+    "Isrc": [("name", 1, _S), ("p", 2, _S), ("n", 3, _S), ("dc", 4, _D)],
+    "Vsrc": [("name", 1, _S), ("p", 2, _S), ("n", 3, _S), ("dc", 4, _D), ("acm", 5, _D)],
+    "TwoTerms": [("p", 2, _S), ("n", 3, _S)],
+    "DiodeModel": [("name", 1, _S)]
+    + [(n, k, _M, _OPT, _DV) for n, k in [("tnom", 2), ("is", 3), ("n", 4), ("tt", 5), ("vj", 6), ("m", 7), ("eg", 8),
+                                          ("xti", 9), ("kf", 10), ("af", 11), ("fc", 12), ("bv", 13), ("ibv", 14),
+                                          ("rs", 15), ("cj0", 16)]],
+    "DiodeInstParams": [("name", 1, _S), ("model", 2, _S), ("area", 4, _M, _OPT, _DV), ("temp", 5, _M, _OPT, _DV)],
+    "Diode": [("name", 1, _S), ("p", 2, _S), ("n", 3, _S), ("model", 4, _S), ("params", 5, _S)],
+    "MosPorts": [("d", 1, _S), ("g", 2, _S), ("s", 3, _S), ("b", 4, _S)],
+    "Mos": [("name", 1, _S), ("model", 2, _S), ("params", 3, _S), ("ports", 4, _M, _OPT, ".spice21.MosPorts")],
+    "Mos1InstParams": [("name", 1, _S)]
+    + [(n, k, _M, _OPT, _DV) for n, k in [("m", 2), ("l", 3), ("w", 4), ("a_d", 5), ("a_s", 6), ("pd", 7), ("ps", 8),
+                                          ("nrd", 9), ("nrs", 10), ("temp", 11)]],
+    "Mos1Model": [("name", 1, _S), ("mos_type", 2, _E, _OPT, ".spice21.MosType")]
+    + [(n, k, _M, _OPT, _DV) for n, k in [("vt0", 3), ("kp", 4), ("gamma", 5), ("phi", 6), ("lambda", 7), ("rd", 8),
+                                          ("rs", 9), ("cbd", 10), ("cbs", 11), ("is", 12), ("pb", 13), ("cgso", 14),
+                                          ("cgdo", 15), ("cgbo", 16), ("rsh", 17), ("cj", 18), ("mj", 19), ("cjsw", 20),
+                                          ("mjsw", 21), ("js", 22), ("tox", 23), ("ld", 24), ("u0", 25), ("fc", 26),
+                                          ("nsub", 27), ("nss", 29), ("tnom", 30), ("kf", 31), ("af", 32)]]
+    + [("tpg", 28, _M, _OPT, _IV)],
+    "Bsim4Model": [("name", 1, _S), ("mos_type", 2, _E, _OPT, ".spice21.MosType")],
+    "Bsim4InstParams": [("name", 1, _S)],
+    "Instance": [("r", 1, _M, _OPT, ".spice21.Resistor", 0), ("c", 2, _M, _OPT, ".spice21.Capacitor", 0),
+                 ("m", 3, _M, _OPT, ".spice21.Mos", 0), ("i", 4, _M, _OPT, ".spice21.Isrc", 0),
+                 ("v", 5, _M, _OPT, ".spice21.Vsrc", 0), ("d", 6, _M, _OPT, ".spice21.Diode", 0),
+                 ("x", 7, _M, _OPT, ".spice21.ModuleInstance", 0)],
+    "Module": [("name", 1, _S), ("ports", 2, _S, _REP), ("signals", 4, _S, _REP),
+               ("comps", 5, _M, _REP, ".spice21.Instance"), ("params", 9, "map", _S, _D)],
+    "ModuleInstance": [("name", 1, _S), ("module", 2, _S), ("ports", 3, "map", _S, _S), ("params", 4, "map", _S, _D)],
+    "Def": [("module", 1, _M, _OPT, ".spice21.Module", 0), ("diodemodel", 2, _M, _OPT, ".spice21.DiodeModel", 0),
+            ("diodeinst", 3, _M, _OPT, ".spice21.DiodeInstParams", 0),
+            ("bsim4model", 4, _M, _OPT, ".spice21.Bsim4Model", 0),
+            ("bsim4inst", 5, _M, _OPT, ".spice21.Bsim4InstParams", 0),
+            ("mos1model", 6, _M, _OPT, ".spice21.Mos1Model", 0),
+            ("mos1inst", 7, _M, _OPT, ".spice21.Mos1InstParams", 0)],
+    "Defs": [("defs", 1, _M, _REP, ".spice21.Def")],
+    "Circuit": [("name", 1, _S), ("signals", 2, _S, _REP), ("defs", 3, _M, _REP, ".spice21.Def"),
+                ("comps", 4, _M, _REP, ".spice21.Instance")],
+    "SimOptions": [(n, k, _M, _OPT, _DV) for n, k in [("temp", 1), ("tnom", 2), ("gmin", 3), ("iabstol", 4),
+                                                      ("reltol", 5)]],
+    "Op": [("ckt", 1, _M, _OPT, ".spice21.Circuit"), ("opts", 2, _M, _OPT, ".spice21.SimOptions")],
+    "OpResult": [("vals", 1, "map", _S, _D)],
+    "TranOptions": [("tstop", 1, _D), ("tstep", 2, _D), ("ic", 3, "map", _S, _D)],
+    "Tran": [("ckt", 1, _M, _OPT, ".spice21.Circuit"), ("opts", 2, _M, _OPT, ".spice21.SimOptions"),
+             ("args", 3, _M, _OPT, ".spice21.TranOptions")],
+    "DoubleArray": [("vals", 1, _D, _REP)],
+    "TranResult": [("time", 1, _M, _OPT, ".spice21.DoubleArray"), ("vals", 2, "map", _S, (_M, ".spice21.DoubleArray"))],
+    "ComplexNum": [("re", 1, _D), ("im", 2, _D)],
+    "ComplexArray": [("vals", 1, _M, _REP, ".spice21.ComplexNum")],
+    "AcOptions": [("fstart", 1, _U64), ("fstop", 2, _U64), ("npts", 3, _U64)],
+    "Ac": [("ckt", 1, _M, _OPT, ".spice21.Circuit"), ("opts", 2, _M, _OPT, ".spice21.SimOptions"),
+           ("args", 3, _M, _OPT, ".spice21.AcOptions")],
+    "AcResult": [("freq", 1, _M, _OPT, ".spice21.DoubleArray"),
+                 ("vals", 2, "map", _S, (_M, ".spice21.ComplexArray"))],
+}
+_ONEOFS = {"Instance": "comp", "Def": "defines"}
+
+
+def _camel(s):
+    return "".join(p[:1].upper() + p[1:] for p in s.split("_"))
+
+
+def _file():
+    fdp = descriptor_pb2.FileDescriptorProto(name="spice21_amd/spice21.proto", package="spice21", syntax="proto3")
+    fdp.dependency.append("google/protobuf/wrappers.proto")
+    e = fdp.enum_type.add(name="MosType")
+    e.value.add(name="NMOS", number=0)
+    e.value.add(name="PMOS", number=1)
+    for mname, fields in _SCHEMA.items():
+        m = fdp.message_type.add(name=mname)
+        if mname in _ONEOFS:
+            m.oneof_decl.add(name=_ONEOFS[mname])
+        for spec in fields:
+            name, num = spec[0], spec[1]
+            if spec[2] == "map":
+                ktype, vtype = spec[3], spec[4]
+                ent = m.nested_type.add(name=_camel(name) + "Entry")
+                ent.options.map_entry = True
+                ent.field.add(name="key", number=1, type=ktype, label=_OPT)
+                if isinstance(vtype, tuple):
+                    ent.field.add(name="value", number=2, type=vtype[0], label=_OPT, type_name=vtype[1])
+                else:
+                    ent.field.add(name="value", number=2, type=vtype, label=_OPT)
+                m.field.add(name=name, number=num, type=_M, label=_REP,
+                            type_name=f".spice21.{mname}.{_camel(name)}Entry")
+                continue
+            ftype = spec[2]
+            label = spec[3] if len(spec) > 3 else _OPT
+            f = m.field.add(name=name, number=num, type=ftype, label=label)
+            if len(spec) > 4 and spec[4]:
+                f.type_name = spec[4]
+            if len(spec) > 5 and spec[5] is not None:
+                f.oneof_index = spec[5]
+    return fdp
+
+
+_pool = descriptor_pool.DescriptorPool()
+_pool.AddSerializedFile(wrappers_pb2.DESCRIPTOR.serialized_pb)
+_pool.Add(_file())
+
+
+def _cls(name):
+    return message_factory.GetMessageClass(_pool.FindMessageTypeByName("spice21." + name))
+
+
+class MosType:
+    NMOS = 0
+    PMOS = 1
+
+
+for _n in _SCHEMA:
+    globals()[_n] = _cls(_n)
+
+__all__ = ["MosType"] + list(_SCHEMA)

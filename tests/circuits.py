@@ -51,40 +51,7 @@ GOLDEN_RUNS = {
 }
 
 
+
 def materialize(nl, overrides, i):
-    """The single-instance netlist of instance i of a batch: every override gets its own model / instance
-    parameter set, as the reference needs one Mos1Model + Mos1InstParams per varied device."""
-    import copy
-
-    from spice21_amd import _abi as A
-
-    m = copy.deepcopy(nl)
-    for comp, target, field, vals in overrides:
-        typ, nodes, model, inst, value, acm, name = m.comps[comp]
-        if target == "value":
-            m.comps[comp] = (typ, nodes, model, inst, float(vals[i]), acm, name)
-            continue
-        if typ == A.S21_COMP_MOS1:
-            table = A.MOS1_MODEL_FIELDS if target == "model" else A.MOS1_INST_FIELDS
-            lst = m.mos1_models if target == "model" else m.mos1_insts
-        else:
-            table = A.DIODE_MODEL_FIELDS if target == "model" else A.DIODE_INST_FIELDS
-            lst = m.diode_models if target == "model" else m.diode_insts
-        f = A._field(table, field) if isinstance(field, str) else field
-        idx = model if target == "model" else inst
-        p = A.s21_params()
-        C_src = lst[idx]
-        for k in range(A.S21_PARAM_SLOTS):
-            p.v[k] = C_src.v[k]
-        p.set = C_src.set
-        p.v[f] = float(vals[i])
-        p.set |= 1 << f
-        lst.append(p)
-        if typ == A.S21_COMP_MOS1:
-            m._mos1_model_r.append(m._mos1_model_r[model]) if target == "model" else None
-        new = len(lst) - 1
-        if target == "model":
-            m.comps[comp] = (typ, nodes, new, inst, value, acm, name)
-        else:
-            m.comps[comp] = (typ, nodes, model, new, value, acm, name)
-    return m
+    """Standalone netlist of instance i of a batch (see Netlist.instance)."""
+    return nl.instance(overrides, i)

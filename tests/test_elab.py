@@ -1,0 +1,134 @@
+"""Protobuf decoding and hierarchy elaboration (circuit.rs, elab.rs restated in csrc/circuit.cpp).
+
+The reference's Module-based ring oscillators must elaborate to the same variables and components as the
+hand-flattened netlists, and drive the oracle to the committed golden waveforms bit for bit."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from spice21_amd import _abi as A
+from spice21_amd import protos as P
+from spice21_amd._lib import S21Error
+from spice21_amd.api import circuit
+from spice21_amd.elab import elaborate
+from tests.circuits import GOLDEN_RUNS, ro3
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def inst(**kw):
+    return P.Instance(**kw)
+
+
+def mos(name, g, d, s, b, model, params="default"):
+    return inst(m=P.Mos(name=name, model=model, params=params, ports=P.MosPorts(g=g, d=d, s=s, b=b)))
+
+
+def ro3_proto(kind):
+    """cmos_ro3 / nmos_ro3 / pmos_ro3 (tests.rs:889-912, 966-990, 1009-1033) as protobuf."""
+    if kind == "cmos":
+        body = [mos("p", "inp", "out", "vdd", "vdd", "pmos"), mos("n", "inp", "out", "vss", "vss", "nmos"),
+                inst(c=P.Capacitor(name="c", p="out", n="vss", c=1e-15))]
+    elif kind == "nmos":
+        body = [mos("m", "inp", "out", "vss", "vss", "nmos"), inst(r=P.Resistor(name="r", p="out", n="vdd", g=1e-6)),
+                inst(c=P.Capacitor(name="c", p="out", n="vdd", c=0.5e-15))]
+    elif kind == "pmos":
+        body = [inst(c=P.Capacitor(name="c", p="out", n="vss", c=1e-16)),
+                inst(r=P.Resistor(name="r", p="out", n="vss", g=1e-6)), mos("m", "inp", "out", "vdd", "vdd", "pmos")]
+    else:  # mos0
+        body = [mos("p", "inp", "out", "vdd", "vdd", "pmos"), mos("n", "inp", "out", "vss", "vss", "nmos"),
+                inst(c=P.Capacitor(name="c", p="out", n="vss", c=1e-15))]
+    module = P.Module(name="inv", ports=["inp", "out", "vdd", "vss"], signals=[], comps=body)
+    c = P.Circuit(name="ro", signals=["1", "2", "3", "vdd"])
+    c.defs.append(P.Def(module=module))
+    if kind != "mos0":
+        c.defs.append(P.Def(mos1model=P.Mos1Model(name="default")))
+        c.defs.append(P.Def(mos1model=P.Mos1Model(name="nmos")))
+        c.defs.append(P.Def(mos1model=P.Mos1Model(name="pmos", mos_type=P.MosType.PMOS)))
+        c.defs.append(P.Def(mos1inst=P.Mos1InstParams(name="default")))
+    c.comps.append(inst(v=P.Vsrc(name="v1", p="vdd", n="", dc=1.0, acm=0.0)))
+    for k, (a, b) in enumerate([("1", "2"), ("2", "3"), ("3", "1")]):
+        x = P.ModuleInstance(name=f"x{k+1}", module="inv")
+        x.ports.update({"inp": a, "out": b, "vdd": "vdd", "vss": ""})
+        c.comps.append(inst(x=x))
+    return c
+
+
+MOS0 = {"default": 0, "nmos": 0, "pmos": 1}
+
+
+@pytest.mark.parametrize("kind", ["cmos", "nmos", "pmos", "mos0"])
+def test_elaboration_matches_hand_flattening(kind):
+    e = elaborate(ro3_proto(kind), mos0=MOS0 if kind == "mos0" else None)
+    h = ro3(kind)
+    assert e.var_names == h.names
+    assert e.var_kinds == h.kinds
+    assert e.net.n_comps == len(h.comps)
+    for k, (typ, nodes, *_rest) in enumerate(h.comps):
+        c = e.comps[k]
+        assert c.type == typ
+        assert list(c.node)[: 6] == nodes
+
+
+@pytest.mark.parametrize("fixture", sorted(GOLDEN_RUNS))
+def test_elaborated_circuit_reproduces_golden(fixture):
+    kind, tstep, tstop = GOLDEN_RUNS[fixture]
+    e = elaborate(ro3_proto(kind), mos0=MOS0 if kind == "mos0" else None)
+    g = json.load(open(os.path.join(GOLDEN, fixture)))
+    t, X, names, _ = O.tran(e, tstep, tstop, [("1", 0.0)])
+    for k, nm in enumerate(names):
+        assert np.array_equal(X[:, k], np.array(g[nm])), nm
+
+
+def test_hier1_counts():  # tests.rs:1378-1409: 11 components, 5 variables
+    good_luck = P.Module(name="good_luck", ports=["inp", "out", "vss"], signals=[],
+                         comps=[inst(r=P.Resistor(name="r1", p="inp", n="out", g=0.001)),
+                                inst(c=P.Capacitor(name="r2", p="out", n="vss", c=0.001))])
+    c = P.Circuit(name="tbd")
+    c.defs.append(P.Def(module=good_luck))
+    c.comps.extend([inst(r=P.Resistor(name="r0", p="inp", n="", g=0.001)),
+                    inst(r=P.Resistor(name="rt", p="out", n="out2", g=0.001)),
+                    inst(c=P.Capacitor(name="ct", p="out2", n="", c=0.001)),
+                    inst(c=P.Capacitor(name="ct", p="out3", n="", c=0.001)),
+                    inst(c=P.Capacitor(name="ct", p="out4", n="", c=0.001))])
+    for name, a, b in [("x1", "inp", "out"), ("x2", "out2", "out3"), ("x3", "out3", "out4")]:
+        x = P.ModuleInstance(name=name, module="good_luck")
+        x.ports.update({"inp": a, "out": b, "vss": ""})
+        c.comps.append(inst(x=x))
+    e = elaborate(c)
+    assert e.net.n_comps == 11
+    assert e.n_vars == 5
+
+
+def test_internal_nodes_and_branch_currents():
+    """Vsrc branch currents are named by instance path; Mos1 rd/rs add {path}.dp / {path}.sp (comps/mos.rs:
+    592-620); a diode with rs adds {path}.r (comps/diode.rs:100-110)."""
+    inner = P.Module(name="cell", ports=["a", "b"], signals=["mid"],
+                     comps=[inst(v=P.Vsrc(name="vs", p="a", n="mid", dc=0.1)),
+                            mos("m", "a", "mid", "b", "b", "rmos"),
+                            inst(d=P.Diode(name="d", p="mid", n="b", model="dm", params="dp"))])
+    c = circuit(P.Mos1Model(name="rmos", rd=P.Mos1Model().rd.__class__(value=10.0)),
+                P.Mos1InstParams(name="default"),
+                P.DiodeModel(name="dm", rs=P.DiodeModel().rs.__class__(value=2.0)), P.DiodeInstParams(name="dp"),
+                P.Vsrc(name="v1", p="top", n="", dc=1.0))
+    c.defs.append(P.Def(module=inner))
+    x = P.ModuleInstance(name="u0", module="cell")
+    x.ports.update({"a": "top", "b": ""})
+    c.comps.append(inst(x=x))
+    e = elaborate(c)
+    assert e.var_names == ["top", "v1", "u0.mid", "u0.vs", "u0.m.dp", "u0.d.r"]
+    assert e.var_kinds == [0, 1, 0, 1, 0, 0]
+
+
+def test_decode_errors():
+    with pytest.raises(S21Error, match="Decode Error"):
+        elaborate(b"\x0a\xff\xff\xff")
+    with pytest.raises(S21Error, match="Model not defined"):
+        elaborate(circuit(P.Mos(name="m", model="nope", params="x", ports=P.MosPorts(g="a", d="a"))))
+    from spice21_amd import api
+
+    with pytest.raises(S21Error, match="No Circuit Provided"):
+        api.dcop(P.Op())
