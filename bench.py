@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of the batched Newton inner loop on one MI355X.
+
+Workload: 4096 instances of a 64-stage CMOS inverter chain (Mos1, NMOS VTO 0.4 V / KP 100e-6, PMOS
+VTO -0.4 V / KP 50e-6, W/L 1u/0.1u and 2u/0.1u), input tied to 0.5 V, per-device VTO ~ N(base, 15 mV) and
+W ~ N(base, 3 %) drawn from seed 0xC2; one DC operating point per instance (68 unknowns, 128 Mos1).
+
+A step is one batched dcop of all 4096 instances from the zero initial guess, with the instances'
+parameters already resident in HBM: one persistent kernel launch that runs every instance's Newton loop
+to convergence.  The headline value is Newton instance-iterations per second of the mixed-precision
+path (FP32 factor and solves, FP64 evaluation, residual, update and convergence test); the plain FP64
+path is timed beside it, with the node-voltage / branch-current difference between the two.
+
+Extra objects on the JSON line:
+  roofline      algorithmic HBM bytes per launch over the kernel's HIP-event duration, against 8 TB/s
+  cpu_baseline  the C++ restatement of the reference algorithm (the oracle) over the same 4096
+                instances on host threads
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "newton_kernel_chain_mc4096_traffic.txt")
+
+
+def algorithmic_bytes(cnet, structure, factor_bytes):
+    """HBM bytes one Newton iteration of one instance must move at minimum:
+    sum over devices of (8 B x parameters + 4 B x terminal indices) + 8 B x Jacobian values assembled
+    + 2 x factor_bytes x LU entries (written by the factorisation, read by the solves) + 4 n-vectors
+    (x, rhs, residual, dx) of 8 B.  Shared topology tables are read once per launch and not counted."""
+    from spice21_amd import _abi as A
+
+    dev = 0
+    for k in range(cnet.net.n_comps):
+        c = cnet.comps[k]
+        if c.type == A.S21_COMP_MOS1:
+            terms = 4 + (c.node[4] != c.node[0]) + (c.node[5] != c.node[2])
+            dev += 8 * 10 + 4 * terms
+        elif c.type == A.S21_COMP_DIODE:
+            dev += 8 * 10 + 4 * (2 + (c.node[2] != c.node[0]))
+        elif c.type == A.S21_COMP_MOS0:
+            dev += 4 * 4
+        elif c.type == A.S21_COMP_V:
+            dev += 8 + 4 * 3
+        else:  # R, C, I
+            dev += 8 + 4 * 2
+    n = cnet.n_vars
+    return dev + 8 * structure["nnz_a"] + 2 * factor_bytes * structure["nnz_lu"] + 4 * 8 * n
+
+
+def measured_traffic():
+    """HBM bytes per launch of the mixed kernel on this workload, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size); None when absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib):
+    """Three launches of torch's device copy kernel over mib MiB: their FETCH_SIZE / WRITE_SIZE readings
+    per dispatch, against the known mib MiB, give the scale of those counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
+
+
+def time_batch(batch, steps, warmup):
+    for _ in range(warmup):
+        batch.op()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.op()  # returns once the launch has completed (HIP event synchronisation)
+        kernel_ms.append(batch.timing()[0])
+    elapsed = time.perf_counter() - t0
+    st, it = batch.status()
+    return elapsed, kernel_ms, st, it
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-inst", type=int, default=4096)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
+                    help="'mixed' / 'fp64' time one path only (for profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (reference reading for the HBM "
+                         "traffic counters) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the batched instances are independent, so a multi-GPU run would "
+              "shard them by instance with no exchange during the solve; run it with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from spice21_amd import _abi as A
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
+
+    nl, over = W.chain_mc(args.n_inst)
+    cn = nl.to_c()
+    group = Group(cn, args.n_inst, over)
+    kinds = np.array(nl.kinds)
+    vmask, imask = kinds == A.S21_VAR_V, kinds == A.S21_VAR_I
+
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        if args.precision not in ("both", name):
+            continue
+        b = Batch(group, precision=prec)
+        elapsed, kms, st, it = time_batch(b, args.steps, args.warmup)
+        res[name] = dict(batch=b, elapsed=elapsed, kernel_ms=kms, status=st, iters=it, x=b.x(),
+                         structure=b.structure())
+        print(f"[bench] {name}: {args.steps} steps in {elapsed:.4f} s, kernel {np.mean(kms):.3f} ms/launch, "
+              f"iterations/instance {it.min()}..{it.max()}, failed {int(np.sum(st != 0))}", file=sys.stderr,
+              flush=True)
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
+    inst_iters = int(h["iters"].sum())  # per step; every step repeats the same deterministic solve
+    value = inst_iters * args.steps / h["elapsed"]
+    kernel_avg_ms = float(np.mean(h["kernel_ms"]))
+    b_iter = algorithmic_bytes(cn, h["structure"], 4 if head == "mixed" else 8)
+    achieved = b_iter * inst_iters / (kernel_avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if head == "mixed" else None
+
+    line = {
+        "metric": "batched Newton instance-iters/s & tran pts/s, mixed vs FP64 at \u22641e-6 V error",
+        "value": value,
+        "unit": "instance-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factor+solve / f64 eval+residual" if head == "mixed" else "f64",
+        "data": "synthetic (seeded Monte Carlo over VTO and W)",
+        "config": {"workload": f"chain_mc: {args.n_inst} x 64-stage CMOS inverter chain dcop", "n_inst": args.n_inst,
+                   "unknowns": cn.n_vars, "mos1_per_instance": 128, "nnz_a": h["structure"]["nnz_a"],
+                   "nnz_lu": h["structure"]["nnz_lu"], "newton_iters_per_step": inst_iters,
+                   "failed_instances": int(np.sum(h["status"] != 0))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_inst_iter": b_iter, "kernel_avg_ms": kernel_avg_ms},
+    }
+    if "fp64" in res:
+        f = res["fp64"]
+        f_iters = int(f["iters"].sum())
+        line["fp64"] = {"value": f_iters * args.steps / f["elapsed"], "ms_per_step": f["elapsed"] / args.steps * 1e3,
+                        "kernel_avg_ms": float(np.mean(f["kernel_ms"])), "newton_iters_per_step": f_iters}
+    if "fp64" in res and "mixed" in res:
+        d = np.abs(res["mixed"]["x"] - res["fp64"]["x"])
+        line["mixed_vs_fp64"] = {"max_node_v": float(d[:, vmask].max()), "max_branch_i": float(d[:, imask].max()),
+                                 "speedup": value / line["fp64"]["value"]}
+
+    if not args.no_cpu:
+        from oracle import oracle_py as O  # the CPU restatement, timed as the baseline and used as the checker
+
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        xo, ito, sto = O.dcop_batch(group, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": float(ito.sum()) / cpu_s, "unit": "instance-iters/s", "cores": threads, "kind": "port",
+            "sample": f"all {args.n_inst} instances, one dcop each ({int(ito.sum())} Newton iterations, "
+                      f"{cpu_s:.2f} s wall on {threads} threads)"}
+        if "fp64" in res:
+            xf = res["fp64"]["x"]
+            rel = np.abs(xf - xo) / np.maximum(np.abs(xo), 1e-9)
+            line["fp64_vs_oracle"] = {"max_rel": float(rel.max()), "max_abs": float(np.abs(xf - xo).max()),
+                                      "iters_equal": bool(np.array_equal(ito, res["fp64"]["iters"]))}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -21,11 +21,13 @@
 #include "../include/spice21_mi355x.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -1611,6 +1613,68 @@ int oracle_dcop(const s21_netlist* net, const s21_options* o, double* x_out, int
         set_err(err, errlen, e.desc);
         return code_of(e.desc);
     }
+}
+
+/* One dcop per instance of a batch: instance i is the netlist with every override's vals[i] applied, a
+ * varied Mos1/Diode getting its own copy of its model or instance parameter set -- the per-instance
+ * circuit a batch group stands for.  Instances i0 .. i0+n_inst-1 run on n_threads host threads.
+ * x_out[n_inst * n_vars], iters_out and status_out[n_inst]. */
+int oracle_dcop_batch(const s21_netlist* net, const s21_options* o, int32_t n_over, const s21_override* over,
+                      int32_t i0, int32_t n_inst, int32_t n_threads, double* x_out, int64_t* iters_out,
+                      int32_t* status_out) {
+    if (!net || n_inst < 0 || n_threads < 1) return S21_INVALID;
+    const size_t nv = (size_t)net->n_vars;
+    std::atomic<int32_t> next(0);
+    auto work = [&]() {
+        std::vector<s21_comp> comps;
+        std::vector<s21_params> mm, mi, dm, di;
+        for (;;) {
+            const int32_t j = next.fetch_add(1);
+            if (j >= n_inst) return;
+            const int32_t i = i0 + j;
+            comps.assign(net->comps, net->comps + net->n_comps);
+            mm.assign(net->mos1_models, net->mos1_models + net->n_mos1_models);
+            mi.assign(net->mos1_insts, net->mos1_insts + net->n_mos1_insts);
+            dm.assign(net->diode_models, net->diode_models + net->n_diode_models);
+            di.assign(net->diode_insts, net->diode_insts + net->n_diode_insts);
+            for (int32_t k = 0; k < n_over; k++) {
+                const s21_override& ov = over[k];
+                s21_comp& c = comps[ov.comp];
+                if (ov.target == S21_OVR_VALUE) {
+                    c.value = ov.vals[i];
+                    continue;
+                }
+                const bool mos = c.type == S21_COMP_MOS1;
+                std::vector<s21_params>& lst =
+                    ov.target == S21_OVR_MODEL ? (mos ? mm : dm) : (mos ? mi : di);
+                int32_t& idx = ov.target == S21_OVR_MODEL ? c.model : c.inst;
+                s21_params p = lst[idx];
+                p.v[ov.field] = ov.vals[i];
+                p.set |= 1ull << ov.field;
+                lst.push_back(p);
+                idx = (int32_t)lst.size() - 1;
+            }
+            s21_netlist n2 = *net;
+            n2.comps = comps.data();
+            n2.mos1_models = mm.data();
+            n2.n_mos1_models = (int32_t)mm.size();
+            n2.mos1_insts = mi.data();
+            n2.n_mos1_insts = (int32_t)mi.size();
+            n2.diode_models = dm.data();
+            n2.n_diode_models = (int32_t)dm.size();
+            n2.diode_insts = di.data();
+            n2.n_diode_insts = (int32_t)di.size();
+            int64_t it = 0;
+            const int st = oracle_dcop(&n2, o, x_out + (size_t)j * nv, &it, nullptr, nullptr, 0);
+            if (iters_out) iters_out[j] = it;
+            if (status_out) status_out[j] = st;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int32_t t = 1; t < n_threads; t++) th.emplace_back(work);
+    work();
+    for (std::thread& t : th) t.join();
+    return S21_OK;
 }
 
 /* Transient (analysis.rs:489-574).  Writes up to max_pts points: t_out[p], x_out[p * n_total + k];

@@ -29,6 +29,8 @@ def lib():
         P = C.POINTER
         d, i32, i64 = C.c_double, C.c_int32, C.c_int64
         L.oracle_dcop.argtypes = [P(A.s21_netlist), P(A.s21_options), P(d), P(i64), P(i64), C.c_char_p, C.c_size_t]
+        L.oracle_dcop_batch.argtypes = [P(A.s21_netlist), P(A.s21_options), i32, P(A.s21_override), i32, i32, i32,
+                                        P(d), P(i64), P(i32)]
         L.oracle_dcop.restype = C.c_int
         L.oracle_tran.argtypes = [P(A.s21_netlist), P(A.s21_options), P(A.s21_tran_options), i64, P(d), P(d), P(i32),
                                   P(i64), C.c_char_p, C.c_size_t]
@@ -91,6 +93,22 @@ def dcop(cnet, opts=None):
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     return x[: cnet.n_vars], it.value, fa.value
+
+
+def dcop_batch(group, i0=0, n_inst=None, threads=1, opts=None):
+    """One dcop per instance of a batch group (``spice21_amd.batch.Group``: a netlist plus per-instance
+    overrides), on ``threads`` host threads.  Returns (x[n_inst, n_vars], iters[n_inst], status[n_inst])."""
+    n = group.n_inst - i0 if n_inst is None else int(n_inst)
+    nv = group.cnet.n_vars
+    x = np.zeros((max(n, 1), max(nv, 1)))
+    it = np.zeros(max(n, 1), dtype=np.int64)
+    st = np.zeros(max(n, 1), dtype=np.int32)
+    rc = lib().oracle_dcop_batch(group.cnet.ptr, C.byref(_opts(opts)), group.n_over, group.over, int(i0), n,
+                                 int(threads), _dp(x), it.ctypes.data_as(C.POINTER(C.c_int64)),
+                                 st.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise OracleError(rc, "oracle_dcop_batch: bad arguments")
+    return x[:n, :nv], it[:n], st[:n]
 
 
 def make_tran_options(tstep, tstop, ics=(), cnet=None):

@@ -698,6 +698,7 @@ struct s21_batch {
     std::vector<double> axis;
     std::unique_ptr<s21::DevMem> d_wave, d_groups;
     int last_mode = -1;
+    bool fetched = true;  // host copies of x / status / iterations are current
     ~s21_batch() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1205,7 +1206,10 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     b->kernel_avg_ms = b->launches ? total / b->launches : 0.0;
 }
 
+// Results stay on the device after an analysis; the first query copies them to the host.
 static void fetch_results(s21_batch* b) {
+    if (b->fetched) return;
+    b->fetched = true;
     for (GroupHost& g : b->groups) {
         const int NI = g.n_inst, n = g.topo.n;
         g.x.resize((size_t)NI * n);
@@ -1339,7 +1343,7 @@ int s21_batch_op(s21_batch* b) {
         prepare_all(b, nullptr);
         Args A = base_args(b, MODE_OP);
         run(b, A, nullptr, 1);
-        fetch_results(b);
+        b->fetched = false;
         b->npts = 0;
         b->axis.clear();
         b->last_mode = MODE_OP;
@@ -1363,7 +1367,7 @@ int s21_batch_tran(s21_batch* b, const s21_tran_options* tr) {
         A.tstop = tr->tstop;
         A.npts = npts;
         run(b, A, (double*)b->d_wave->p, npts);
-        fetch_results(b);
+        b->fetched = false;
         b->npts = npts;
         b->axis.resize(npts);
         double t = 0.0;
@@ -1404,6 +1408,7 @@ const char* s21_batch_var_name(const s21_batch* b, int32_t g, int32_t k) {
 int s21_batch_get_x(const s21_batch* b, int32_t g, double* out) {
     return guard([&] {
         if (!b || g < 0 || g >= (int)b->groups.size() || !out) fail(S21_INVALID, "bad arguments");
+        fetch_results(const_cast<s21_batch*>(b));
         const GroupHost& gh = b->groups[g];
         std::copy(gh.x.begin(), gh.x.end(), out);
     });
@@ -1411,6 +1416,7 @@ int s21_batch_get_x(const s21_batch* b, int32_t g, double* out) {
 int s21_batch_get_status(const s21_batch* b, int32_t g, int32_t* status, int64_t* iters) {
     return guard([&] {
         if (!b || g < 0 || g >= (int)b->groups.size()) fail(S21_INVALID, "bad arguments");
+        fetch_results(const_cast<s21_batch*>(b));
         const GroupHost& gh = b->groups[g];
         for (int i = 0; i < gh.n_inst && i < (int)gh.status.size(); i++) {
             if (status) status[i] = gh.status[i];
