@@ -13,8 +13,12 @@
 #ifndef SPICE21_MI355X_H
 #define SPICE21_MI355X_H
 
+#ifdef __HIPCC_RTC__
+#include <cstdint> /* runtime-compiled kernels see only the type and status declarations */
+#else
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -202,14 +206,27 @@ S21_API int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* laun
 S21_API int s21_symbolic(const s21_netlist* net, const s21_options* opts, const s21_tran_options* tran,
                          int32_t* row_i2e, int32_t* col_i2e, int64_t* nnz_a, int64_t* nnz_lu, int64_t* levels,
                          int64_t* flops_lu);
+/* Host only (no GPU): symbolic phase, then the generated factorisation of the frozen pattern compiled
+ * for gfx950 without loading it; reports its statements and code-object size. */
+S21_API int s21_jit_build(const s21_netlist* net, const s21_options* opts, const s21_tran_options* tran,
+                          int32_t precision, int64_t* n_ops, int64_t* code_bytes);
 /* Pivot order chosen by the symbolic phase of `group` for its last analysis: internal-to-
  * external row and column maps (AxisMapping.i2e, sparse21/mod.rs:126-162), each [n_vars]. */
 S21_API int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e, int32_t* col_i2e);
-/* Sizes of the frozen structure: nnz of A, nnz of L+U (incl. fill), LU levels, contributions. */
+/* Sizes of the frozen structure: nnz of A, nnz of L+U (incl. fill), LU levels, hub contributions. */
 S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_a, int64_t* nnz_lu,
                         int64_t* levels, int64_t* n_contrib);
+/* Launch shape (no counterpart in the reference, whose solver is single-threaded): instances per
+ * wavefront (1, 2, 4, 8 or 16; 0 = chosen per launch from the batch size), and whether the kernel
+ * runs a factorisation generated and compiled at run time for the group's frozen pattern (1 = always,
+ * -1 = never, 0 = for a single-pattern batch of at least 512 instances).  s21_batch_config reports the
+ * last launch: instances per wavefront, wavefronts per workgroup, generated factorisation used. */
+S21_API int s21_batch_set_ipw(s21_batch* b, int32_t ipw);
+S21_API int s21_batch_set_jit(s21_batch* b, int32_t mode);
+S21_API int s21_batch_config(const s21_batch* b, int32_t* ipw, int32_t* waves, int32_t* jit);
 /* Per-phase device cycle counters (s_memtime) of the Newton kernel, for profiling: while on, each
- * instance accumulates [eval, gather, residual, factor, sweeps, update, commit, total] cycles. */
+ * instance accumulates [device evaluation, hub gather, residual, factor (+ forward sweep), backward
+ * sweep, update, convergence/commit, total] cycles. */
 S21_API int s21_batch_set_profiling(s21_batch* b, int32_t on);
 S21_API int s21_batch_profile(const s21_batch* b, int32_t group, int64_t* out /* [n_inst][8] */);
 /* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the

@@ -159,7 +159,24 @@ class Batch:
         check(lib().s21_batch_set_profiling(self.h, 1 if on else 0))
         return self
 
-    PHASES = ("eval", "gather", "residual", "factor", "sweeps", "update", "commit", "total")
+    def set_ipw(self, ipw):
+        """Instances per wavefront (1, 2, 4, 8, 16), or 0 to let each launch choose."""
+        check(lib().s21_batch_set_ipw(self.h, int(ipw)))
+        return self
+
+    def set_jit(self, mode):
+        """Generated factorisation: 1 always, -1 never, 0 for large single-pattern batches."""
+        check(lib().s21_batch_set_jit(self.h, int(mode)))
+        return self
+
+    def config(self):
+        """(instances per wavefront, wavefronts per workgroup, generated factorisation used) of the last
+        launch."""
+        a, w, j = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().s21_batch_config(self.h, C.byref(a), C.byref(w), C.byref(j)))
+        return a.value, w.value, bool(j.value)
+
+    PHASES = ("devices", "hub", "residual", "factor", "backward", "update", "commit", "total")
 
     def profile(self, g=0):
         """Per-instance device cycles of the last analysis by phase: array [n_inst, 8]."""

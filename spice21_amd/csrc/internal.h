@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/spice21_mi355x.h"
+#include "layout.h"
 #include "models.h"
 
 namespace s21 {
@@ -38,6 +39,7 @@ struct Comp {
 };
 
 // ---- per-topology structure (topology.cpp) ----
+constexpr int HUB_CONTRIB = 4;
 struct Topology {
     int n = 0;
     std::vector<std::string> names;
@@ -66,26 +68,24 @@ struct Topology {
     std::vector<int> clvl_ptr, ch_row, ch_q0, ch_q1;
     std::vector<int> res_ptr, res_pos, res_xcol;  // residual rows (internal) over A entries
 
-    // devices by type, in component order within each type
-    std::vector<int> m1_comp, m0_comp, d_comp, lin_comp;
-    std::vector<int> m1_nodes;   // [6 per device]
-    std::vector<uint32_t> m1_mask;
-    std::vector<int> m1_base;
-    std::vector<uint8_t> m1_caps;  // any instance has gate capacitance
-    std::vector<int> m0_nodes;   // [4 per device]
-    std::vector<uint32_t> m0_mask;
-    std::vector<int> m0_base;
-    std::vector<double> m0_p;
-    std::vector<int> d_nodes;    // [3 per device]
-    std::vector<uint32_t> d_mask;
-    std::vector<int> d_base;
-    std::vector<int> lin_type, lin_nodes;  // [3 per device]
-    std::vector<uint32_t> lin_mask;
-    std::vector<int> lin_base;
+    // Devices in kernel order: classes Mos1, Mos0, Diode, linear (R/C/V/I); within a class by colour,
+    // then component order.  Devices of one colour share no matrix / RHS slot other than hub slots, so
+    // the kernel adds a colour's stamps straight into A and the RHS; a hub slot (one with more than
+    // HUB_CONTRIB contributing components) keeps every contribution apart and sums them in component
+    // order afterwards.
+    std::vector<int> dev_comp;            // component of each device position
+    int cls_start[N_CLS + 1] = {0};       // device positions of each class
+    std::vector<int> col_ptr, col_cls;    // colours: device-position ranges [col_ptr[c], col_ptr[c+1]) and class
+    std::vector<uint16_t> dev_rec;        // per device: node + 1 for six terminals, flags, 0
+    std::vector<uint16_t> codes;          // per device entry: slot in A|RHS, 0x8000 | hub index, or 0xFFFF
+    int code_base[N_CLS] = {0};           // first code of each class (stride code_stride(class))
+    int n_hub = 0;                        // hub contributions
+    std::vector<int> hub_slot;            // hub slots, ascending
+    std::vector<int> g_ptr, g_idx;        // per hub slot: its hub contributions in component-then-entry order
 
     // Kernel records (16-bit fields, packed in fours or pairs), built by topo_records:
-    //   gt {slot, k0, k1, part}   gather tasks over g_idx[k0, k1); part = 0xFFFF writes the slot itself
-    //   gc {slot, p0, p1, 0}      sums of the partials of a split slot
+    //   gt {slot, k0, k1, part}   hub gather tasks over g_idx[k0, k1); part = 0xFFFF writes the slot
+    //   gc {slot, p0, p1, 0}      sums of the partials of a split hub slot
     //   rt {row, e0, e1, part}    residual tasks over re[e0, e1); re {LU position, x column}
     //   rc {row, p0, p1, 0}       sums of the partials of a split residual row
     //   fch {row, l0, l1, u0}     factorisation chunks, level order (clvl_ptr)
@@ -94,11 +94,8 @@ struct Topology {
     std::vector<uint16_t> gt, gc, rt, re, rc, fch, fl, fu, bro, bu;
     int n_part = 0;  // partial-sum slots needed by the split gather / residual tasks
 
-    int n_contrib = 0;
     int sym_status = S21_OK;  // failure of the pivot search, reported when an instance needs a solve
     std::string sym_error;
-    std::vector<int> g_ptr, g_idx;  // gather lists for nnz LU slots followed by n RHS slots
-    std::vector<int> contrib_comp;  // component of each contribution (introspection)
     int64_t flops_lu = 0;
 };
 

@@ -1,0 +1,25 @@
+// Run-time compiled Newton kernels for one frozen pattern (jit.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace s21 {
+
+struct JitKernel {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+};
+// Statements of the generated factorisation and sweeps (its size).
+int64_t jit_ops(const Topology& t);
+// Source of the kernel with the factorisation of t written out, for FP64 or mixed precision.
+std::string jit_source(const Topology& t, bool mixed);
+// Code object of a source for gfx950 (no device needed).
+std::vector<char> jit_code(const std::string& src);
+// Compiled kernel for a source (compiled once per process and source).
+JitKernel jit_compile(const std::string& src);
+
+}  // namespace s21

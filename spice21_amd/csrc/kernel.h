@@ -1,0 +1,696 @@
+// Batched Newton kernel for MI355X (gfx950): device code shared by the precompiled kernels and the
+// kernels compiled at run time with a factorisation generated for one frozen pattern.
+//
+// One persistent launch runs every instance's whole analysis: the operating point and, for a transient,
+// every time step.  A workgroup holds up to MAX_WAVES wavefronts; each wavefront holds `ipw` instances
+// of one group, each owning L = 64 / ipw consecutive lanes and one LDS region:
+//     x[n] | c[n] | a[nnz] | rhs[n] | hub[n_hub] | part[n_part] (| lu32[nnz] in mixed precision)
+// next to the workgroup's shared copy of the group's records.  Each Newton iteration (Solver::solve,
+// analysis.rs:169-210) runs these phases:
+//   1. device evaluation by colour classes: the devices of one colour share no matrix / RHS slot other
+//      than hub slots, so each lane evaluates its devices and adds their stamps straight into A and the
+//      RHS without atomics; stamps into a hub slot go to their own contribution cell;
+//   2. hub gather: each hub slot sums its contributions in component-then-stamp order, the order
+//      Solver::update adds them (analysis.rs:153-168);
+//   3. KCL residual res = rhs - A x in FP64 over the pivoted rows (sparse21/mod.rs:298-327) and the
+//      convergence test on the previous update and the residual (analysis.rs:331-345);
+//   4. numeric LU on the pivot order and fill pattern frozen by the host symbolic phase, level-scheduled
+//      by rows (IKJ order, which applies every update to an entry in the same pivot order as the
+//      reference's right-looking elimination, sparse21/mod.rs:865-919), with the forward sweep fused in;
+//   5. level-scheduled backward sweep (sparse21/mod.rs:966-979);
+//   6. the max-|dx| <= 1 clamp and the update x += dx (analysis.rs:196-207).
+// In mixed precision phases 4-5 run in FP32 while 1-3 and 6 stay FP64, so Newton iterates as
+// iterative refinement.  Instances of one wavefront that finish early leave their lanes idle.
+#pragma once
+#ifndef __HIPCC_RTC__  // run-time compilation provides the HIP declarations itself
+#include <hip/hip_runtime.h>
+#endif
+
+#include "layout.h"
+#include "spice21_mi355x.h"
+
+namespace s21 {
+
+// ============================================================================================
+// Device-side descriptors
+// ============================================================================================
+struct TopoGPU {
+    int n, nnz, n_hub, n_part;
+    int n_cls[N_CLS];  // devices per class (Mos1, Mos0, Diode, linear)
+    int cls_start[N_CLS + 1];
+    int n_col;
+    int n_gt, n_gc, n_rt, n_rc, n_clvl, n_blvl;
+    int sym_status;
+    int blob_vec;       // 16-byte vectors of the record blob copied into LDS at launch
+    const uint4* blob;  // the blob in global memory
+    // offsets (uint16 units, 8-byte aligned) of the record tables inside the blob
+    int o_gt, o_gidx, o_gc, o_rt, o_re, o_rc, o_ri2e, o_ce2i, o_diag, o_clvl, o_fch, o_fl, o_fu, o_blvl, o_bro, o_bu;
+    int o_col;   // colours: n_col + 1 device-position starts, then {class, class-local first index, first code}
+    int o_dev;   // per device: node + 1 for six terminals, flags, 0
+    int o_code;  // per device entry: slot, 0x8000 | hub index, or 0xFFFF
+};
+
+struct GroupGPU {
+    TopoGPU t;
+    int n_inst, blk0;
+    const double* m1_core;  // [11][n_inst][n_m1]   (class-local device order)
+    const double* m1_caps;  // [4][n_inst][n_m1]
+    double* m1_state;       // [14][n_inst][n_m1]: committed op (7) then guess (7)
+    const double* d_par;    // [14][n_inst][n_d]
+    double* d_state;        // [3][n_inst][n_d]: guess vd, guess charge, committed charge
+    const double* lin_val;  // [n_inst][n_lin] operating-point values
+    const double* lin_val_tran;
+    double* c_q;            // [n_inst][n_lin] committed capacitor charge
+    double* x;              // [n_inst][n]
+    int* status;            // [n_inst]
+    long long* iters;       // [n_inst]
+    double* wave;           // [n_inst][npts][n] (transient) or nullptr
+    double* dbg;            // assembly dump [nnz + n] (assemble mode)
+    long long* prof;        // [n_inst][8] phase cycle counters (profiling mode) or nullptr
+};
+
+enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2 };
+struct Args {
+    int mode;
+    int max_iter;
+    double gmin, iabstol, reltol;
+    double tstep, tstop;
+    long long npts;
+    int dbg_inst;
+    int prof;
+    int waves;       // wavefronts per workgroup
+    int ipw;         // instances per wavefront (1, 2, 4, 8 or 16)
+    int inst_bytes;  // LDS bytes per instance
+};
+
+// Address spaces: pointers read out of descriptors carry no address space, so they are re-qualified in
+// device code to keep LDS accesses on ds_* and global accesses on global_* instructions.
+#define LDS __attribute__((address_space(3)))
+#define GLB __attribute__((address_space(1)))
+template <typename T>
+__device__ inline GLB T* glb(T* p) {
+    return (GLB T*)p;
+}
+
+constexpr int WAVE = 64;
+constexpr int MAX_WAVES = 4;  // wavefronts per workgroup; the kernel's launch bound
+constexpr size_t LDS_PER_CU = 160 * 1024;  // gfx950
+constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 3;
+constexpr int NPROF = 8;  // eval, gather, residual, factor (+forward sweep), backward sweep, update, commit, total
+constexpr int RUNNING = -1;
+
+using u16 = unsigned short;
+
+// Orders LDS accesses between the lanes of one wavefront (a wavefront executes its LDS operations in
+// order; the fences keep the compiler from moving accesses across this point).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// LDS views of the record tables.
+struct Idx {
+    LDS const uint64_t *gt, *gc, *rt, *rc, *fch, *fl, *bro;  // four 16-bit fields
+    LDS const uint32_t *re, *fu, *bu;                         // two 16-bit fields
+    LDS const u16 *gidx, *ri2e, *ce2i, *diag, *clvl, *blvl, *col, *dev, *code;
+};
+struct R4 {
+    int x, y, z, w;
+};
+__device__ inline R4 r4(uint64_t v) {
+    return R4{(int)(v & 0xFFFF), (int)((v >> 16) & 0xFFFF), (int)((v >> 32) & 0xFFFF), (int)(v >> 48)};
+}
+struct R2 {
+    int x, y;
+};
+__device__ inline R2 r2(uint32_t v) { return R2{(int)(v & 0xFFFF), (int)(v >> 16)}; }
+
+struct Ctx {
+    const GroupGPU* G;
+    TopoGPU T;
+    Idx I;
+    int inst;       // instance of this lane's segment
+    int sl, L;      // lane within the segment, lanes per segment
+    int seg0;       // first lane of the segment
+    LDS double *x, *c, *a, *rhs, *hub, *part;
+    LDS float* lu32;
+};
+
+// Ballot restricted to the lane's segment.
+__device__ inline bool seg_any(const Ctx& C, bool p) {
+    const uint64_t b = __ballot(p);
+    if (C.L == WAVE) return b != 0ull;
+    return ((b >> C.seg0) & ((1ull << C.L) - 1ull)) != 0ull;
+}
+// Maximum over the lane's segment (every lane of the wavefront calls it).
+__device__ inline double seg_max(const Ctx& C, double v) {
+    for (int o = C.L >> 1; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, WAVE);
+        v = (w > v) ? w : v;  // the reference's fold drops NaN (analysis.rs:198)
+    }
+    return v;
+}
+
+__device__ inline double xv(LDS const double* x, int v) { return v < 0 ? 0.0 : x[v]; }
+
+// Sink of a device's stamps: entry e goes to its slot of A|RHS (added in order) or to its hub cell.
+struct Put {
+    LDS double* ar;
+    LDS double* hub;
+    LDS const u16* code;
+    __device__ void operator()(int e, double v) const {
+        const int c = code[e];
+        if (c == 0xFFFF) return;
+        if (c & 0x8000)
+            hub[c & 0x7FFF] = v;
+        else
+            ar[c] = ar[c] + v;
+    }
+};
+
+// A device's stamps collected in registers (entry index fixed at compile time), then applied in one batch.
+// The host makes every slot that one device stamps twice a hub slot, so a device's non-hub entries reach
+// distinct slots and all of its slot loads may precede all of its stores.
+template <int N>
+struct Stamps {
+    double v[N];
+    __device__ void operator()(int e, double x) { v[e] = x; }
+};
+struct M1Stamps {
+    static constexpr bool raw_stamps = true;  // Mos1: reference stamp order, swap applied at the codes
+    double v[MOS1_NE];
+    bool rev;
+    __device__ void stamp(int k, bool r, double x) {
+        v[k] = x;
+        rev = r;
+    }
+};
+template <int N, bool SIGMA>
+__device__ inline void apply(const Ctx& C, LDS const u16* code, const double (&v)[N], bool rev) {
+    constexpr int B = 8;  // entries per batch
+#pragma unroll
+    for (int k0 = 0; k0 < N; k0 += B) {
+        int c[B];
+        double cur[B];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) c[k] = code[(SIGMA && rev) ? mos1_sigma(k0 + k) : k0 + k];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) cur[k] = c[k] < 0x8000 ? C.a[c[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            if (k0 + k >= N) continue;
+            if (c[k] < 0x8000)
+                C.a[c[k]] = cur[k] + v[k0 + k];
+            else if (c[k] != 0xFFFF)
+                C.hub[c[k] & 0x7FFF] = v[k0 + k];
+        }
+    }
+}
+
+// ---- device evaluation ---------------------------------------------------------------------------
+__device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
+    const GroupGPU& G = *C.G;
+    const size_t s = (size_t)G.n_inst * C.T.n_cls[CLS_MOS1], o = (size_t)C.inst * C.T.n_cls[CLS_MOS1] + j;
+    GLB const double* p = glb(G.m1_core);
+    Mos1Core P;
+    P.p = p[0 * s + o];
+    P.vt0_t = p[1 * s + o];
+    P.beta = p[2 * s + o];
+    P.phi_t = p[3 * s + o];
+    P.gamma = p[4 * s + o];
+    P.lambda = p[5 * s + o];
+    P.vtherm = p[6 * s + o];
+    P.isat_d = p[7 * s + o];
+    P.isat_s = p[8 * s + o];
+    P.grd = p[9 * s + o];
+    P.grs = p[10 * s + o];
+    const double vd = xv(C.x, (int)dv[RD] - 1), vg = xv(C.x, (int)dv[RG] - 1), vs = xv(C.x, (int)dv[RS] - 1),
+                 vb = xv(C.x, (int)dv[RB] - 1);
+    M1Stamps sv;
+    if (!(dv[6] & DEV_CAPS)) {
+        mos1_load<false>(P, nullptr, nullptr, vd, vg, vs, vb, A.gmin, A.tstep, sv, nullptr);
+        apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
+        return;
+    }
+    // gate capacitance: capacitances and operating-point state from global memory
+    Mos1Caps K;
+    GLB const double* kc = glb(G.m1_caps);
+    K.cox = kc[0 * s + o];
+    K.cgs_ov = kc[1 * s + o];
+    K.cgd_ov = kc[2 * s + o];
+    K.cgb_ov = kc[3 * s + o];
+    Mos1State op, gs;
+    GLB double* st = glb(G.m1_state);
+    op.vgs = st[0 * s + o];
+    op.vgd = st[1 * s + o];
+    op.vgb = st[2 * s + o];
+    op.cgs = st[3 * s + o];
+    op.cgd = st[4 * s + o];
+    op.cgb = st[5 * s + o];
+    op.reversed = st[6 * s + o];
+    if (tran)
+        mos1_load<true>(P, &K, &op, vd, vg, vs, vb, A.gmin, A.tstep, sv, &gs);
+    else
+        mos1_load<false>(P, &K, &op, vd, vg, vs, vb, A.gmin, A.tstep, sv, &gs);
+    apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
+    st[7 * s + o] = gs.vgs;
+    st[8 * s + o] = gs.vgd;
+    st[9 * s + o] = gs.vgb;
+    st[10 * s + o] = gs.cgs;
+    st[11 * s + o] = gs.cgd;
+    st[12 * s + o] = gs.cgb;
+    st[13 * s + o] = gs.reversed;
+}
+
+__device__ inline void eval_diode(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran,
+                                  const Args& A) {
+    const GroupGPU& G = *C.G;
+    const size_t s = (size_t)G.n_inst * C.T.n_cls[CLS_DIODE], o = (size_t)C.inst * C.T.n_cls[CLS_DIODE] + j;
+    GLB const double* q = glb(G.d_par);
+    DiodeP P;
+    P.isat = q[0 * s + o];
+    P.vte = q[1 * s + o];
+    P.vcrit = q[2 * s + o];
+    P.gspr = q[3 * s + o];
+    P.bv = q[4 * s + o];
+    P.has_bv = q[5 * s + o];
+    P.cz = q[6 * s + o];
+    P.cz2 = q[7 * s + o];
+    P.dep = q[8 * s + o];
+    P.f1 = q[9 * s + o];
+    P.f3 = q[10 * s + o];
+    P.tt = q[11 * s + o];
+    P.vj = q[12 * s + o];
+    P.m = q[13 * s + o];
+    GLB double* st = glb(G.d_state);
+    const double vr = xv(C.x, (int)dv[DR_] - 1), vn = xv(C.x, (int)dv[DN_] - 1);
+    DiodeGuess gs;
+    Stamps<DIODE_NE> sv;
+    if (tran)
+        diode_load<true>(P, vr, vn, A.gmin, A.tstep, st[0 * s + o], st[2 * s + o], sv, &gs);
+    else
+        diode_load<false>(P, vr, vn, A.gmin, A.tstep, st[0 * s + o], st[2 * s + o], sv, &gs);
+    apply<DIODE_NE, false>(C, code, sv.v, false);
+    st[0 * s + o] = gs.vd;
+    st[1 * s + o] = gs.charge;
+}
+
+__device__ inline void eval_lin(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
+    const GroupGPU& G = *C.G;
+    Stamps<6> put;
+    const int typ = dv[6] >> 4;
+    const size_t o = (size_t)C.inst * C.T.n_cls[CLS_LIN] + j;
+    const double val = tran ? glb(G.lin_val_tran)[o] : glb(G.lin_val)[o];
+    if (typ == S21_COMP_R) {
+        put(0, val);
+        put(1, val);
+        put(2, -val);
+        put(3, -val);
+    } else if (typ == S21_COMP_C) {
+        if (!tran) return;  // no stamps in the operating point (comps/mod.rs:205-209)
+        cap_load_tran(val, xv(C.x, (int)dv[0] - 1) - xv(C.x, (int)dv[1] - 1), glb(G.c_q)[o], A.tstep, put);
+    } else if (typ == S21_COMP_V) {
+        put(0, 1.0);
+        put(1, 1.0);
+        put(2, -1.0);
+        put(3, -1.0);
+        put(4, val);
+    } else {
+        put(0, val);
+        put(1, -val);
+    }
+    apply<6, false>(C, code, put.v, false);
+}
+
+__device__ inline long long cycles() { return (long long)__builtin_readcyclecounter(); }
+
+// Assembly of A and the RHS at the lane segment's x: colours in order, then the hub gather.  With
+// profiling on, pc[0] gets the device evaluation and pc[1] the hub gather.
+__device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A, long long* pc = nullptr) {
+    const long long t0 = pc ? cycles() : 0;
+    const TopoGPU& T = C.T;
+    const Idx& I = C.I;
+    const int ns = T.nnz + T.n;
+    for (int k = act ? C.sl : ns; k < ns; k += C.L) C.a[k] = 0.0;
+    wave_sync();
+    for (int cc = 0; cc < T.n_col; cc++) {
+        const int p0 = I.col[cc], p1 = I.col[cc + 1];
+        LDS const u16* cd = I.col + T.n_col + 1 + 3 * cc;
+        const int cls = cd[0], j0 = cd[1], code0 = cd[2];
+        const int stride = code_stride(cls);
+        for (int p = act ? p0 + C.sl : p1; p < p1; p += C.L) {
+            LDS const u16* dv = I.dev + 8 * p;
+            LDS const u16* code = I.code + code0 + (p - p0) * stride;
+            const int j = j0 + (p - p0);
+            if (cls == CLS_MOS1)
+                eval_mos1(C, j, dv, code, tran, A);
+            else if (cls == CLS_MOS0)
+                mos0_load((dv[6] & DEV_PMOS) ? -1.0 : 1.0, xv(C.x, (int)dv[RD] - 1), xv(C.x, (int)dv[RG] - 1),
+                          xv(C.x, (int)dv[RS] - 1), A.gmin, Put{C.a, C.hub, code});
+            else if (cls == CLS_DIODE)
+                eval_diode(C, j, dv, code, tran, A);
+            else
+                eval_lin(C, j, dv, code, tran, A);
+        }
+        wave_sync();
+    }
+    const long long t1 = pc ? cycles() : 0;
+    // hub slots: ordered sums of their contributions (long slots in chunks whose partials add in order)
+    for (int t = act ? C.sl : T.n_gt; t < T.n_gt; t += C.L) {
+        const R4 r = r4(I.gt[t]);
+        double acc = 0.0;
+        for (int k = r.y; k < r.z; k++) acc = acc + C.hub[I.gidx[k]];
+        if (r.w == 0xFFFF)
+            C.a[r.x] = acc;
+        else
+            C.part[r.w] = acc;
+    }
+    wave_sync();
+    for (int t = act ? C.sl : T.n_gc; t < T.n_gc; t += C.L) {
+        const R4 r = r4(I.gc[t]);
+        double acc = C.part[r.y];
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + C.part[p];
+        C.a[r.x] = acc;
+    }
+    if (pc) {
+        wave_sync();
+        const long long t2 = cycles();
+        pc[0] += t1 - t0;
+        pc[1] += t2 - t1;
+    }
+}
+
+// c[i] = rhs[row_i2e[i]] - (A x)[i] over internal rows (sparse21/mod.rs:298-327).  Returns whether this
+// lane saw |res| > iabstol.
+__device__ inline bool residual(const Ctx& C, bool act, double iabstol) {
+    const Idx& I = C.I;
+    bool bad = false;
+    for (int t = act ? C.sl : C.T.n_rt; t < C.T.n_rt; t += C.L) {
+        const R4 r = r4(I.rt[t]);
+        double acc = 0.0;
+        for (int e = r.y; e < r.z; e++) {
+            const R2 q = r2(I.re[e]);
+            acc = acc + C.a[q.x] * C.x[q.y];
+        }
+        if (r.w == 0xFFFF) {
+            const double v = C.rhs[I.ri2e[r.x]] - acc;
+            C.c[r.x] = v;
+            bad = bad || (fabs(v) > iabstol);
+        } else {
+            C.part[r.w] = acc;
+        }
+    }
+    wave_sync();
+    for (int t = act ? C.sl : C.T.n_rc; t < C.T.n_rc; t += C.L) {
+        const R4 r = r4(I.rc[t]);
+        double acc = C.part[r.y];
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + C.part[p];
+        const double v = C.rhs[I.ri2e[r.x]] - acc;
+        C.c[r.x] = v;
+        bad = bad || (fabs(v) > iabstol);
+    }
+    return bad;
+}
+
+// Factorisation with the forward sweep fused in.  A chunk of row i processes L entries (i, k) once row k
+// is final: l_ik = a_ik / u_kk, a_ij -= l_ik u_kj over row k's U entries, and c_i -= c_k l_ik -- for each
+// target in increasing k, the order of the reference's right-looking elimination and column-oriented
+// forward substitution (sparse21/mod.rs:865-919, 946-964).  Returns whether this lane saw a zero pivot
+// other than the last (the assertion at sparse21/mod.rs:872).
+template <typename V>
+__device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_clvl; l++) {
+        const int e = I.clvl[l + 1];
+        for (int ch = act ? I.clvl[l] + C.sl : e; ch < e; ch += C.L) {
+            const R4 h = r4(I.fch[ch]);
+            V y = (V)C.c[h.x];
+            int u = h.w;
+            for (int j = h.y; j < h.z; j++) {
+                const R4 L = r4(I.fl[j]);
+                const V lv = lu[L.x] / lu[L.y];
+                lu[L.x] = lv;
+                const V ck = (V)C.c[L.z];
+                if (ck != V(0)) y = y - ck * lv;  // the sweep skips zero entries (sparse21/mod.rs:953)
+                for (int m = 0; m < L.w; m++, u++) {
+                    const R2 q = r2(I.fu[u]);
+                    const V prod = lu[q.x] * lv;
+                    lu[q.y] = lu[q.y] - prod;
+                }
+            }
+            C.c[h.x] = (double)y;
+        }
+        wave_sync();
+    }
+    bool zero_pivot = false;
+    for (int i = act ? C.sl : C.T.n; i < C.T.n - 1; i += C.L)
+        if (lu[I.diag[i]] == V(0)) zero_pivot = true;
+    return zero_pivot;
+}
+
+// Backward sweep over U by row levels (sparse21/mod.rs:966-979).
+template <typename V>
+__device__ inline void backward(const Ctx& C, bool act, LDS const V* lu) {
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_blvl; l++) {
+        const int e = I.blvl[l + 1];
+        for (int r = act ? I.blvl[l] + C.sl : e; r < e; r += C.L) {
+            const R4 h = r4(I.bro[r]);
+            V y = (V)C.c[h.x];
+            for (int k = h.z; k < h.w; k++) {
+                const R2 q = r2(I.bu[k]);
+                const V prod = (V)C.c[q.y] * lu[q.x];
+                y = y - prod;
+            }
+            C.c[h.x] = (double)(y / lu[h.y]);
+        }
+        wave_sync();
+    }
+}
+
+// Commit the accepted operating point (Component::commit) of every stateful device of the segment.
+__device__ inline void commit(const Ctx& C, bool act) {
+    const GroupGPU& G = *C.G;
+    const TopoGPU& T = C.T;
+    const int nl = T.n_cls[CLS_LIN], nd = T.n_cls[CLS_DIODE], nm = T.n_cls[CLS_MOS1];
+    for (int d = act ? C.sl : nl; d < nl; d += C.L) {
+        LDS const u16* dv = C.I.dev + 8 * (T.cls_start[CLS_LIN] + d);
+        if ((dv[6] >> 4) != S21_COMP_C) continue;
+        const size_t o = (size_t)C.inst * nl + d;
+        const double vd = xv(C.x, (int)dv[0] - 1) - xv(C.x, (int)dv[1] - 1);
+        glb(G.c_q)[o] = glb(G.lin_val)[o] * vd;  // guess.q = c * vd at the accepted voltages
+    }
+    for (int d = act ? C.sl : nd; d < nd; d += C.L) {
+        const size_t s = (size_t)G.n_inst * nd, o = (size_t)C.inst * nd + d;
+        glb(G.d_state)[2 * s + o] = glb(G.d_state)[1 * s + o];
+    }
+    for (int d = act ? C.sl : nm; d < nm; d += C.L) {
+        LDS const u16* dv = C.I.dev + 8 * (T.cls_start[CLS_MOS1] + d);
+        if (!(dv[6] & DEV_CAPS)) continue;
+        const size_t s = (size_t)G.n_inst * nm, o = (size_t)C.inst * nm + d;
+        for (int f = 0; f < M1_STATE_F; f++) glb(G.m1_state)[f * s + o] = glb(G.m1_state)[(M1_STATE_F + f) * s + o];
+    }
+}
+
+// One Newton solve (Solver::solve, analysis.rs:169-210) of every active segment of the wavefront.
+// `iters` accumulates loads.  Returns the segment's S21_* status (S21_OK for a lane that was inactive).
+// The LU policy supplies factor<V>(C, act, lu) -> zero pivot seen, and backward<V>(C, act, lu).
+struct LuLevels {
+    template <typename V>
+    static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {
+        return factor_forward<V>(C, act, lu);
+    }
+    template <typename V>
+    static __device__ void backward(const Ctx& C, bool act, LDS const V* lu) {
+        s21::backward<V>(C, act, lu);
+    }
+};
+
+template <bool MIXED, class LU>
+__device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc) {
+    const TopoGPU& T = C.T;
+    const bool prof = A.prof != 0;
+    long long t0 = 0, t1 = 0;
+    int st = act ? RUNNING : S21_OK;
+    bool prev_ok = true;  // dx starts at zero
+    for (int it = 0; it < A.max_iter; it++) {
+        if (!wave_any(act)) break;
+        if (act) iters++;
+        assemble(C, act, tran, A, prof ? pc : nullptr);
+        wave_sync();
+        if (prof) t0 = cycles();
+        const bool bad = seg_any(C, residual(C, act, A.iabstol));
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
+        const bool conv = act && prev_ok && !bad;
+        if (wave_any(conv)) {
+            commit(C, conv);
+            wave_sync();
+        }
+        if (conv) {
+            st = S21_OK;
+            act = false;
+        }
+        if (act && T.sym_status != S21_OK) {
+            st = T.sym_status;
+            act = false;
+        }
+        if (prof) { t1 = cycles(); pc[6] += t1 - t0; t0 = t1; }
+        if (!wave_any(act)) break;
+        bool zero;
+        if (MIXED) {
+            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) C.lu32[p] = (float)C.a[p];
+            wave_sync();
+            zero = seg_any(C, LU::template factor<float>(C, act, C.lu32));
+        } else {
+            zero = seg_any(C, LU::template factor<double>(C, act, C.a));
+        }
+        if (act && zero) {
+            st = S21_PIVOT;
+            act = false;
+        }
+        if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+        if (MIXED)
+            LU::template backward<float>(C, act, C.lu32);
+        else
+            LU::template backward<double>(C, act, C.a);
+        if (prof) { t1 = cycles(); pc[4] += t1 - t0; t0 = t1; }
+        // clamp and update (analysis.rs:196-207)
+        double mx = 0.0;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            const double dx = fabs(C.c[C.I.ce2i[k]]);
+            mx = (dx > mx) ? dx : mx;
+        }
+        mx = seg_max(C, mx);
+        bool ok = true;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            double dx = C.c[C.I.ce2i[k]];
+            if (mx > 1.0) dx = dx * 1.0 / mx;
+            ok = ok && !(fabs(dx) > A.reltol);
+            C.x[k] = C.x[k] + dx;
+        }
+        prev_ok = !seg_any(C, !ok);
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[5] += t1 - t0; }
+    }
+    return st == RUNNING ? S21_CONVERGENCE : st;
+}
+
+// Workgroup = A.waves wavefronts x A.ipw instances, sharing one LDS copy of the group's records.
+template <bool MIXED, class LU>
+__device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_groups, const Args& A) {
+    int g = 0;
+    const int b = blockIdx.x;
+    while (g + 1 < n_groups && groups[g + 1].blk0 <= b) g++;
+    const GroupGPU& G = groups[g];
+    Ctx C;
+    C.G = &G;
+    C.T = G.t;
+    const TopoGPU& T = C.T;
+    extern __shared__ uint4 smem_u4[];
+    {  // records -> LDS (whole workgroup)
+        GLB const uint64_t* src = (GLB const uint64_t*)T.blob;
+        LDS uint64_t* dst = (LDS uint64_t*)smem_u4;
+        for (int k = threadIdx.x; k < 2 * T.blob_vec; k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int lane = threadIdx.x % WAVE;
+    C.L = WAVE / A.ipw;
+    const int sub = lane / C.L;
+    C.sl = lane % C.L;
+    C.seg0 = sub * C.L;
+    const int slot = w * A.ipw + sub;  // instance slot within the workgroup
+    int inst;
+    bool valid;
+    if (A.mode == MODE_ASSEMBLE) {
+        inst = A.dbg_inst;
+        valid = slot == 0;
+    } else {
+        inst = (b - G.blk0) * A.waves * A.ipw + slot;
+        valid = inst < G.n_inst;
+    }
+    if (!wave_any(valid)) return;
+    if (!valid) inst = 0;  // idle lanes touch no instance data
+    C.inst = inst;
+    LDS const u16* b16 = (LDS const u16*)smem_u4;
+    C.I.gt = (LDS const uint64_t*)(b16 + T.o_gt);
+    C.I.gidx = b16 + T.o_gidx;
+    C.I.gc = (LDS const uint64_t*)(b16 + T.o_gc);
+    C.I.rt = (LDS const uint64_t*)(b16 + T.o_rt);
+    C.I.re = (LDS const uint32_t*)(b16 + T.o_re);
+    C.I.rc = (LDS const uint64_t*)(b16 + T.o_rc);
+    C.I.ri2e = b16 + T.o_ri2e;
+    C.I.ce2i = b16 + T.o_ce2i;
+    C.I.diag = b16 + T.o_diag;
+    C.I.clvl = b16 + T.o_clvl;
+    C.I.fch = (LDS const uint64_t*)(b16 + T.o_fch);
+    C.I.fl = (LDS const uint64_t*)(b16 + T.o_fl);
+    C.I.fu = (LDS const uint32_t*)(b16 + T.o_fu);
+    C.I.blvl = b16 + T.o_blvl;
+    C.I.bro = (LDS const uint64_t*)(b16 + T.o_bro);
+    C.I.bu = (LDS const uint32_t*)(b16 + T.o_bu);
+    C.I.col = b16 + T.o_col;
+    C.I.dev = b16 + T.o_dev;
+    C.I.code = b16 + T.o_code;
+    LDS double* dbase = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) + (size_t)slot * A.inst_bytes);
+    C.x = dbase;
+    C.c = C.x + T.n;
+    C.a = C.c + T.n;
+    C.rhs = C.a + T.nnz;
+    C.hub = C.rhs + T.n;
+    C.part = C.hub + T.n_hub;
+    C.lu32 = (LDS float*)(C.part + T.n_part);
+
+    for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) C.x[k] = G.x[(size_t)inst * T.n + k];
+    wave_sync();
+
+    if (A.mode == MODE_ASSEMBLE) {
+        assemble(C, valid, false, A);
+        wave_sync();
+        for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
+        return;
+    }
+
+    long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const long long tstart = A.prof ? cycles() : 0;
+    long long iters = 0;
+    int st = newton<MIXED, LU>(C, valid, false, A, iters, pc);
+    if (A.mode == MODE_TRAN) {
+        bool alive = valid && st == S21_OK;
+        double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
+        for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
+        for (long long pt = 1; pt < A.npts; pt++) {  // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
+            if (!wave_any(alive)) break;
+            const int s2 = newton<MIXED, LU>(C, alive, true, A, iters, pc);
+            if (alive) {
+                st = s2;
+                alive = s2 == S21_OK;
+            }
+            for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
+        }
+    }
+    if (!valid) return;
+    for (int k = C.sl; k < T.n; k += C.L) G.x[(size_t)inst * T.n + k] = C.x[k];
+    if (C.sl == 0) {
+        G.status[inst] = st;
+        G.iters[inst] = iters;
+        if (A.prof && G.prof) {
+            pc[7] = cycles() - tstart;
+            for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
+        }
+    }
+}
+
+// Precompiled kernel: the level-scheduled factorisation of the pattern's records.
+template <bool MIXED>
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel(const GroupGPU* __restrict__ groups, int n_groups,
+                                                                  Args A) {
+    newton_body<MIXED, LuLevels>(groups, n_groups, A);
+}
+
+}  // namespace s21

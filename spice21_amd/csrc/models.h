@@ -6,10 +6,15 @@
 // entry sigma(k), which is where the reference's `matps[(dr, ..)]` indexing sends it (comps/mos.rs:
 // 833-868, 1085-1097).  Entries therefore map statically onto matrix elements.
 #pragma once
+#ifndef __HIPCC_RTC__  // run-time compilation provides the HIP declarations itself
 #include <hip/hip_runtime.h>
+#endif
 
+#ifndef __HIPCC_RTC__
 #include <cmath>
+#endif
 #include <cstdint>
+#include <type_traits>
 
 namespace s21 {
 
@@ -56,6 +61,13 @@ struct Mos1State {
     double vgs, vgd, vgb, cgs, cgd, cgb;
     double reversed;  // 0 or 1
 };
+
+// A sink that declares `raw_stamps` receives Mos1 stamps as stamp(k, reversed, value) with the
+// reference's stamp index k, and applies the source/drain swap itself.
+template <class P, class = void>
+struct RawSink : std::false_type {};
+template <class P>
+struct RawSink<P, std::void_t<decltype(P::raw_stamps)>> : std::true_type {};
 
 // Backward-Euler companion (analysis.rs:420-428).
 struct Integ {
@@ -170,33 +182,38 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
     }
     const double irhs = ids - gm * vgs - gds * vds;
     const double grd = P.grd, grs = P.grs;
-    auto E = [reversed](int k) { return reversed ? mos1_sigma(k) : k; };
-    put(E(0), gds + grd + gbd + trgd_g);
-    put(E(1), gm + gds + grs + gbs + gmbs + trgs_g);
-    put(E(2), -gm - gds - gmbs);
-    put(E(3), -gds);
-    put(E(4), gm - trgd_g);
-    put(E(5), -gm - trgs_g);
-    put(E(6), (trgd_g + trgs_g + trgb_g));
-    put(E(7), (gbd + gbs + trgb_g));
-    put(E(8), -trgb_g);
-    put(E(9), -trgd_g);
-    put(E(10), -trgs_g);
-    put(E(11), -trgb_g);
-    put(E(12), -gbd);
-    put(E(13), -gbs);
-    put(E(14), -gbd + gmbs);
-    put(E(15), -gbs - gmbs);
-    put(E(16), -grd);
-    put(E(17), -grd);
-    put(E(18), grd);
-    put(E(19), -grs);
-    put(E(20), -grs);
-    put(E(21), grs);
-    put(E(22), p * (-irhs + ibd_rhs + trgd_rhs));
-    put(E(23), p * (irhs + ibs_rhs + trgs_rhs));
-    put(E(24), -p * (trgs_rhs + trgb_rhs + trgd_rhs));
-    put(E(25), -p * (ibd_rhs + ibs_rhs - trgb_rhs));
+    auto E = [&](int k, double v) {
+        if constexpr (RawSink<std::remove_reference_t<Put>>::value)
+            put.stamp(k, reversed, v);
+        else
+            put(reversed ? mos1_sigma(k) : k, v);
+    };
+    E(0, gds + grd + gbd + trgd_g);
+    E(1, gm + gds + grs + gbs + gmbs + trgs_g);
+    E(2, -gm - gds - gmbs);
+    E(3, -gds);
+    E(4, gm - trgd_g);
+    E(5, -gm - trgs_g);
+    E(6, (trgd_g + trgs_g + trgb_g));
+    E(7, (gbd + gbs + trgb_g));
+    E(8, -trgb_g);
+    E(9, -trgd_g);
+    E(10, -trgs_g);
+    E(11, -trgb_g);
+    E(12, -gbd);
+    E(13, -gbs);
+    E(14, -gbd + gmbs);
+    E(15, -gbs - gmbs);
+    E(16, -grd);
+    E(17, -grd);
+    E(18, grd);
+    E(19, -grs);
+    E(20, -grs);
+    E(21, grs);
+    E(22, p * (-irhs + ibd_rhs + trgd_rhs));
+    E(23, p * (irhs + ibs_rhs + trgs_rhs));
+    E(24, -p * (trgs_rhs + trgb_rhs + trgd_rhs));
+    E(25, -p * (ibd_rhs + ibs_rhs - trgb_rhs));
 }
 
 // ---- Mos0 (comps/mos.rs:1051-1099) -------------------------------------------------------------
@@ -234,15 +251,15 @@ __host__ __device__ inline void mos0_load(double p, double vd, double vg, double
         }
     }
     const double irhs = ids - gm * vgs - gds * vds;
-    auto E = [reversed](int k) { return reversed ? mos0_sigma(k) : k; };
-    put(E(0), gds + gmin);
-    put(E(1), (gm + gds + gmin));
-    put(E(2), -(gm + gds + gmin));
-    put(E(3), -gds - gmin);
-    put(E(4), gm);
-    put(E(5), -gm);
-    put(E(6), -p * irhs);
-    put(E(7), p * irhs);
+    auto E = [&](int k, double v) { put(reversed ? mos0_sigma(k) : k, v); };
+    E(0, gds + gmin);
+    E(1, (gm + gds + gmin));
+    E(2, -(gm + gds + gmin));
+    E(3, -gds - gmin);
+    E(4, gm);
+    E(5, -gm);
+    E(6, -p * irhs);
+    E(7, p * irhs);
 }
 
 // ---- Diode (comps/diode.rs:228-354) -------------------------------------------------------------

@@ -9,7 +9,7 @@
 //                       searched (sparse21/mod.rs:647-919).  Restated over permutation arrays instead of
 //                       orthogonal linked lists, visiting candidates in the same order.
 //  * topo_schedule   -- L+U pattern in internal order, IKJ update lists, row levels for the
-//                       factorisation and both sweeps, contribution slots and gather lists.
+//                       factorisation and both sweeps, device colours, per-entry slot codes and hub gather lists.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -543,37 +543,27 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
         t.res_ptr[i + 1] = (int)t.res_pos.size();
     }
 
-    // Contributions: every live (component, entry) gets one slot, in component then entry order, which
-    // is the order the reference adds stamps into an element.
+    // Devices: the live (entry, slot) pairs of every component, in entry order.  A slot is an LU position
+    // of A (0 .. nnz-1) or an RHS row (nnz + row); grounded entries are dropped.
     std::unordered_map<int64_t, int> elem;
     for (size_t k = 0; k < t.erow.size(); k++) elem[key(t.erow[k], t.ecol[k], t.n)] = (int)k;
-    std::vector<std::vector<int>> slot(t.nnz + n);
-    t.m1_comp.clear(); t.m0_comp.clear(); t.d_comp.clear(); t.lin_comp.clear();
-    t.m1_nodes.clear(); t.m1_mask.clear(); t.m1_base.clear(); t.m1_caps.clear();
-    t.m0_nodes.clear(); t.m0_mask.clear(); t.m0_base.clear(); t.m0_p.clear();
-    t.d_nodes.clear(); t.d_mask.clear(); t.d_base.clear();
-    t.lin_type.clear(); t.lin_nodes.clear(); t.lin_mask.clear(); t.lin_base.clear();
-    t.contrib_comp.clear();
-    int nc = 0;
-    auto target = [&](int r, int c) -> int {  // slot of a matrix entry or RHS entry, -1 if grounded
+    auto target = [&](int r, int c) -> int {
         if (r < 0) return -1;
         if (c == -2) return t.nnz + r;  // RHS
         if (c < 0) return -1;
         return t.elem_pos[elem.at(key(r, c, t.n))];
     };
+    const int ncomp = (int)t.comps.size();
+    std::vector<std::vector<std::pair<int, int>>> live(ncomp);
+    std::vector<int> cls_of(ncomp, CLS_LIN);
+    std::vector<uint16_t> flags(ncomp, 0);
     int im1 = 0;
-    for (size_t ci = 0; ci < t.comps.size(); ci++) {
+    for (int ci = 0; ci < ncomp; ci++) {
         const Comp& c = t.comps[ci];
         const int* nd = c.node;
-        uint32_t mask = 0;
-        int base = nc;
-        auto live = [&](int e, int r, int col) {
-            int s = target(r, col);
-            if (s < 0) return;
-            mask |= 1u << e;
-            slot[s].push_back(nc);
-            t.contrib_comp.push_back((int)ci);
-            nc++;
+        auto add = [&](int e, int r, int col) {
+            const int sl = target(r, col);
+            if (sl >= 0) live[ci].push_back({e, sl});
         };
         switch (c.type) {
         case S21_COMP_MOS1: {
@@ -582,83 +572,163 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
             for (int e = 0; e < MOS1_NE; e++) {
                 if (mos1_entry_tr_only(e) && !(caps && t.tran)) continue;  // exactly zero
                 if (mos1_entry_r_only(e) && !rdrs) continue;             // grd = grs = 0
-                int r = nd[mos1_entry_row(e)];
-                int col = mos1_entry_col(e) < 0 ? -2 : nd[mos1_entry_col(e)];
+                const int r = nd[mos1_entry_row(e)];
+                const int col = mos1_entry_col(e) < 0 ? -2 : nd[mos1_entry_col(e)];
                 if (mos1_entry_col(e) >= 0 && col < 0) continue;
-                live(e, r, col);
+                add(e, r, col);
             }
-            t.m1_comp.push_back((int)ci);
-            for (int k = 0; k < 6; k++) t.m1_nodes.push_back(nd[k]);
-            t.m1_mask.push_back(mask);
-            t.m1_base.push_back(base);
-            t.m1_caps.push_back(caps && t.tran ? 1 : 0);
+            cls_of[ci] = CLS_MOS1;
+            flags[ci] = caps && t.tran ? DEV_CAPS : 0;
             break;
         }
         case S21_COMP_MOS0:
             for (int e = 0; e < MOS0_NE; e++) {
-                int r = nd[mos0_entry_row(e)];
-                int col = mos0_entry_col(e) < 0 ? -2 : nd[mos0_entry_col(e)];
+                const int r = nd[mos0_entry_row(e)];
+                const int col = mos0_entry_col(e) < 0 ? -2 : nd[mos0_entry_col(e)];
                 if (mos0_entry_col(e) >= 0 && col < 0) continue;
-                live(e, r, col);
+                add(e, r, col);
             }
-            t.m0_comp.push_back((int)ci);
-            for (int k = 0; k < 4; k++) t.m0_nodes.push_back(nd[k]);
-            t.m0_mask.push_back(mask);
-            t.m0_base.push_back(base);
-            t.m0_p.push_back(c.model == 1 ? -1.0 : 1.0);
+            cls_of[ci] = CLS_MOS0;
+            flags[ci] = c.model == 1 ? DEV_PMOS : 0;
             break;
         case S21_COMP_DIODE:
             for (int e = 0; e < DIODE_NE; e++) {
-                int r = nd[diode_entry_row(e)];
-                int col = diode_entry_col(e) < 0 ? -2 : nd[diode_entry_col(e)];
+                const int r = nd[diode_entry_row(e)];
+                const int col = diode_entry_col(e) < 0 ? -2 : nd[diode_entry_col(e)];
                 if (diode_entry_col(e) >= 0 && col < 0) continue;
-                live(e, r, col);
+                add(e, r, col);
             }
-            t.d_comp.push_back((int)ci);
-            for (int k = 0; k < 3; k++) t.d_nodes.push_back(nd[k]);
-            t.d_mask.push_back(mask);
-            t.d_base.push_back(base);
+            cls_of[ci] = CLS_DIODE;
             break;
         default: {
             const int p = nd[0], q = nd[1], iv = nd[2];
-            if (c.type == S21_COMP_R) {
-                live(0, p, p < 0 ? -1 : p);
-                live(1, q, q < 0 ? -1 : q);
-                live(2, p, q < 0 ? -1 : q);
-                live(3, q, p < 0 ? -1 : p);
-            } else if (c.type == S21_COMP_C) {
-                if (t.tran) {
-                    live(0, p, p < 0 ? -1 : p);
-                    live(1, q, q < 0 ? -1 : q);
-                    live(2, p, q < 0 ? -1 : q);
-                    live(3, q, p < 0 ? -1 : p);
-                    live(4, p, -2);
-                    live(5, q, -2);
+            if (c.type == S21_COMP_R || (c.type == S21_COMP_C && t.tran)) {
+                add(0, p, p < 0 ? -1 : p);
+                add(1, q, q < 0 ? -1 : q);
+                add(2, p, q < 0 ? -1 : q);
+                add(3, q, p < 0 ? -1 : p);
+                if (c.type == S21_COMP_C) {
+                    add(4, p, -2);
+                    add(5, q, -2);
                 }
             } else if (c.type == S21_COMP_V) {
-                live(0, p, iv);
-                live(1, iv, p < 0 ? -1 : p);
-                live(2, q, iv);
-                live(3, iv, q < 0 ? -1 : q);
-                live(4, iv, -2);
+                add(0, p, iv);
+                add(1, iv, p < 0 ? -1 : p);
+                add(2, q, iv);
+                add(3, iv, q < 0 ? -1 : q);
+                add(4, iv, -2);
             } else if (c.type == S21_COMP_I) {
-                live(0, p, -2);
-                live(1, q, -2);
+                add(0, p, -2);
+                add(1, q, -2);
             }
-            t.lin_comp.push_back((int)ci);
-            t.lin_type.push_back(c.type);
-            for (int k = 0; k < 3; k++) t.lin_nodes.push_back(nd[k]);
-            t.lin_mask.push_back(mask);
-            t.lin_base.push_back(base);
+            cls_of[ci] = CLS_LIN;
+            flags[ci] = (uint16_t)(c.type << 4);
         }
         }
     }
-    t.n_contrib = nc;
-    t.g_ptr.assign(t.nnz + n + 1, 0);
+    const int nslot = t.nnz + n;
+    if (nslot >= 0x8000) fail(S21_INVALID, "circuit too large for the batched kernel (slot index)");
+    // Hub slots: more than HUB_CONTRIB distinct contributing components ...
+    std::vector<int> ncontrib(nslot, 0), lastc(nslot, -1);
+    for (int ci = 0; ci < ncomp; ci++)
+        for (auto& es : live[ci])
+            if (lastc[es.second] != ci) {
+                lastc[es.second] = ci;
+                ncontrib[es.second]++;
+            }
+    std::vector<uint8_t> hub(nslot, 0);
+    for (int sl = 0; sl < nslot; sl++) hub[sl] = ncontrib[sl] > HUB_CONTRIB;
+    // ... and every slot one component stamps twice (the kernel applies a device's stamps as a batch)
+    std::fill(lastc.begin(), lastc.end(), -1);
+    for (int ci = 0; ci < ncomp; ci++)
+        for (auto& es : live[ci]) {
+            if (lastc[es.second] == ci) hub[es.second] = 1;
+            lastc[es.second] = ci;
+        }
+    // Greedy colouring, class by class, components in order: the first colour of the class in which no
+    // earlier member shares a non-hub slot with the component.
+    std::vector<std::vector<int>> slot_cols(nslot);
+    std::vector<std::vector<int>> cols;  // members per colour (component order)
+    std::vector<int> colcls;
+    std::vector<uint8_t> used;
+    for (int k = 0; k < N_CLS; k++) {
+        const size_t first = cols.size();
+        for (int ci = 0; ci < ncomp; ci++) {
+            if (cls_of[ci] != k) continue;
+            used.assign(cols.size() + 1, 0);
+            for (auto& es : live[ci])
+                if (!hub[es.second])
+                    for (int cc : slot_cols[es.second]) used[cc] = 1;
+            size_t cc = first;
+            while (cc < cols.size() && used[cc]) cc++;
+            if (cc == cols.size()) {
+                cols.emplace_back();
+                colcls.push_back(k);
+            }
+            cols[cc].push_back(ci);
+            for (auto& es : live[ci])
+                if (!hub[es.second]) {
+                    std::vector<int>& v = slot_cols[es.second];
+                    if (std::find(v.begin(), v.end(), (int)cc) == v.end()) v.push_back((int)cc);
+                }
+        }
+    }
+    // Device order, records and codes.
+    t.dev_comp.clear();
+    t.col_ptr.assign(1, 0);
+    t.col_cls.clear();
+    for (int k = 0; k < N_CLS; k++) t.cls_start[k] = -1;
+    for (size_t cc = 0; cc < cols.size(); cc++) {
+        if (t.cls_start[colcls[cc]] < 0) t.cls_start[colcls[cc]] = (int)t.dev_comp.size();
+        for (int ci : cols[cc]) t.dev_comp.push_back(ci);
+        t.col_ptr.push_back((int)t.dev_comp.size());
+        t.col_cls.push_back(colcls[cc]);
+    }
+    const int ndev = (int)t.dev_comp.size();
+    t.cls_start[N_CLS] = ndev;
+    for (int k = N_CLS - 1; k >= 0; k--)
+        if (t.cls_start[k] < 0) t.cls_start[k] = t.cls_start[k + 1];
+    std::vector<int> pos_of(ncomp, -1);
+    for (int p = 0; p < ndev; p++) pos_of[t.dev_comp[p]] = p;
+    int nc = 0;
+    for (int k = 0; k < N_CLS; k++) {
+        t.code_base[k] = nc;
+        nc += (t.cls_start[k + 1] - t.cls_start[k]) * code_stride(k);
+    }
+    if (nc > 65535 || ndev > 65535) fail(S21_INVALID, "circuit too large for the batched kernel (devices)");
+    t.codes.assign(nc, 0xFFFF);
+    t.dev_rec.assign((size_t)8 * ndev, 0);
+    for (int p = 0; p < ndev; p++) {
+        const int ci = t.dev_comp[p];
+        for (int j = 0; j < 6; j++) t.dev_rec[8 * p + j] = (uint16_t)(t.comps[ci].node[j] + 1);
+        t.dev_rec[8 * p + 6] = flags[ci];
+    }
+    auto code_at = [&](int ci, int e) -> uint16_t& {
+        const int k = cls_of[ci], p = pos_of[ci];
+        return t.codes[t.code_base[k] + (p - t.cls_start[k]) * code_stride(k) + e];
+    };
+    // Hub contributions numbered in component-then-entry order: each hub slot's list is then the order
+    // Solver::update adds its stamps.
+    std::vector<std::vector<int>> hub_list(nslot);
+    t.n_hub = 0;
+    for (int ci = 0; ci < ncomp; ci++)
+        for (auto& es : live[ci]) {
+            if (hub[es.second]) {
+                if (t.n_hub >= 0x7FFF) fail(S21_INVALID, "circuit too large for the batched kernel (hub)");
+                code_at(ci, es.first) = (uint16_t)(0x8000 | t.n_hub);
+                hub_list[es.second].push_back(t.n_hub++);
+            } else {
+                code_at(ci, es.first) = (uint16_t)es.second;
+            }
+        }
+    t.hub_slot.clear();
+    t.g_ptr.assign(1, 0);
     t.g_idx.clear();
-    for (int s = 0; s < t.nnz + n; s++) {
-        for (int c : slot[s]) t.g_idx.push_back(c);
-        t.g_ptr[s + 1] = (int)t.g_idx.size();
+    for (int sl = 0; sl < nslot; sl++) {
+        if (!hub[sl]) continue;
+        t.hub_slot.push_back(sl);
+        for (int h : hub_list[sl]) t.g_idx.push_back(h);
+        t.g_ptr.push_back((int)t.g_idx.size());
     }
 }
 
@@ -678,8 +748,9 @@ void topo_records(Topology& t) {
     int part = 0, part_r = 0;
     t.gt.clear();
     t.gc.clear();
-    for (int s = 0; s < t.nnz + n; s++) {
-        const int k0 = t.g_ptr[s], k1 = t.g_ptr[s + 1];
+    for (size_t h = 0; h < t.hub_slot.size(); h++) {
+        const int s = t.hub_slot[h];
+        const int k0 = t.g_ptr[h], k1 = t.g_ptr[h + 1];
         if (k1 - k0 <= CHUNK) {
             for (int v : {s, k0, k1, 65535}) t.gt.push_back(u(v));
             continue;

@@ -121,3 +121,42 @@ def test_mixed_tran_close_to_fp64():
     m = bm.wave()
     log("mixed vs fp64 tran", float(np.max(np.abs(m[:, :4] - a[:, :4]))))
     assert np.max(np.abs(m[:, :4] - a[:, :4])) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_generated_factorisation_matches_level_schedule(precision):
+    """The run-time generated factorisation performs the record-driven one's operations in the same order."""
+    n_inst = 600
+    nl, over = W.chain_mc(n_inst=n_inst)
+    g = Group(nl.to_c(), n_inst, over)
+    log("levels", precision)
+    a = Batch(g, precision=precision).set_jit(-1).op()
+    assert a.config()[2] is False
+    log("generated", precision)
+    b = Batch(g, precision=precision).set_jit(1).op()
+    assert b.config()[2] is True
+    sa, ia = a.status()
+    sb, ib = b.status()
+    assert np.all(sa == 0) and np.all(sb == 0)
+    assert np.array_equal(ia, ib)
+    assert np.max(np.abs(a.x() - b.x())) <= 1e-12
+    if precision == FP64:
+        x0, _, _ = O.dcop(materialize(nl, over, 7).to_c())
+        assert rel_err(b.x()[7], x0) < 1e-9
+
+
+@pytest.mark.parametrize("ipw", [1, 4])
+def test_generated_factorisation_transient_matches_golden(ipw):
+    kind, tstep, tstop = GOLDEN_RUNS["test_mos1_cmos_ro_tran.json"]
+    g = json.load(open(os.path.join(GOLDEN, "test_mos1_cmos_ro_tran.json")))
+    cn = ro3(kind).to_c()
+    log("generated tran ipw", ipw)
+    b = Batch(Group(cn, 3)).set_jit(1).set_ipw(ipw).tran(tstep, tstop, [("1", 0.0)])
+    assert b.config()[2] is True
+    st, _ = b.status()
+    assert np.all(st == 0)
+    names = b.var_names()
+    for inst in range(3):
+        X = b.wave(0, inst)
+        worst = max(float(np.max(np.abs(X[:, k] - np.array(g[nm])))) for k, nm in enumerate(names))
+        assert worst <= 1e-6, (inst, worst)
