@@ -26,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "newton_kernel_chain_mc4096_traffic.txt")
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "s21_newton_jit_chain_mc4096_traffic.txt")
 
 
 def algorithmic_bytes(cnet, structure, factor_bytes):
@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--n-inst", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the instances-per-wavefront autotuning before the timed steps")
+    ap.add_argument("--ipw", type=int, default=0, help="instances per wavefront (implies --no-autotune)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="'mixed' / 'fp64' time one path only (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -136,12 +139,18 @@ def main():
         if args.precision not in ("both", name):
             continue
         b = Batch(group, precision=prec)
+        if args.ipw:
+            b.set_ipw(args.ipw)
+        tune = b.autotune(lambda bb: bb.op()) if args.autotune and not args.ipw else {}
         elapsed, kms, st, it = time_batch(b, args.steps, args.warmup)
+        ipw, waves, jit = b.config()
         res[name] = dict(batch=b, elapsed=elapsed, kernel_ms=kms, status=st, iters=it, x=b.x(),
-                         structure=b.structure())
+                         structure=b.structure(), launch={"instances_per_wavefront": ipw, "wavefronts_per_workgroup": waves,
+                                                          "generated_factorisation": jit,
+                                                          "autotune_ms": {str(k): v for k, v in tune.items()}})
         print(f"[bench] {name}: {args.steps} steps in {elapsed:.4f} s, kernel {np.mean(kms):.3f} ms/launch, "
-              f"iterations/instance {it.min()}..{it.max()}, failed {int(np.sum(st != 0))}", file=sys.stderr,
-              flush=True)
+              f"iterations/instance {it.min()}..{it.max()}, failed {int(np.sum(st != 0))}, launch {res[name]['launch']}",
+              file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in res else "fp64"
     h = res[head]
     inst_iters = int(h["iters"].sum())  # per step; every step repeats the same deterministic solve
@@ -167,7 +176,7 @@ def main():
         "config": {"workload": f"chain_mc: {args.n_inst} x 64-stage CMOS inverter chain dcop", "n_inst": args.n_inst,
                    "unknowns": cn.n_vars, "mos1_per_instance": 128, "nnz_a": h["structure"]["nnz_a"],
                    "nnz_lu": h["structure"]["nnz_lu"], "newton_iters_per_step": inst_iters,
-                   "failed_instances": int(np.sum(h["status"] != 0))},
+                   "failed_instances": int(np.sum(h["status"] != 0)), "launch": h["launch"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_inst_iter": b_iter, "kernel_avg_ms": kernel_avg_ms},

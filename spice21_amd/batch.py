@@ -169,6 +169,24 @@ class Batch:
         check(lib().s21_batch_set_jit(self.h, int(mode)))
         return self
 
+    def autotune(self, run, candidates=(1, 2, 4), repeats=2):
+        """Times ``run(self)`` (an analysis call) at each instances-per-wavefront candidate and keeps the
+        fastest; returns {ipw: kernel milliseconds}.  Each candidate runs once untimed first, which also
+        builds its kernel."""
+        times = {}
+        for ipw in candidates:
+            self.set_ipw(ipw)
+            run(self)
+            ms = []
+            for _ in range(repeats):
+                run(self)
+                ms.append(self.timing()[0])
+            times[ipw] = min(ms)
+        best = min(times, key=times.get)
+        self.set_ipw(best)
+        run(self)
+        return times
+
     def config(self):
         """(instances per wavefront, wavefronts per workgroup, generated factorisation used) of the last
         launch."""

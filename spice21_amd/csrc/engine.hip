@@ -72,6 +72,7 @@ struct GroupHost {
     size_t blob_bytes = 0, inst_bytes = 0;  // record blob; LDS bytes one instance needs
     JitKernel jit;                          // kernel with this pattern's factorisation written out
     bool has_jit = false;
+    int jit_ipw = 1;                        // instances per wavefront the generated kernel is built for
     std::string prepared;  // analysis the uploaded structure was built for ("" = none)
 };
 
@@ -255,7 +256,7 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
     topo_schedule(t, has_caps);
 
     // Per-instance parameters, SoA [field][instance][class-local device] in the kernel's device order.
-    const int n_m1 = t.cls_start[CLS_MOS1 + 1] - t.cls_start[CLS_MOS1];
+    const int n_m1 = t.cls_start[CLS_MOS1T + 1] - t.cls_start[CLS_MOS1];  // both Mos1 classes
     const int n_d = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE];
     const int n_lin = t.cls_start[CLS_LIN + 1] - t.cls_start[CLS_LIN];
     hp.n_m1 = n_m1;
@@ -319,6 +320,21 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
     }
 }
 
+// SIMDs of the current device.
+static int device_simds() {
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return 4 * std::max(cus, 1);
+}
+// Instances per wavefront chosen for a batch: the largest power of two up to 16 that still gives every
+// SIMD of the chip two wavefronts (one to run while the other waits on memory).
+static int auto_ipw(int total_inst, int simds) {
+    int ipw = 1;
+    while (ipw < 16 && total_inst / (ipw * 2) >= 2 * simds) ipw *= 2;
+    return ipw;
+}
+
 // Derive per-instance parameters, build the topology and upload everything for one analysis.
 static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr) {
     HostPrep hp;
@@ -351,7 +367,7 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     for (size_t cc = 0; cc < t.col_cls.size(); cc++) {
         const int k = t.col_cls[cc], p0 = t.col_ptr[cc];
         col.push_back(k);
-        col.push_back(p0 - t.cls_start[k]);
+        col.push_back(p0 - t.cls_start[k == CLS_MOS1T ? CLS_MOS1 : k]);  // Mos1 parameters span both classes
         col.push_back(t.code_base[k] + (p0 - t.cls_start[k]) * code_stride(k));
     }
     TopoGPU& T = g.gg.t;
@@ -397,22 +413,34 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     HIPCHECK(hipMemcpy(g.d_topo->p, blob.data(), blob_bytes, hipMemcpyHostToDevice));
     T.blob = (const uint4*)g.d_topo->p;
 
-    // ---- parameters (read-only) ----
+    // ---- parameters (read-only), packed by field: a field no instance overrides is stored once ----
     auto pad = [](size_t n) { return ((n * sizeof(double) + 255) / 256) * 256; };
-    const size_t s_core = pad(hp.m1_core.size()), s_caps = pad(hp.m1_caps.size()), s_dp = pad(hp.d_par.size()),
-                 s_lin = pad(hp.lin_val.size());
-    g.d_par.reset(new DevMem());
-    g.d_par->alloc(s_core + s_caps + s_dp + 2 * s_lin + 256);
-    char* pp = (char*)g.d_par->p;
-    auto up = [&](size_t off, const std::vector<double>& v) {
-        if (!v.empty()) HIPCHECK(hipMemcpy(pp + off, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-        return (const double*)(pp + off);
+    std::vector<double> par;
+    auto pack = [&](const std::vector<double>& full, int nf, int ndev, long long* off, int* istr) {
+        const size_t s = (size_t)NI * ndev;
+        for (int f = 0; f < nf; f++) {
+            const double* v = full.data() + f * s;
+            bool shared = true;
+            for (size_t i = ndev; i < s && shared; i++) shared = v[i] == v[i % ndev];
+            while (par.size() % 32) par.push_back(0.0);  // 256-byte aligned fields
+            off[f] = (long long)par.size();
+            istr[f] = shared ? 0 : ndev;
+            par.insert(par.end(), v, v + (shared ? (size_t)ndev : s));
+        }
     };
-    g.gg.m1_core = up(0, hp.m1_core);
-    g.gg.m1_caps = up(s_core, hp.m1_caps);
-    g.gg.d_par = up(s_core + s_caps, hp.d_par);
-    g.gg.lin_val = up(s_core + s_caps + s_dp, hp.lin_val);
-    g.gg.lin_val_tran = up(s_core + s_caps + s_dp + s_lin, hp.lin_tran);
+    {
+        std::vector<double> m1(hp.m1_core);
+        m1.insert(m1.end(), hp.m1_caps.begin(), hp.m1_caps.end());
+        pack(m1, M1_CORE_F + M1_CAPS_F, hp.n_m1, g.gg.m1_off, g.gg.m1_istr);
+        pack(hp.d_par, D_PAR_F, hp.n_d, g.gg.d_off, g.gg.d_istr);
+        std::vector<double> lin(hp.lin_val);
+        lin.insert(lin.end(), hp.lin_tran.begin(), hp.lin_tran.end());
+        pack(lin, 2, hp.n_lin, g.gg.lin_off, g.gg.lin_istr);
+    }
+    g.d_par.reset(new DevMem());
+    g.d_par->alloc(std::max<size_t>(par.size(), 1) * sizeof(double));
+    if (!par.empty()) HIPCHECK(hipMemcpy(g.d_par->p, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
+    g.gg.par = (const double*)g.d_par->p;
 
     // ---- state (zero-initialised: Default op points, comps/mos.rs:524-546) ----
     const size_t s_m1s = pad((size_t)M1_STATE_F * 2 * NI * hp.n_m1), s_ds = pad((size_t)D_STATE_F * NI * hp.n_d),
@@ -443,10 +471,13 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
 
     HIPCHECK(hipStreamSynchronize(b->stream));
     // Generated factorisation: for a batch of one pattern with enough instances to repay the compilation.
+    // Its launch shape is fixed here: one instance per wavefront asks the compiler for four resident
+    // wavefronts per SIMD, more instances per wavefront for two.
     g.has_jit = false;
     const bool want = b->jit > 0 || (b->jit == 0 && b->groups.size() == 1 && NI >= JIT_MIN_INST);
     if (want && t.sym_status == S21_OK && jit_ops(t) <= JIT_MAX_OPS) {
-        g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED));
+        g.jit_ipw = b->ipw > 0 ? b->ipw : auto_ipw(NI, device_simds());
+        g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED, g.jit_ipw == 1 ? 4 : 2));
         g.has_jit = true;
     }
     g.blob_bytes = blob_bytes;
@@ -500,16 +531,13 @@ static size_t inst_stride(size_t raw, int ipw) {
     return ((raw + 255) / 256) * 256 + (L * 8) % 256;
 }
 
-// Launch shape: instances per wavefront (the batch's setting, or the largest power of two up to 16 that
-// still gives every SIMD of the chip a wavefront) and wavefronts per workgroup (the most resident
-// instances per CU within the LDS).
+// Launch shape: instances per wavefront (the batch's setting, or auto_ipw) and wavefronts per workgroup
+// (the most resident instances per CU within the LDS).
 static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_inst, size_t max_block, int simds,
                         Args& A, size_t& stride) {
     int ipw = b->ipw;
-    if (ipw <= 0) {
-        ipw = 1;
-        while (ipw < 16 && total_inst / (ipw * 2) >= simds) ipw *= 2;
-    }
+    if (b->groups.size() == 1 && b->groups[0].has_jit) ipw = b->groups[0].jit_ipw;
+    if (ipw <= 0) ipw = auto_ipw(total_inst, simds);
     for (;; ipw /= 2) {
         stride = inst_stride(raw, ipw);
         int best = 0;
@@ -717,7 +745,7 @@ int s21_jit_build(const s21_netlist* net, const s21_options* opts, const s21_tra
         const Topology& t = g.topo;
         if (t.sym_status != S21_OK) fail(t.sym_status, t.sym_error);
         if (n_ops) *n_ops = jit_ops(t);
-        const std::vector<char> code = jit_code(jit_source(t, precision == S21_MIXED));
+        const std::vector<char> code = jit_code(jit_source(t, precision == S21_MIXED, 2));
         if (code_bytes) *code_bytes = (int64_t)code.size();
     });
 }
@@ -865,6 +893,9 @@ int s21_batch_set_ipw(s21_batch* b, int32_t ipw) {
         if (!b) fail(S21_INVALID, "null batch");
         if (ipw != 0 && ipw != 1 && ipw != 2 && ipw != 4 && ipw != 8 && ipw != 16)
             fail(S21_INVALID, "instances per wavefront must be 0, 1, 2, 4, 8 or 16");
+        if (ipw != b->ipw)
+            for (GroupHost& g : b->groups)
+                if (g.has_jit) g.prepared.clear();  // the generated kernel is built for one shape
         b->ipw = ipw;
     });
 }

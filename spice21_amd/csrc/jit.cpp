@@ -3,7 +3,8 @@
 // The level-scheduled factorisation walks 16-bit records in LDS, so every dependent step pays a record
 // load before its data load.  For a batch whose instances share one pattern, this file writes the
 // factorisation with the fused forward sweep and the backward sweep out as straight-line code -- every
-// LU position a constant offset -- in exactly the order of the records (IKJ by rows in pivot order:
+// LU position a constant offset -- in exactly the order of the records and with the same division
+// (ldiv) (IKJ by rows in pivot order:
 // l_ik = a_ik / u_kk, the row-k updates, c_i -= c_k l_ik skipped when c_k is zero; then rows from the last,
 // c_i = (c_i - sum u_ij c_j) / u_ii; sparse21/mod.rs:865-919, 946-979), compiles it together with the
 // kernel body of kernel.h through hiprtc, and keeps the module for the life of the process.  One lane per
@@ -11,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <mutex>
 #include <sstream>
@@ -37,7 +40,7 @@ int64_t jit_ops(const Topology& t) {
     return ops;
 }
 
-std::string jit_source(const Topology& t, bool mixed) {
+std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
     std::ostringstream o;
     o << "#include \"kernel.h\"\nnamespace s21 {\n";
     // factorisation + forward sweep
@@ -48,7 +51,7 @@ std::string jit_source(const Topology& t, bool mixed) {
         o << "    y = (V)c[" << i << "];\n";
         for (int q = t.rp[i]; q < t.diag[i]; q++) {
             const int k = t.ci[q];
-            o << "    l = lu[" << q << "] / lu[" << t.diag[k] << "]; lu[" << q << "] = l; ck = (V)c[" << k
+            o << "    l = ldiv<V>(lu[" << q << "], lu[" << t.diag[k] << "]); lu[" << q << "] = l; ck = (V)c[" << k
               << "]; if (ck != V(0)) y = y - ck * l;\n";
             for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++)
                 o << "    lu[" << t.upd_dst[x] << "] = lu[" << t.upd_dst[x] << "] - lu[" << t.upd_src[x] << "] * l;\n";
@@ -64,7 +67,7 @@ std::string jit_source(const Topology& t, bool mixed) {
     for (int i = t.n - 1; i >= 0; i--) {
         o << "    y = (V)c[" << i << "];";
         for (int q = t.diag[i] + 1; q < t.rp[i + 1]; q++) o << " y = y - (V)c[" << t.ci[q] << "] * lu[" << q << "];";
-        o << " c[" << i << "] = (double)(y / lu[" << t.diag[i] << "]);\n";
+        o << " c[" << i << "] = (double)ldiv<V>(y, lu[" << t.diag[i] << "]);\n";
     }
     o << "}\n";
     o << "struct LuGen {\n"
@@ -81,7 +84,8 @@ std::string jit_source(const Topology& t, bool mixed) {
          "        wave_sync();\n"
          "    }\n"
          "};\n}  // namespace s21\n";
-    o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES) void s21_newton_jit("
+    o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES, " << waves_per_simd
+      << ") void s21_newton_jit("
          "const s21::GroupGPU* __restrict__ groups, int n_groups, s21::Args A) {\n"
          "    s21::newton_body<"
       << (mixed ? "true" : "false") << ", s21::LuGen>(groups, n_groups, A);\n}\n";
@@ -100,6 +104,12 @@ Cache& cache() {
 }  // namespace
 
 std::vector<char> jit_code(const std::string& src) {
+    if (const char* dump = std::getenv("S21_JIT_DUMP")) {  // debugging aid: the generated source
+        if (FILE* f = std::fopen(dump, "w")) {
+            std::fwrite(src.data(), 1, src.size(), f);
+            std::fclose(f);
+        }
+    }
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "s21_newton_jit.hip", kJitHeaderCount, kJitHeaderSources,
                             kJitHeaderNames) != HIPRTC_SUCCESS)

@@ -15,8 +15,9 @@ struct JitKernel {
 };
 // Statements of the generated factorisation and sweeps (its size).
 int64_t jit_ops(const Topology& t);
-// Source of the kernel with the factorisation of t written out, for FP64 or mixed precision.
-std::string jit_source(const Topology& t, bool mixed);
+// Source of the kernel with the factorisation of t written out, for FP64 or mixed precision, compiled
+// for at least `waves_per_simd` resident wavefronts per SIMD (its register budget).
+std::string jit_source(const Topology& t, bool mixed, int waves_per_simd);
 // Code object of a source for gfx950 (no device needed).
 std::vector<char> jit_code(const std::string& src);
 // Compiled kernel for a source (compiled once per process and source).

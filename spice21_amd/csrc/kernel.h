@@ -34,6 +34,8 @@ namespace s21 {
 // ============================================================================================
 // Device-side descriptors
 // ============================================================================================
+constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 3;
+
 struct TopoGPU {
     int n, nnz, n_hub, n_part;
     int n_cls[N_CLS];  // devices per class (Mos1, Mos0, Diode, linear)
@@ -53,13 +55,17 @@ struct TopoGPU {
 struct GroupGPU {
     TopoGPU t;
     int n_inst, blk0;
-    const double* m1_core;  // [11][n_inst][n_m1]   (class-local device order)
-    const double* m1_caps;  // [4][n_inst][n_m1]
-    double* m1_state;       // [14][n_inst][n_m1]: committed op (7) then guess (7)
-    const double* d_par;    // [14][n_inst][n_d]
-    double* d_state;        // [3][n_inst][n_d]: guess vd, guess charge, committed charge
-    const double* lin_val;  // [n_inst][n_lin] operating-point values
-    const double* lin_val_tran;
+    // Parameter fields, packed: field f of device j (class-local order) of instance i is
+    // par[off[f] + i * istr[f] + j]; a field no instance overrides is stored once (istr = 0).
+    const double* par;
+    long long m1_off[M1_CORE_F + M1_CAPS_F];  // Mos1: 11 core fields, then 4 capacitances
+    int m1_istr[M1_CORE_F + M1_CAPS_F];
+    long long d_off[D_PAR_F];  // diode
+    int d_istr[D_PAR_F];
+    long long lin_off[2];  // linear element value: operating point, transient
+    int lin_istr[2];
+    double* m1_state;  // [14][n_inst][n_m1]: committed op (7) then guess (7)
+    double* d_state;   // [3][n_inst][n_d]: guess vd, guess charge, committed charge
     double* c_q;            // [n_inst][n_lin] committed capacitor charge
     double* x;              // [n_inst][n]
     int* status;            // [n_inst]
@@ -93,9 +99,8 @@ __device__ inline GLB T* glb(T* p) {
 }
 
 constexpr int WAVE = 64;
-constexpr int MAX_WAVES = 4;  // wavefronts per workgroup; the kernel's launch bound
+constexpr int MAX_WAVES = 8;  // wavefronts per workgroup; the kernel's launch bound
 constexpr size_t LDS_PER_CU = 160 * 1024;  // gfx950
-constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 3;
 constexpr int NPROF = 8;  // eval, gather, residual, factor (+forward sweep), backward sweep, update, commit, total
 constexpr int RUNNING = -1;
 
@@ -211,38 +216,59 @@ __device__ inline void apply(const Ctx& C, LDS const u16* code, const double (&v
     }
 }
 
+// Tied Mos1: stamps to entries (the source/drain swap), coinciding entries merged, then one batch.
+__device__ inline void apply_tied(const Ctx& C, LDS const u16* code, const M1Stamps& sv) {
+    double v[MOS1_NE];
+#pragma unroll
+    for (int e = 0; e < MOS1_NE; e++) v[e] = sv.rev ? sv.v[mos1_sigma(e)] : sv.v[e];
+    v[1] = ((v[1] + v[7]) + v[13]) + v[15];  // (s,s)
+    v[2] = v[2] + v[14];                     // (d,s)
+    v[3] = v[3] + v[12];                     // (s,d)
+    v[5] = v[5] + v[11];                     // (s,g)
+    v[8] = v[8] + v[10];                     // (g,s)
+    v[23] = v[23] + v[25];                   // rhs s
+    apply<MOS1_NE, false>(C, code, v, false);
+}
+
+__device__ inline int mos1_count(const TopoGPU& T) { return T.cls_start[CLS_MOS1T + 1] - T.cls_start[CLS_MOS1]; }
+
 // ---- device evaluation ---------------------------------------------------------------------------
-__device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
+__device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A,
+                                 bool tied) {
     const GroupGPU& G = *C.G;
-    const size_t s = (size_t)G.n_inst * C.T.n_cls[CLS_MOS1], o = (size_t)C.inst * C.T.n_cls[CLS_MOS1] + j;
-    GLB const double* p = glb(G.m1_core);
+    const int nm = mos1_count(C.T);
+    const size_t s = (size_t)G.n_inst * nm, o = (size_t)C.inst * nm + j;
+    GLB const double* par = glb(G.par);
+    auto F = [&](int f) { return par[G.m1_off[f] + (long long)C.inst * G.m1_istr[f] + j]; };
     Mos1Core P;
-    P.p = p[0 * s + o];
-    P.vt0_t = p[1 * s + o];
-    P.beta = p[2 * s + o];
-    P.phi_t = p[3 * s + o];
-    P.gamma = p[4 * s + o];
-    P.lambda = p[5 * s + o];
-    P.vtherm = p[6 * s + o];
-    P.isat_d = p[7 * s + o];
-    P.isat_s = p[8 * s + o];
-    P.grd = p[9 * s + o];
-    P.grs = p[10 * s + o];
+    P.p = F(0);
+    P.vt0_t = F(1);
+    P.beta = F(2);
+    P.phi_t = F(3);
+    P.gamma = F(4);
+    P.lambda = F(5);
+    P.vtherm = F(6);
+    P.isat_d = F(7);
+    P.isat_s = F(8);
+    P.grd = F(9);
+    P.grs = F(10);
     const double vd = xv(C.x, (int)dv[RD] - 1), vg = xv(C.x, (int)dv[RG] - 1), vs = xv(C.x, (int)dv[RS] - 1),
                  vb = xv(C.x, (int)dv[RB] - 1);
     M1Stamps sv;
     if (!(dv[6] & DEV_CAPS)) {
         mos1_load<false>(P, nullptr, nullptr, vd, vg, vs, vb, A.gmin, A.tstep, sv, nullptr);
-        apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
+        if (tied)
+            apply_tied(C, code, sv);
+        else
+            apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
         return;
     }
     // gate capacitance: capacitances and operating-point state from global memory
     Mos1Caps K;
-    GLB const double* kc = glb(G.m1_caps);
-    K.cox = kc[0 * s + o];
-    K.cgs_ov = kc[1 * s + o];
-    K.cgd_ov = kc[2 * s + o];
-    K.cgb_ov = kc[3 * s + o];
+    K.cox = F(M1_CORE_F + 0);
+    K.cgs_ov = F(M1_CORE_F + 1);
+    K.cgd_ov = F(M1_CORE_F + 2);
+    K.cgb_ov = F(M1_CORE_F + 3);
     Mos1State op, gs;
     GLB double* st = glb(G.m1_state);
     op.vgs = st[0 * s + o];
@@ -256,7 +282,10 @@ __device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS con
         mos1_load<true>(P, &K, &op, vd, vg, vs, vb, A.gmin, A.tstep, sv, &gs);
     else
         mos1_load<false>(P, &K, &op, vd, vg, vs, vb, A.gmin, A.tstep, sv, &gs);
-    apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
+    if (tied)
+        apply_tied(C, code, sv);
+    else
+        apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
     st[7 * s + o] = gs.vgs;
     st[8 * s + o] = gs.vgd;
     st[9 * s + o] = gs.vgb;
@@ -270,22 +299,23 @@ __device__ inline void eval_diode(const Ctx& C, int j, LDS const u16* dv, LDS co
                                   const Args& A) {
     const GroupGPU& G = *C.G;
     const size_t s = (size_t)G.n_inst * C.T.n_cls[CLS_DIODE], o = (size_t)C.inst * C.T.n_cls[CLS_DIODE] + j;
-    GLB const double* q = glb(G.d_par);
+    GLB const double* par = glb(G.par);
+    auto F = [&](int f) { return par[G.d_off[f] + (long long)C.inst * G.d_istr[f] + j]; };
     DiodeP P;
-    P.isat = q[0 * s + o];
-    P.vte = q[1 * s + o];
-    P.vcrit = q[2 * s + o];
-    P.gspr = q[3 * s + o];
-    P.bv = q[4 * s + o];
-    P.has_bv = q[5 * s + o];
-    P.cz = q[6 * s + o];
-    P.cz2 = q[7 * s + o];
-    P.dep = q[8 * s + o];
-    P.f1 = q[9 * s + o];
-    P.f3 = q[10 * s + o];
-    P.tt = q[11 * s + o];
-    P.vj = q[12 * s + o];
-    P.m = q[13 * s + o];
+    P.isat = F(0);
+    P.vte = F(1);
+    P.vcrit = F(2);
+    P.gspr = F(3);
+    P.bv = F(4);
+    P.has_bv = F(5);
+    P.cz = F(6);
+    P.cz2 = F(7);
+    P.dep = F(8);
+    P.f1 = F(9);
+    P.f3 = F(10);
+    P.tt = F(11);
+    P.vj = F(12);
+    P.m = F(13);
     GLB double* st = glb(G.d_state);
     const double vr = xv(C.x, (int)dv[DR_] - 1), vn = xv(C.x, (int)dv[DN_] - 1);
     DiodeGuess gs;
@@ -304,7 +334,8 @@ __device__ inline void eval_lin(const Ctx& C, int j, LDS const u16* dv, LDS cons
     Stamps<6> put;
     const int typ = dv[6] >> 4;
     const size_t o = (size_t)C.inst * C.T.n_cls[CLS_LIN] + j;
-    const double val = tran ? glb(G.lin_val_tran)[o] : glb(G.lin_val)[o];
+    const int f = tran ? 1 : 0;
+    const double val = glb(G.par)[G.lin_off[f] + (long long)C.inst * G.lin_istr[f] + j];
     if (typ == S21_COMP_R) {
         put(0, val);
         put(1, val);
@@ -346,8 +377,8 @@ __device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A
             LDS const u16* dv = I.dev + 8 * p;
             LDS const u16* code = I.code + code0 + (p - p0) * stride;
             const int j = j0 + (p - p0);
-            if (cls == CLS_MOS1)
-                eval_mos1(C, j, dv, code, tran, A);
+            if (cls_is_mos1(cls))
+                eval_mos1(C, j, dv, code, tran, A, cls == CLS_MOS1T);
             else if (cls == CLS_MOS0)
                 mos0_load((dv[6] & DEV_PMOS) ? -1.0 : 1.0, xv(C.x, (int)dv[RD] - 1), xv(C.x, (int)dv[RG] - 1),
                           xv(C.x, (int)dv[RS] - 1), A.gmin, Put{C.a, C.hub, code});
@@ -416,6 +447,18 @@ __device__ inline bool residual(const Ctx& C, bool act, double iabstol) {
     return bad;
 }
 
+// Division inside the factorisation and sweeps: exact in FP64 (as the reference divides); in the FP32
+// factorisation of mixed precision, which only preconditions the FP64 Newton update, the hardware
+// reciprocal (1 ulp) times the numerator.
+template <typename V>
+__device__ inline V ldiv(V a, V b) {
+    return a / b;
+}
+template <>
+__device__ inline float ldiv<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 // Factorisation with the forward sweep fused in.  A chunk of row i processes L entries (i, k) once row k
 // is final: l_ik = a_ik / u_kk, a_ij -= l_ik u_kj over row k's U entries, and c_i -= c_k l_ik -- for each
 // target in increasing k, the order of the reference's right-looking elimination and column-oriented
@@ -432,7 +475,7 @@ __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
             int u = h.w;
             for (int j = h.y; j < h.z; j++) {
                 const R4 L = r4(I.fl[j]);
-                const V lv = lu[L.x] / lu[L.y];
+                const V lv = ldiv<V>(lu[L.x], lu[L.y]);
                 lu[L.x] = lv;
                 const V ck = (V)C.c[L.z];
                 if (ck != V(0)) y = y - ck * lv;  // the sweep skips zero entries (sparse21/mod.rs:953)
@@ -466,7 +509,7 @@ __device__ inline void backward(const Ctx& C, bool act, LDS const V* lu) {
                 const V prod = (V)C.c[q.y] * lu[q.x];
                 y = y - prod;
             }
-            C.c[h.x] = (double)(y / lu[h.y]);
+            C.c[h.x] = (double)ldiv<V>(y, lu[h.y]);
         }
         wave_sync();
     }
@@ -476,13 +519,14 @@ __device__ inline void backward(const Ctx& C, bool act, LDS const V* lu) {
 __device__ inline void commit(const Ctx& C, bool act) {
     const GroupGPU& G = *C.G;
     const TopoGPU& T = C.T;
-    const int nl = T.n_cls[CLS_LIN], nd = T.n_cls[CLS_DIODE], nm = T.n_cls[CLS_MOS1];
+    const int nl = T.n_cls[CLS_LIN], nd = T.n_cls[CLS_DIODE], nm = mos1_count(T);
     for (int d = act ? C.sl : nl; d < nl; d += C.L) {
         LDS const u16* dv = C.I.dev + 8 * (T.cls_start[CLS_LIN] + d);
         if ((dv[6] >> 4) != S21_COMP_C) continue;
         const size_t o = (size_t)C.inst * nl + d;
         const double vd = xv(C.x, (int)dv[0] - 1) - xv(C.x, (int)dv[1] - 1);
-        glb(G.c_q)[o] = glb(G.lin_val)[o] * vd;  // guess.q = c * vd at the accepted voltages
+        const double cval = glb(G.par)[G.lin_off[0] + (long long)C.inst * G.lin_istr[0] + d];
+        glb(G.c_q)[o] = cval * vd;  // guess.q = c * vd at the accepted voltages
     }
     for (int d = act ? C.sl : nd; d < nd; d += C.L) {
         const size_t s = (size_t)G.n_inst * nd, o = (size_t)C.inst * nd + d;

@@ -569,15 +569,17 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
         case S21_COMP_MOS1: {
             const bool caps = m1_caps[im1++] != 0;
             const bool rdrs = nd[RDP] != nd[RD] || nd[RSP] != nd[RS];
+            const bool tied = !rdrs && nd[RS] == nd[RB];
             for (int e = 0; e < MOS1_NE; e++) {
                 if (mos1_entry_tr_only(e) && !(caps && t.tran)) continue;  // exactly zero
                 if (mos1_entry_r_only(e) && !rdrs) continue;             // grd = grs = 0
+                if (tied && mos1t_merged(e)) continue;                   // merged in registers
                 const int r = nd[mos1_entry_row(e)];
                 const int col = mos1_entry_col(e) < 0 ? -2 : nd[mos1_entry_col(e)];
                 if (mos1_entry_col(e) >= 0 && col < 0) continue;
                 add(e, r, col);
             }
-            cls_of[ci] = CLS_MOS1;
+            cls_of[ci] = tied ? CLS_MOS1T : CLS_MOS1;
             flags[ci] = caps && t.tran ? DEV_CAPS : 0;
             break;
         }
