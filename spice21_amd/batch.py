@@ -110,12 +110,21 @@ class Batch:
     # ---- analyses ----
     def op(self):
         check(lib().s21_batch_op(self.h))
+        self._ac = False
+        return self
+
+    def ac(self, fstart, fstop, npts):
+        """Operating point, then the small-signal sweep from fstart to fstop Hz (npts steps, log-spaced)."""
+        a = A.s21_ac_options(int(fstart), int(fstop), int(npts))
+        check(lib().s21_batch_ac(self.h, C.byref(a)))
+        self._ac = True
         return self
 
     def tran(self, tstep, tstop, ics=()):
         names = self.groups[0].cnet.var_names
         t = make_tran_options(tstep, tstop, ics, names)
         check(lib().s21_batch_tran(self.h, C.byref(t)))
+        self._ac = False
         return self
 
     # ---- results ----
@@ -145,7 +154,12 @@ class Batch:
         return out
 
     def wave(self, g=0, inst=0):
+        """Transient waveform [points, vars], or after ac() the complex sweep [frequencies, vars]."""
         n = lib().s21_batch_n_points(self.h)
+        if getattr(self, "_ac", False):
+            out = np.zeros((n, self.n_vars(g), 2))
+            check(lib().s21_batch_get_wave(self.h, g, inst, _dp(out)))
+            return out[:, :, 0] + 1j * out[:, :, 1]
         out = np.zeros((n, self.n_vars(g)))
         check(lib().s21_batch_get_wave(self.h, g, inst, _dp(out)))
         return out

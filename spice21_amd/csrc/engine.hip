@@ -65,6 +65,7 @@ struct GroupHost {
     // the last analysis
     Topology topo;
     std::unique_ptr<DevMem> d_topo, d_par, d_state, d_out;
+    std::unique_ptr<DevMem> d_acop;  // Mos1 small-signal operating points captured for AC
     GroupGPU gg{};
     std::vector<double> x;
     std::vector<int> status;
@@ -180,8 +181,17 @@ static void build_comps(GroupHost& g, const s21_tran_options* tr, Topology& t) {
 // Host half of the preparation: the frozen structure, then per-instance derived parameters in the
 // kernel's device order.
 struct HostPrep {
-    std::vector<double> m1_core, m1_caps, d_par, lin_val, lin_tran;
+    std::vector<double> m1_core, m1_caps, d_par, lin_val, lin_tran, lin_ac;
     int n_m1 = 0, n_d = 0, n_lin = 0;
+};
+
+// Inputs of an AC structure: the first angular frequency and each instance's Mos1 operating point
+// captured after the DC solution, [9][n_inst][n_dc] in that solution's device order (dc_comp: the
+// component of each).
+struct AcInput {
+    double omega0 = 0.0;
+    std::vector<double> acop;
+    std::vector<int> dc_comp;
 };
 
 // Parameter sets of component `ci` for instance i with the group's overrides applied.
@@ -199,10 +209,12 @@ static void inst_params(const GroupHost& g, const std::vector<const Override*>* 
     }
 }
 
-static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_options* tr, HostPrep& hp) {
+static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_options* tr, HostPrep& hp,
+                         const AcInput* ac = nullptr) {
     Topology& t = g.topo;
     t = Topology();
     build_comps(g, tr, t);
+    t.ac = ac != nullptr;
     topo_elements(t);
     const int NI = g.n_inst;
     std::unordered_map<int, std::vector<const Override*>> ov;
@@ -250,9 +262,46 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
             im++;
         }
     }
-    std::vector<double> x0(t.n, 0.0), vals, rhs;
-    host_assemble_op(t, h0, o, x0, vals, rhs);
-    topo_pivot(t, vals);
+    // AC: the captured operating points by component (Mos1 k in component order <- DC local index).
+    std::vector<int> dc_of_m1;
+    if (ac) {
+        std::unordered_map<int, int> dc_pos;
+        for (size_t j = 0; j < ac->dc_comp.size(); j++) dc_pos[ac->dc_comp[j]] = (int)j;
+        for (size_t i = 0; i < t.comps.size(); i++)
+            if (t.comps[i].type == S21_COMP_MOS1) dc_of_m1.push_back(dc_pos.at((int)i));
+    }
+    const int n_dc = ac ? (int)ac->dc_comp.size() : 0;
+    auto acop_of = [&](int k, int inst) {  // Mos1 k (component order), instance inst
+        const size_t s = (size_t)NI * n_dc, off = (size_t)inst * n_dc + dc_of_m1[k];
+        const double* v = ac->acop.data();
+        return Mos1AcOp{v[0 * s + off], v[1 * s + off], v[2 * s + off], v[3 * s + off], v[4 * s + off],
+                        v[5 * s + off], v[6 * s + off], v[7 * s + off], v[8 * s + off]};
+    };
+    if (ac) {  // pivot on instance 0's first AC matrix (analysis.rs:761-832: fstart, x = 0)
+        std::vector<Mos1AcOp> op0;
+        std::vector<double> grd0, grs0, lin0;
+        int k = 0, il = 0;
+        for (const Comp& c : t.comps) {
+            if (c.type == S21_COMP_MOS1) {
+                op0.push_back(acop_of(k, 0));
+                grd0.push_back(h0.m1[k].core.grd);
+                grs0.push_back(h0.m1[k].core.grs);
+                k++;
+            } else if (c.type == S21_COMP_R || c.type == S21_COMP_C || c.type == S21_COMP_V) {
+                lin0.push_back(c.type == S21_COMP_V ? c.acm : h0.lin_value[il]);
+                il++;
+            } else if (c.type == S21_COMP_I) {
+                il++;
+            }
+        }
+        std::vector<double> re, im;
+        host_assemble_ac(t, op0, grd0, grs0, lin0, ac->omega0, re, im);
+        topo_pivot_cx(t, re, im);
+    } else {
+        std::vector<double> x0(t.n, 0.0), vals, rhs;
+        host_assemble_op(t, h0, o, x0, vals, rhs);
+        topo_pivot(t, vals);
+    }
     topo_schedule(t, has_caps);
 
     // Per-instance parameters, SoA [field][instance][class-local device] in the kernel's device order.
@@ -270,6 +319,9 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
         const auto* L = ov_of(ci);
         const size_t s = (size_t)NI * n_m1;
         Mos1Derived D{};
+        int k_comp = 0;  // Mos1 index of this device in component order
+        if (ac)
+            for (int c2 = 0; c2 < ci; c2++) k_comp += t.comps[c2].type == S21_COMP_MOS1;
         for (int i = 0; i < NI; i++) {
             if (i == 0 || L) {
                 s21_params mp, ip;
@@ -277,9 +329,15 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
                 D = derive_mos1(mp, ip, o);
             }
             const size_t off = (size_t)i * n_m1 + d;
-            const double core[M1_CORE_F] = {D.core.p,      D.core.vt0_t,  D.core.beta,   D.core.phi_t,
-                                            D.core.gamma,  D.core.lambda, D.core.vtherm, D.core.isat_d,
-                                            D.core.isat_s, D.core.grd,    D.core.grs};
+            double core[M1_CORE_F] = {D.core.p,      D.core.vt0_t,  D.core.beta,   D.core.phi_t,
+                                      D.core.gamma,  D.core.lambda, D.core.vtherm, D.core.isat_d,
+                                      D.core.isat_s, D.core.grd,    D.core.grs};
+            if (ac) {  // AC fields: the operating point of load_ac, then the terminal conductances
+                const Mos1AcOp op = acop_of(k_comp, i);
+                const double a[M1_CORE_F] = {op.gm,  op.gds, op.gmbs,     op.gbs,     op.gbd,    op.cgs,
+                                             op.cgd, op.cgb, op.reversed, D.core.grd, D.core.grs};
+                for (int f = 0; f < M1_CORE_F; f++) core[f] = a[f];
+            }
             for (int f = 0; f < M1_CORE_F; f++) hp.m1_core[f * s + off] = core[f];
             const double cp[M1_CAPS_F] = {D.caps.cox, D.caps.cgs_ov, D.caps.cgd_ov, D.caps.cgb_ov};
             for (int f = 0; f < M1_CAPS_F; f++) hp.m1_caps[f * s + off] = cp[f];
@@ -306,6 +364,7 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
     }
     hp.lin_val.assign((size_t)NI * n_lin, 0.0);
     hp.lin_tran.assign((size_t)NI * n_lin, 0.0);
+    hp.lin_ac.assign((size_t)NI * n_lin, 0.0);
     for (int d = 0; d < n_lin; d++) {
         const int ci = t.dev_comp[t.cls_start[CLS_LIN] + d];
         const Comp& c = t.comps[ci];
@@ -316,6 +375,7 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
                 for (const Override* x : *L) v = vt = x->vals[i];
             hp.lin_val[(size_t)i * n_lin + d] = v;
             hp.lin_tran[(size_t)i * n_lin + d] = vt;
+            hp.lin_ac[(size_t)i * n_lin + d] = c.type == S21_COMP_V ? c.acm : v;  // AC value: g, c or acm
         }
     }
 }
@@ -336,9 +396,9 @@ static int auto_ipw(int total_inst, int simds) {
 }
 
 // Derive per-instance parameters, build the topology and upload everything for one analysis.
-static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr) {
+static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr, const AcInput* ac = nullptr) {
     HostPrep hp;
-    host_prepare(b->opts, g, tr, hp);
+    host_prepare(b->opts, g, tr, hp, ac);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
 
@@ -435,7 +495,8 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
         pack(hp.d_par, D_PAR_F, hp.n_d, g.gg.d_off, g.gg.d_istr);
         std::vector<double> lin(hp.lin_val);
         lin.insert(lin.end(), hp.lin_tran.begin(), hp.lin_tran.end());
-        pack(lin, 2, hp.n_lin, g.gg.lin_off, g.gg.lin_istr);
+        lin.insert(lin.end(), hp.lin_ac.begin(), hp.lin_ac.end());
+        pack(lin, 3, hp.n_lin, g.gg.lin_off, g.gg.lin_istr);
     }
     g.d_par.reset(new DevMem());
     g.d_par->alloc(std::max<size_t>(par.size(), 1) * sizeof(double));
@@ -474,15 +535,17 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     // Its launch shape is fixed here: one instance per wavefront asks the compiler for four resident
     // wavefronts per SIMD, more instances per wavefront for two.
     g.has_jit = false;
-    const bool want = b->jit > 0 || (b->jit == 0 && b->groups.size() == 1 && NI >= JIT_MIN_INST);
+    const bool want = !ac && (b->jit > 0 || (b->jit == 0 && b->groups.size() == 1 && NI >= JIT_MIN_INST));
     if (want && t.sym_status == S21_OK && jit_ops(t) <= JIT_MAX_OPS) {
         g.jit_ipw = b->ipw > 0 ? b->ipw : auto_ipw(NI, device_simds());
         g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED, g.jit_ipw == 1 ? 4 : 2));
         g.has_jit = true;
     }
     g.blob_bytes = blob_bytes;
-    g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) +
-                   (b->precision == S21_MIXED ? (size_t)t.nnz * sizeof(float) : 0);
+    const size_t vw = ac ? 2 : 1;  // values per entry: complex for AC
+    g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) * vw +
+                   (b->precision == S21_MIXED ? (size_t)t.nnz * sizeof(float) * vw : 0);
+    g.gg.acop = nullptr;
 }
 
 // Key of the structure an analysis needs: the operating point, or a transient with its IC set.
@@ -649,6 +712,8 @@ static Args base_args(const s21_batch* b, int mode) {
     A.waves = 1;
     A.ipw = 1;
     A.inst_bytes = 0;
+    A.capture = 0;
+    A.omega = nullptr;
     return A;
 }
 
@@ -818,10 +883,82 @@ int s21_batch_tran(s21_batch* b, const s21_tran_options* tr) {
 }
 
 int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
-    (void)a;
     return guard([&] {
-        if (!b) fail(S21_INVALID, "null batch");
-        fail(S21_ERR, "AC analysis is not available in this build");
+        if (!b || !a) fail(S21_INVALID, "null argument");
+        // Frequencies (analysis.rs:786-826): f *= fstep up to fstop, both ends included.
+        std::vector<double> freqs;
+        {
+            double f = (double)a->fstart;
+            const double fstop = (double)a->fstop;
+            const double fstep = std::pow(10.0, std::log10(fstop / f) / (double)a->npts);
+            while (f <= fstop) {
+                freqs.push_back(f);
+                if (f == fstop) break;
+                f = std::fmin(f * fstep, fstop);
+            }
+        }
+        if (freqs.empty()) fail(S21_INVALID, "empty frequency range");
+        const int64_t F = (int64_t)freqs.size();
+        for (GroupHost& g : b->groups)
+            for (const s21_comp& c : g.comps)
+                if (c.type != S21_COMP_R && c.type != S21_COMP_C && c.type != S21_COMP_V && c.type != S21_COMP_MOS1)
+                    fail(S21_ERR, "AC Not Implemented For This Component!");  // comps/mod.rs:86-88
+
+        // Operating point of every instance, each Mos1's small-signal operating point captured.
+        prepare_all(b, nullptr);
+        for (GroupHost& g : b->groups) {
+            const int nm = g.topo.cls_start[CLS_MOS1T + 1] - g.topo.cls_start[CLS_MOS1];
+            if (!g.d_acop) g.d_acop.reset(new DevMem());
+            g.d_acop->alloc(std::max<size_t>((size_t)M1_ACOP_F * g.n_inst * nm, 1) * sizeof(double));
+            g.gg.acop = (double*)g.d_acop->p;
+        }
+        Args A = base_args(b, MODE_OP);
+        A.capture = 1;
+        run(b, A, nullptr, 1);
+        std::vector<std::vector<int>> dc_status(b->groups.size());
+        std::vector<AcInput> acin(b->groups.size());
+        for (size_t gi = 0; gi < b->groups.size(); gi++) {
+            GroupHost& g = b->groups[gi];
+            const Topology& t = g.topo;
+            dc_status[gi].resize(g.n_inst);
+            HIPCHECK(hipMemcpy(dc_status[gi].data(), g.gg.status, g.n_inst * sizeof(int), hipMemcpyDeviceToHost));
+            AcInput& in = acin[gi];
+            in.omega0 = 2.0 * M_PI * freqs[0];
+            in.dc_comp.assign(t.dev_comp.begin() + t.cls_start[CLS_MOS1], t.dev_comp.begin() + t.cls_start[CLS_MOS1T + 1]);
+            in.acop.resize((size_t)M1_ACOP_F * g.n_inst * in.dc_comp.size());
+            if (!in.acop.empty())
+                HIPCHECK(hipMemcpy(in.acop.data(), g.d_acop->p, in.acop.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+
+        // AC structure (pivots on instance 0's matrix at fstart) and one launch over every frequency.
+        for (size_t gi = 0; gi < b->groups.size(); gi++) {
+            GroupHost& g = b->groups[gi];
+            g.prepared.clear();
+            prepare_group(b, g, nullptr, &acin[gi]);
+            g.prepared = "ac";
+        }
+        std::vector<double> omega(F);
+        for (int64_t p = 0; p < F; p++) omega[p] = 2.0 * M_PI * freqs[p];  // AcState::omega, analysis.rs:801
+        DevMem d_omega;
+        d_omega.alloc(F * sizeof(double));
+        HIPCHECK(hipMemcpy(d_omega.p, omega.data(), F * sizeof(double), hipMemcpyHostToDevice));
+        size_t total = 0;
+        for (GroupHost& g : b->groups) total += (size_t)g.n_inst * (size_t)F * g.topo.n * 2;
+        if (!b->d_wave) b->d_wave.reset(new DevMem());
+        b->d_wave->alloc(std::max<size_t>(total, 1) * sizeof(double));
+        Args B = base_args(b, MODE_AC);
+        B.npts = F;
+        B.omega = (const double*)d_omega.p;
+        run(b, B, (double*)b->d_wave->p, 2 * F);
+        b->fetched = false;
+        b->npts = F;
+        b->axis = freqs;
+        b->last_mode = MODE_AC;
+        // An instance whose operating point failed reports that failure.
+        fetch_results(b);
+        for (size_t gi = 0; gi < b->groups.size(); gi++)
+            for (int i = 0; i < b->groups[gi].n_inst; i++)
+                if (dc_status[gi][i] != S21_OK) b->groups[gi].status[i] = dc_status[gi][i];
     });
 }
 
@@ -870,12 +1007,15 @@ int s21_batch_get_axis(const s21_batch* b, double* out) {
 int s21_batch_get_wave(const s21_batch* b, int32_t g, int32_t inst, double* out) {
     return guard([&] {
         if (!b || g < 0 || g >= (int)b->groups.size() || !out) fail(S21_INVALID, "bad arguments");
-        if (b->last_mode != MODE_TRAN || !b->d_wave) fail(S21_INVALID, "no transient waveform");
+        if ((b->last_mode != MODE_TRAN && b->last_mode != MODE_AC) || !b->d_wave)
+            fail(S21_INVALID, "no transient or AC waveform");
         const GroupHost& gh = b->groups[g];
         if (inst < 0 || inst >= gh.n_inst) fail(S21_INVALID, "instance out of range");
+        const size_t vw = b->last_mode == MODE_AC ? 2 : 1;  // AC points are complex (re, im)
         size_t off = 0;  // groups' waveforms follow each other in group order (see run())
-        for (int gi = 0; gi < g; gi++) off += (size_t)b->groups[gi].n_inst * (size_t)b->npts * b->groups[gi].topo.n;
-        const size_t per = (size_t)b->npts * gh.topo.n;
+        for (int gi = 0; gi < g; gi++)
+            off += (size_t)b->groups[gi].n_inst * (size_t)b->npts * b->groups[gi].topo.n * vw;
+        const size_t per = (size_t)b->npts * gh.topo.n * vw;
         HIPCHECK(hipMemcpy(out, (double*)b->d_wave->p + off + (size_t)inst * per, per * sizeof(double),
                            hipMemcpyDeviceToHost));
     });

@@ -46,6 +46,7 @@ struct Topology {
     std::vector<int> kinds;
     std::vector<Comp> comps;
     bool tran = false;  // built for a transient (capacitor and charge entries live)
+    bool ac = false;    // built for AC: small-signal stamps (capacitance entries live, no Mos1 RHS)
 
     // matrix elements in Matrix::make creation order (external coordinates)
     std::vector<int> erow, ecol;
@@ -104,6 +105,8 @@ struct Topology {
 void topo_elements(Topology& t);
 void topo_pivot(Topology& t, const std::vector<double>& first_values);
 void topo_pivot_search(Topology& t, const std::vector<double>& first_values);
+// The same on the complex matrix of the first AC iteration (Matrix<Complex<f64>>: magnitudes are norms).
+void topo_pivot_cx(Topology& t, const std::vector<double>& re, const std::vector<double>& im);
 void topo_schedule(Topology& t, const std::vector<uint8_t>& mos1_has_caps);
 void topo_records(Topology& t);
 
@@ -115,5 +118,11 @@ struct HostInstance {
 };
 void host_assemble_op(const Topology& t, const HostInstance& h, const s21_options& o, const std::vector<double>& x,
                       std::vector<double>& vals, std::vector<double>& rhs);
+// Reference-order AC assembly (analysis.rs:237-251) at angular frequency omega for the pivot search:
+// acop[k] / grd[k] / grs[k] per Mos1 in component order, lin_ac per linear component (R: g, C: c,
+// V: acm).
+void host_assemble_ac(const Topology& t, const std::vector<Mos1AcOp>& acop, const std::vector<double>& grd,
+                      const std::vector<double>& grs, const std::vector<double>& lin_ac, double omega,
+                      std::vector<double>& re, std::vector<double>& im);
 
 }  // namespace s21

@@ -62,9 +62,10 @@ struct GroupGPU {
     int m1_istr[M1_CORE_F + M1_CAPS_F];
     long long d_off[D_PAR_F];  // diode
     int d_istr[D_PAR_F];
-    long long lin_off[2];  // linear element value: operating point, transient
-    int lin_istr[2];
+    long long lin_off[3];  // linear element value: operating point, transient, AC (R: g, C: c, V: acm)
+    int lin_istr[3];
     double* m1_state;  // [14][n_inst][n_m1]: committed op (7) then guess (7)
+    double* acop;      // [9][n_inst][n_m1] small-signal operating point (capture), or nullptr
     double* d_state;   // [3][n_inst][n_d]: guess vd, guess charge, committed charge
     double* c_q;            // [n_inst][n_lin] committed capacitor charge
     double* x;              // [n_inst][n]
@@ -75,7 +76,7 @@ struct GroupGPU {
     long long* prof;        // [n_inst][8] phase cycle counters (profiling mode) or nullptr
 };
 
-enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2 };
+enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2, MODE_AC = 3 };
 struct Args {
     int mode;
     int max_iter;
@@ -87,6 +88,8 @@ struct Args {
     int waves;       // wavefronts per workgroup
     int ipw;         // instances per wavefront (1, 2, 4, 8 or 16)
     int inst_bytes;  // LDS bytes per instance
+    int capture;     // operating point: store each Mos1's small-signal operating point for AC
+    const double* omega;  // AC: angular frequency of each point
 };
 
 // Address spaces: pointers read out of descriptors carry no address space, so they are re-qualified in
@@ -624,6 +627,359 @@ __device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long lon
     return st == RUNNING ? S21_CONVERGENCE : st;
 }
 
+// ============================================================================================
+// Small-signal operating point (for AC) and the AC analysis
+// ============================================================================================
+struct NullSink {
+    __device__ void operator()(int, double) const {}
+};
+
+// After the operating point: each Mos1's conductances and Meyer capacitances at the solution (the
+// op that Mos1::commit keeps, comps/mos.rs:904-907).
+__device__ inline void capture_acop(const Ctx& C, bool act, const Args& A) {
+    const GroupGPU& G = *C.G;
+    const TopoGPU& T = C.T;
+    const int nm = T.cls_start[CLS_MOS1T + 1] - T.cls_start[CLS_MOS1];
+    GLB const double* par = glb(G.par);
+    for (int j = act ? C.sl : nm; j < nm; j += C.L) {
+        LDS const u16* dv = C.I.dev + 8 * (T.cls_start[CLS_MOS1] + j);
+        auto F = [&](int f) { return par[G.m1_off[f] + (long long)C.inst * G.m1_istr[f] + j]; };
+        const Mos1Core P{F(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7), F(8), F(9), F(10)};
+        const Mos1Caps K{F(M1_CORE_F + 0), F(M1_CORE_F + 1), F(M1_CORE_F + 2), F(M1_CORE_F + 3)};
+        Mos1AcOp op;
+        mos1_load<false>(P, &K, nullptr, xv(C.x, (int)dv[RD] - 1), xv(C.x, (int)dv[RG] - 1),
+                         xv(C.x, (int)dv[RS] - 1), xv(C.x, (int)dv[RB] - 1), A.gmin, A.tstep, NullSink{}, nullptr,
+                         &op);
+        const size_t s = (size_t)G.n_inst * nm, o = (size_t)C.inst * nm + j;
+        GLB double* out = glb(G.acop);
+        const double v[M1_ACOP_F] = {op.gm, op.gds, op.gmbs, op.gbs, op.gbd, op.cgs, op.cgd, op.cgb, op.reversed};
+        for (int f = 0; f < M1_ACOP_F; f++) out[f * s + o] = v[f];
+    }
+}
+
+// Complex values with num-complex arithmetic (the reference's Complex<f64>; division by the naive
+// formula, norm = hypot).
+template <typename R>
+struct Z {
+    R re, im;
+};
+using zd = Z<double>;
+using zf = Z<float>;
+template <typename R>
+__device__ inline Z<R> operator+(Z<R> a, Z<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R>
+__device__ inline Z<R> operator-(Z<R> a, Z<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R>
+__device__ inline Z<R> operator*(Z<R> a, Z<R> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline zd zdiv(zd a, zd b) {
+    const double ns = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
+}
+__device__ inline zf zdiv(zf a, zf b) {  // FP32 factorisation of mixed precision: reciprocal of the norm
+    const float r = __builtin_amdgcn_rcpf(b.re * b.re + b.im * b.im);
+    return {(a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r};
+}
+template <typename R>
+__device__ inline bool zzero(Z<R> a) { return a.re == R(0) && a.im == R(0); }
+__device__ inline double znorm(zd a) { return hypot(a.re, a.im); }
+template <typename V>
+__device__ inline V zcvt(zd a) { return V{(decltype(V::re))a.re, (decltype(V::re))a.im}; }
+template <typename V>
+__device__ inline zd zwide(V a) { return zd{(double)a.re, (double)a.im}; }
+
+// Complex arrays live in LDS as interleaved (re, im) pairs.
+struct AcCtx {
+    LDS double *x, *c, *a, *rhs, *hub, *part;
+    LDS float* lu32;
+};
+template <typename R>
+__device__ inline Z<R> ldz(LDS const R* p, int i) {
+    return Z<R>{p[2 * i], p[2 * i + 1]};
+}
+template <typename R>
+__device__ inline void stz(LDS R* p, int i, Z<R> v) {
+    p[2 * i] = v.re;
+    p[2 * i + 1] = v.im;
+}
+
+// A device's complex stamps, applied as one batch like the real ones (distinct non-hub slots).
+template <int N, bool SIGMA>
+__device__ inline void apply_z(const AcCtx& Z_, LDS const u16* code, const double (&vr)[N], const double (&vi)[N],
+                               bool rev) {
+    constexpr int B = 8;
+#pragma unroll
+    for (int k0 = 0; k0 < N; k0 += B) {
+        int c[B];
+        zd cur[B];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) c[k] = code[(SIGMA && rev) ? mos1_sigma(k0 + k) : k0 + k];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) cur[k] = c[k] < 0x8000 ? ldz(Z_.a, c[k]) : zd{0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            if (k0 + k >= N) continue;
+            const zd v{vr[k0 + k], vi[k0 + k]};
+            if (c[k] < 0x8000)
+                stz(Z_.a, c[k], cur[k] + v);
+            else if (c[k] != 0xFFFF)
+                stz(Z_.hub, c[k] & 0x7FFF, v);
+        }
+    }
+}
+struct AcM1Stamps {
+    static constexpr bool raw_stamps = true;
+    double re[MOS1_NE], im[MOS1_NE];
+    bool rev;
+    __device__ void stamp(int e, bool r, double a, double b) {
+        re[e] = re[e] + a;
+        im[e] = im[e] + b;
+        rev = r;
+    }
+};
+
+__device__ inline void eval_ac_mos1(const Ctx& C, const AcCtx& Z_, int j, LDS const u16* code, double omega,
+                                    bool tied) {
+    const GroupGPU& G = *C.G;
+    GLB const double* par = glb(G.par);
+    auto F = [&](int f) { return par[G.m1_off[f] + (long long)C.inst * G.m1_istr[f] + j]; };
+    const Mos1AcOp op{F(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7), F(8)};
+    AcM1Stamps sv;
+#pragma unroll
+    for (int e = 0; e < MOS1_NE; e++) sv.re[e] = sv.im[e] = 0.0;
+    sv.rev = false;
+    mos1_load_ac(op, F(9), F(10), omega, sv);
+    if (!tied) {
+        apply_z<MOS1_NE, true>(Z_, code, sv.re, sv.im, sv.rev);
+        return;
+    }
+    double r[MOS1_NE], i[MOS1_NE];
+#pragma unroll
+    for (int e = 0; e < MOS1_NE; e++) {
+        r[e] = sv.rev ? sv.re[mos1_sigma(e)] : sv.re[e];
+        i[e] = sv.rev ? sv.im[mos1_sigma(e)] : sv.im[e];
+    }
+    const int from[6][4] = {{1, 7, 13, 15}, {2, 14, -1, -1}, {3, 12, -1, -1}, {5, 11, -1, -1}, {8, 10, -1, -1},
+                            {23, 25, -1, -1}};
+#pragma unroll
+    for (int g = 0; g < 6; g++)
+#pragma unroll
+        for (int k = 1; k < 4; k++)
+            if (from[g][k] >= 0) {
+                r[from[g][0]] = r[from[g][0]] + r[from[g][k]];
+                i[from[g][0]] = i[from[g][0]] + i[from[g][k]];
+            }
+    apply_z<MOS1_NE, false>(Z_, code, r, i, false);
+}
+
+__device__ inline void eval_ac_lin(const Ctx& C, const AcCtx& Z_, int j, LDS const u16* dv, LDS const u16* code,
+                                   double omega) {
+    const GroupGPU& G = *C.G;
+    const int typ = dv[6] >> 4;
+    const double val = glb(G.par)[G.lin_off[2] + (long long)C.inst * G.lin_istr[2] + j];
+    double r[6] = {0, 0, 0, 0, 0, 0}, i[6] = {0, 0, 0, 0, 0, 0};
+    if (typ == S21_COMP_R) {  // comps/mod.rs:311-322
+        r[0] = val;
+        r[1] = val;
+        r[2] = -val;
+        r[3] = -val;
+    } else if (typ == S21_COMP_C) {  // comps/mod.rs:223-238
+        const double wc = omega * val;
+        i[0] = wc;
+        i[1] = wc;
+        i[2] = -wc;
+        i[3] = -wc;
+    } else {  // Vsrc, comps/mod.rs:139-150
+        r[0] = 1.0;
+        r[1] = 1.0;
+        r[2] = -1.0;
+        r[3] = -1.0;
+        r[4] = val;
+    }
+    apply_z<6, false>(Z_, code, r, i, false);
+}
+
+// Complex A|RHS at angular frequency omega (Solver<Complex>::update, analysis.rs:237-251).
+__device__ inline void assemble_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega) {
+    const TopoGPU& T = C.T;
+    const Idx& I = C.I;
+    const int ns = T.nnz + T.n;
+    for (int k = act ? C.sl : ns; k < ns; k += C.L) stz(Z_.a, k, zd{0.0, 0.0});
+    wave_sync();
+    for (int cc = 0; cc < T.n_col; cc++) {
+        const int p0 = I.col[cc], p1 = I.col[cc + 1];
+        LDS const u16* cd = I.col + T.n_col + 1 + 3 * cc;
+        const int cls = cd[0], j0 = cd[1], code0 = cd[2];
+        const int stride = code_stride(cls);
+        for (int p = act ? p0 + C.sl : p1; p < p1; p += C.L) {
+            LDS const u16* code = I.code + code0 + (p - p0) * stride;
+            const int j = j0 + (p - p0);
+            if (cls_is_mos1(cls))
+                eval_ac_mos1(C, Z_, j, code, omega, cls == CLS_MOS1T);
+            else
+                eval_ac_lin(C, Z_, j, I.dev + 8 * p, code, omega);
+        }
+        wave_sync();
+    }
+    for (int t = act ? C.sl : T.n_gt; t < T.n_gt; t += C.L) {
+        const R4 r = r4(I.gt[t]);
+        zd acc{0.0, 0.0};
+        for (int k = r.y; k < r.z; k++) acc = acc + ldz(Z_.hub, I.gidx[k]);
+        if (r.w == 0xFFFF)
+            stz(Z_.a, r.x, acc);
+        else
+            stz(Z_.part, r.w, acc);
+    }
+    wave_sync();
+    for (int t = act ? C.sl : T.n_gc; t < T.n_gc; t += C.L) {
+        const R4 r = r4(I.gc[t]);
+        zd acc = ldz(Z_.part, r.y);
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + ldz(Z_.part, p);
+        stz(Z_.a, r.x, acc);
+    }
+}
+
+// Complex residual; true when this lane saw |res| >= 1e-9 (the AC test is strict, analysis.rs:268-270).
+__device__ inline bool residual_ac(const Ctx& C, const AcCtx& Z_, bool act) {
+    const Idx& I = C.I;
+    bool bad = false;
+    for (int t = act ? C.sl : C.T.n_rt; t < C.T.n_rt; t += C.L) {
+        const R4 r = r4(I.rt[t]);
+        zd acc{0.0, 0.0};
+        for (int e = r.y; e < r.z; e++) {
+            const R2 q = r2(I.re[e]);
+            acc = acc + ldz(Z_.a, q.x) * ldz(Z_.x, q.y);
+        }
+        if (r.w == 0xFFFF) {
+            const zd v = ldz(Z_.rhs, I.ri2e[r.x]) - acc;
+            stz(Z_.c, r.x, v);
+            bad = bad || !(znorm(v) < 1e-9);
+        } else {
+            stz(Z_.part, r.w, acc);
+        }
+    }
+    wave_sync();
+    for (int t = act ? C.sl : C.T.n_rc; t < C.T.n_rc; t += C.L) {
+        const R4 r = r4(I.rc[t]);
+        zd acc = ldz(Z_.part, r.y);
+        for (int p = r.y + 1; p < r.z; p++) acc = acc + ldz(Z_.part, p);
+        const zd v = ldz(Z_.rhs, I.ri2e[r.x]) - acc;
+        stz(Z_.c, r.x, v);
+        bad = bad || !(znorm(v) < 1e-9);
+    }
+    return bad;
+}
+
+template <typename R>
+__device__ inline bool factor_forward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS R* lu) {
+    using V = Z<R>;
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_clvl; l++) {
+        const int e = I.clvl[l + 1];
+        for (int ch = act ? I.clvl[l] + C.sl : e; ch < e; ch += C.L) {
+            const R4 h = r4(I.fch[ch]);
+            V y = zcvt<V>(ldz(Z_.c, h.x));
+            int u = h.w;
+            for (int j = h.y; j < h.z; j++) {
+                const R4 L = r4(I.fl[j]);
+                const V lv = zdiv(ldz(lu, L.x), ldz(lu, L.y));
+                stz(lu, L.x, lv);
+                const V ck = zcvt<V>(ldz(Z_.c, L.z));
+                if (!zzero(ck)) y = y - ck * lv;
+                for (int m = 0; m < L.w; m++, u++) {
+                    const R2 q = r2(I.fu[u]);
+                    stz(lu, q.y, ldz(lu, q.y) - ldz(lu, q.x) * lv);
+                }
+            }
+            stz(Z_.c, h.x, zwide(y));
+        }
+        wave_sync();
+    }
+    bool zero_pivot = false;
+    for (int i = act ? C.sl : C.T.n; i < C.T.n - 1; i += C.L)
+        if (zzero(ldz(lu, I.diag[i]))) zero_pivot = true;
+    return zero_pivot;
+}
+
+template <typename R>
+__device__ inline void backward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS const R* lu) {
+    using V = Z<R>;
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_blvl; l++) {
+        const int e = I.blvl[l + 1];
+        for (int r = act ? I.blvl[l] + C.sl : e; r < e; r += C.L) {
+            const R4 h = r4(I.bro[r]);
+            V y = zcvt<V>(ldz(Z_.c, h.x));
+            for (int k = h.z; k < h.w; k++) {
+                const R2 q = r2(I.bu[k]);
+                y = y - zcvt<V>(ldz(Z_.c, q.y)) * ldz(lu, q.x);
+            }
+            stz(Z_.c, h.x, zwide(zdiv(y, ldz(lu, h.y))));
+        }
+        wave_sync();
+    }
+}
+
+// Complex Newton at one frequency (Solver<Complex>::solve, analysis.rs:253-304): at most 20 iterations,
+// converged when every |dx| < 1e-3 and every |res| < 1e-9, |dx| clamped to 1.
+template <bool MIXED>
+__device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, long long& iters) {
+    const TopoGPU& T = C.T;
+    int st = act ? RUNNING : S21_OK;
+    bool prev_ok = true;  // dx starts at zero
+    for (int it = 0; it < 20; it++) {
+        if (!wave_any(act)) break;
+        if (act) iters++;
+        assemble_ac(C, Z_, act, omega);
+        wave_sync();
+        const bool bad = seg_any(C, residual_ac(C, Z_, act));
+        wave_sync();
+        const bool conv = act && prev_ok && !bad;
+        if (conv) {  // Component::commit changes nothing the next frequency reads
+            st = S21_OK;
+            act = false;
+        }
+        if (act && T.sym_status != S21_OK) {
+            st = T.sym_status;
+            act = false;
+        }
+        if (!wave_any(act)) break;
+        bool zero;
+        if (MIXED) {
+            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
+            wave_sync();
+            zero = seg_any(C, factor_forward_ac<float>(C, Z_, act, Z_.lu32));
+        } else {
+            zero = seg_any(C, factor_forward_ac<double>(C, Z_, act, Z_.a));
+        }
+        if (act && zero) {
+            st = S21_PIVOT;
+            act = false;
+        }
+        if (MIXED)
+            backward_ac<float>(C, Z_, act, Z_.lu32);
+        else
+            backward_ac<double>(C, Z_, act, Z_.a);
+        double mx = 0.0;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            const double d = znorm(ldz(Z_.c, C.I.ce2i[k]));
+            mx = (d > mx) ? d : mx;
+        }
+        mx = seg_max(C, mx);
+        bool ok = true;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            zd dx = ldz(Z_.c, C.I.ce2i[k]);
+            if (mx > 1.0) dx = zd{dx.re * 1.0 / mx, dx.im * 1.0 / mx};
+            ok = ok && znorm(dx) < 1e-3;
+            stz(Z_.x, k, ldz(Z_.x, k) + dx);
+        }
+        prev_ok = !seg_any(C, !ok);
+        wave_sync();
+    }
+    return st == RUNNING ? S21_CONVERGENCE : st;
+}
+
 // Workgroup = A.waves wavefronts x A.ipw instances, sharing one LDS copy of the group's records.
 template <bool MIXED, class LU>
 __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_groups, const Args& A) {
@@ -682,6 +1038,37 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     C.I.dev = b16 + T.o_dev;
     C.I.code = b16 + T.o_code;
     LDS double* dbase = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) + (size_t)slot * A.inst_bytes);
+    if (A.mode == MODE_AC) {  // complex: x | c | A | RHS | hub | partials (| complex-float LU)
+        AcCtx Z_;
+        Z_.x = dbase;
+        Z_.c = Z_.x + 2 * T.n;
+        Z_.a = Z_.c + 2 * T.n;
+        Z_.rhs = Z_.a + 2 * T.nnz;
+        Z_.hub = Z_.rhs + 2 * T.n;
+        Z_.part = Z_.hub + 2 * T.n_hub;
+        Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
+        C.x = Z_.x;
+        for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});  // analysis.rs:215-233
+        wave_sync();
+        long long iters = 0;
+        bool alive = valid;
+        int st = S21_OK;
+        for (long long pt = 0; pt < A.npts; pt++) {
+            if (!wave_any(alive)) break;
+            const int s2 = newton_ac<MIXED>(C, Z_, alive, A.omega[pt], iters);
+            if (alive) {
+                st = s2;
+                alive = s2 == S21_OK;
+            }
+            zd* wv = (zd*)G.wave + ((size_t)inst * (size_t)A.npts + pt) * T.n;
+            for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = ldz(Z_.x, k);
+        }
+        if (valid && C.sl == 0) {
+            G.status[inst] = st;
+            G.iters[inst] = iters;
+        }
+        return;
+    }
     C.x = dbase;
     C.c = C.x + T.n;
     C.a = C.c + T.n;
@@ -704,6 +1091,7 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     const long long tstart = A.prof ? cycles() : 0;
     long long iters = 0;
     int st = newton<MIXED, LU>(C, valid, false, A, iters, pc);
+    if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
     if (A.mode == MODE_TRAN) {
         bool alive = valid && st == S21_OK;
         double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;

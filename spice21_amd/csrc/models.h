@@ -69,6 +69,13 @@ struct RawSink : std::false_type {};
 template <class P>
 struct RawSink<P, std::void_t<decltype(P::raw_stamps)>> : std::true_type {};
 
+// The operating-point fields Mos1::load_ac reads (comps/mos.rs:914-968): small-signal conductances and
+// the intrinsic Meyer capacitances of the last load, with the source/drain orientation.
+struct Mos1AcOp {
+    double gm, gds, gmbs, gbs, gbd, cgs, cgd, cgb, reversed;
+};
+constexpr int M1_ACOP_F = 9;
+
 // Backward-Euler companion (analysis.rs:420-428).
 struct Integ {
     double g, i, rhs;
@@ -88,7 +95,7 @@ __host__ __device__ inline Integ be_integ(double dq, double dq_dv, double vguess
 template <bool TRAN, class Put>
 __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* caps, const Mos1State* op, double vd_,
                                           double vg_, double vs_, double vb_, double gmin, double dt, Put&& put,
-                                          Mos1State* guess) {
+                                          Mos1State* guess, Mos1AcOp* acop = nullptr) {
     const double p = P.p;
     const bool reversed = p * (vd_ - vs_) < 0.0;
     const double vd = reversed ? vs_ : vd_;
@@ -170,6 +177,11 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
             trgb_g = b.g;
             trgb_rhs = b.rhs;
         }
+        if (acop) {
+            acop->cgs = cgs1;
+            acop->cgd = cgd1;
+            acop->cgb = cgb1;
+        }
         if (guess) {
             guess->vgs = vgs;
             guess->vgd = vgd;
@@ -182,6 +194,15 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
     }
     const double irhs = ids - gm * vgs - gds * vds;
     const double grd = P.grd, grs = P.grs;
+    if (acop) {
+        acop->gm = gm;
+        acop->gds = gds;
+        acop->gmbs = gmbs;
+        acop->gbs = gbs;
+        acop->gbd = gbd;
+        acop->reversed = reversed ? 1.0 : 0.0;
+        if (!caps) acop->cgs = acop->cgd = acop->cgb = 0.0;
+    }
     auto E = [&](int k, double v) {
         if constexpr (RawSink<std::remove_reference_t<Put>>::value)
             put.stamp(k, reversed, v);
@@ -214,6 +235,46 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
     E(23, p * (irhs + ibs_rhs + trgs_rhs));
     E(24, -p * (trgs_rhs + trgb_rhs + trgd_rhs));
     E(25, -p * (ibd_rhs + ibs_rhs - trgb_rhs));
+}
+
+// Mos1::load_ac (comps/mos.rs:914-968): 23 complex stamps from the committed operating point, sent to
+// the DC entry list (stamp k of the reference list goes to entry ac_entry(k), sigma-swapped when the
+// operating point is reversed).  The (g, dr) admittance is stamped twice (mos.rs:951 and :954), so
+// entry 9 receives two stamps.  put(entry, re, im), or stamp(k-entry, reversed, re, im) for a raw sink.
+template <class Put>
+__host__ __device__ inline void mos1_load_ac(const Mos1AcOp& op, double grd, double grs, double omega, Put&& put) {
+    const double gcgs = omega * op.cgs, gcgd = omega * op.cgd, gcgb = omega * op.cgb;
+    const double gm = op.gm, gds = op.gds, gmbs = op.gmbs, gbs = op.gbs, gbd = op.gbd;
+    const bool rev = op.reversed != 0.0;
+    auto E = [&](int e, double re, double im) {
+        if constexpr (RawSink<std::remove_reference_t<Put>>::value)
+            put.stamp(e, rev, re, im);
+        else
+            put(rev ? mos1_sigma(e) : e, re, im);
+    };
+    E(0, gds + grd + gbd, gcgd);
+    E(1, gm + gds + grs + gbs + gmbs, gcgs);
+    E(2, -gm - gds - gmbs, 0.0);
+    E(3, -gds, 0.0);
+    E(4, gm, -gcgd);
+    E(5, -gm, -gcgs);
+    E(6, 0.0, gcgd + gcgs + gcgb);
+    E(7, gbd + gbs, gcgb);
+    E(8, 0.0, -gcgb);
+    E(9, 0.0, -gcgd);
+    E(10, 0.0, -gcgs);
+    E(11, 0.0, -gcgb);
+    E(9, 0.0, -gcgd);
+    E(12, -gbd, 0.0);
+    E(13, -gbs, 0.0);
+    E(14, -gbd + gmbs, 0.0);
+    E(15, -gbs - gmbs, 0.0);
+    E(16, -grd, 0.0);
+    E(17, -grd, 0.0);
+    E(18, grd, 0.0);
+    E(19, -grs, 0.0);
+    E(20, -grs, 0.0);
+    E(21, grs, 0.0);
 }
 
 // ---- Mos0 (comps/mos.rs:1051-1099) -------------------------------------------------------------

@@ -172,14 +172,85 @@ void host_assemble_op(const Topology& t, const HostInstance& h, const s21_option
     }
 }
 
+void host_assemble_ac(const Topology& t, const std::vector<Mos1AcOp>& acop, const std::vector<double>& grd,
+                      const std::vector<double>& grs, const std::vector<double>& lin_ac, double omega,
+                      std::vector<double>& re, std::vector<double>& im) {
+    std::unordered_map<int64_t, int> elem;
+    for (size_t k = 0; k < t.erow.size(); k++) elem[key(t.erow[k], t.ecol[k], t.n)] = (int)k;
+    re.assign(t.erow.size(), 0.0);
+    im.assign(t.erow.size(), 0.0);
+    auto addm = [&](int r, int c, double vr, double vi) {
+        if (r < 0 || c < 0) return;
+        const int e = elem.at(key(r, c, t.n));
+        re[e] += vr;
+        im[e] += vi;
+    };
+    int im1 = 0, il = 0;
+    for (const Comp& c : t.comps) {
+        const int* nd = c.node;
+        switch (c.type) {
+        case S21_COMP_MOS1: {
+            const int k = im1++;
+            mos1_load_ac(acop[k], grd[k], grs[k], omega, [&](int e, double vr, double vi) {
+                addm(nd[mos1_entry_row(e)], nd[mos1_entry_col(e)], vr, vi);
+            });
+            break;
+        }
+        case S21_COMP_R: {  // comps/mod.rs:311-322
+            const double g = lin_ac[il++];
+            addm(nd[0], nd[0], g, 0.0);
+            addm(nd[1], nd[1], g, 0.0);
+            addm(nd[0], nd[1], -g, 0.0);
+            addm(nd[1], nd[0], -g, 0.0);
+            break;
+        }
+        case S21_COMP_C: {  // comps/mod.rs:223-238
+            const double wc = omega * lin_ac[il++];
+            addm(nd[0], nd[0], 0.0, wc);
+            addm(nd[1], nd[1], 0.0, wc);
+            addm(nd[0], nd[1], 0.0, -wc);
+            addm(nd[1], nd[0], 0.0, -wc);
+            break;
+        }
+        case S21_COMP_V:  // comps/mod.rs:139-150 (the RHS does not enter the pivot search)
+            il++;
+            addm(nd[0], nd[2], 1.0, 0.0);
+            addm(nd[2], nd[0], 1.0, 0.0);
+            addm(nd[1], nd[2], -1.0, 0.0);
+            addm(nd[2], nd[1], -1.0, 0.0);
+            break;
+        default:
+            fail(S21_ERR, "AC Not Implemented For This Component!");
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Pivot search (restated from sparse21/mod.rs:647-919).
 // ------------------------------------------------------------------------------------------------
 namespace {
+// Complex values of the AC matrix, with the arithmetic of num-complex (naive division) that the
+// reference's Matrix<Complex<f64>> uses; magnitudes are the norm (hypot).
+struct HCx {
+    double re = 0.0, im = 0.0;
+};
+inline HCx operator+(HCx a, HCx b) { return {a.re + b.re, a.im + b.im}; }
+inline HCx operator-(HCx a, HCx b) { return {a.re - b.re, a.im - b.im}; }
+inline HCx operator*(HCx a, HCx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline HCx operator/(HCx a, HCx b) {
+    const double ns = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
+}
+inline double mag(double v) { return std::fabs(v); }
+inline double mag(HCx v) { return std::hypot(v.re, v.im); }
+inline bool is_zero(double v) { return v == 0.0; }
+inline bool is_zero(HCx v) { return v.re == 0.0 && v.im == 0.0; }
+
+template <class T>
 struct PivotState {
     int n;
     std::vector<int> orow, ocol;     // element -> external (original) row / col
-    std::vector<double> val;
+    std::vector<T> val;
     std::vector<std::vector<int>> rows, cols;  // external row / col -> element ids
     std::unordered_map<int64_t, int> at;       // (external row, external col) -> element
     std::vector<int> r_i2e, r_e2i, c_i2e, c_e2i;
@@ -191,7 +262,7 @@ struct PivotState {
     }
     int irow(int e) const { return r_e2i[orow[e]]; }
     int icol(int e) const { return c_e2i[ocol[e]]; }
-    int add(int orr, int occ, double v) {
+    int add(int orr, int occ, T v) {
         int e = (int)orow.size();
         orow.push_back(orr);
         ocol.push_back(occ);
@@ -221,11 +292,11 @@ struct PivotState {
     }
     int max_of(const std::vector<int>& lst) const {  // first element with the largest |value|
         int best = lst[0];
-        double bv = fabs(val[best]);
+        double bv = mag(val[best]);
         for (size_t i = 1; i < lst.size(); i++)
-            if (fabs(val[lst[i]]) > bv) {
+            if (mag(val[lst[i]]) > bv) {
                 best = lst[i];
-                bv = fabs(val[lst[i]]);
+                bv = mag(val[lst[i]]);
             }
         return best;
     }
@@ -240,17 +311,17 @@ struct PivotState {
             col_from(k, n0, tmp);
             if (tmp.empty()) continue;
             int mx = max_of(tmp);
-            double threshold = 1e-3 * fabs(val[mx]) + 0.0;
-            if (fabs(val[d]) < threshold) continue;
+            double threshold = 1e-3 * mag(val[mx]) + 0.0;
+            if (mag(val[d]) < threshold) continue;
             long m = mark(d);
             if (best_mark < 0 || m < best_mark) {
                 ties = 0;
                 best = d;
                 best_mark = m;
-                best_ratio = fabs(val[d] / val[mx]);
+                best_ratio = mag(val[d] / val[mx]);
             } else if (m == best_mark) {
                 ties += 1;
-                double ratio = fabs(val[d] / val[mx]);
+                double ratio = mag(val[d] / val[mx]);
                 if (ratio > best_ratio) {
                     best = d;
                     best_mark = m;
@@ -273,9 +344,9 @@ struct PivotState {
             if (best_mark < 0 || m < best_mark) {
                 best = e;
                 best_mark = m;
-                best_ratio = fabs(val[e] / val[mx]);
+                best_ratio = mag(val[e] / val[mx]);
             } else if (m == best_mark) {
-                double ratio = fabs(val[e] / val[mx]);
+                double ratio = mag(val[e] / val[mx]);
                 if (ratio > best_ratio) {
                     best = e;
                     best_mark = m;
@@ -291,9 +362,9 @@ struct PivotState {
         for (int k = n0; k < n; k++) {
             col_from(k, n0, tmp);
             for (int e : tmp)
-                if (fabs(val[e]) > bv) {
+                if (mag(val[e]) > bv) {
                     best = e;
-                    bv = fabs(val[e]);
+                    bv = mag(val[e]);
                 }
         }
         return best;
@@ -313,8 +384,8 @@ struct PivotState {
         std::swap(mc[a], mc[b]);
     }
     void eliminate(int pivot, int n0, std::vector<int>& L, std::vector<int>& U) {  // sparse21/mod.rs:865-919
-        const double pv = val[pivot];
-        if (pv == 0.0) fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
+        const T pv = val[pivot];
+        if (is_zero(pv)) fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
         col_from(n0, n0 + 1, L);
         row_from(n0, n0 + 1, U);
         for (int l : L) val[l] = val[l] / pv;
@@ -323,11 +394,11 @@ struct PivotState {
             for (int l : L) {
                 int t = lookup(orow[l], uc);
                 if (t < 0) {
-                    t = add(orow[l], uc, 0.0);
+                    t = add(orow[l], uc, T{});
                     mr[irow(t)] += 1;  // fill-in while factoring bumps both counts (sparse21/mod.rs:350-353)
                     mc[icol(t)] += 1;
                 }
-                const double v = val[u] * val[l];
+                const T v = val[u] * val[l];
                 val[t] = val[t] - v;
             }
             mc[icol(u)] -= 1;
@@ -367,11 +438,14 @@ static void topo_identity_pattern(Topology& t) {
     t.nnz = (int)t.ci.size();
 }
 
+template <class T>
+static void pivot_search(Topology& t, const std::vector<T>& first_values);
+
 void topo_pivot(Topology& t, const std::vector<double>& first_values) {
     t.sym_status = S21_OK;
     t.sym_error.clear();
     try {
-        topo_pivot_search(t, first_values);
+        pivot_search<double>(t, first_values);
     } catch (Error& e) {
         t.sym_status = e.code;
         t.sym_error = e.what();
@@ -379,9 +453,26 @@ void topo_pivot(Topology& t, const std::vector<double>& first_values) {
     }
 }
 
-void topo_pivot_search(Topology& t, const std::vector<double>& first_values) {
+void topo_pivot_cx(Topology& t, const std::vector<double>& re, const std::vector<double>& im) {
+    t.sym_status = S21_OK;
+    t.sym_error.clear();
+    std::vector<HCx> v(re.size());
+    for (size_t k = 0; k < v.size(); k++) v[k] = HCx{re[k], im[k]};
+    try {
+        pivot_search<HCx>(t, v);
+    } catch (Error& e) {
+        t.sym_status = e.code;
+        t.sym_error = e.what();
+        topo_identity_pattern(t);
+    }
+}
+
+void topo_pivot_search(Topology& t, const std::vector<double>& first_values) { pivot_search<double>(t, first_values); }
+
+template <class T>
+static void pivot_search(Topology& t, const std::vector<T>& first_values) {
     const int n = t.n;
-    PivotState s;
+    PivotState<T> s;
     s.n = n;
     s.rows.resize(n);
     s.cols.resize(n);
@@ -571,9 +662,10 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
             const bool rdrs = nd[RDP] != nd[RD] || nd[RSP] != nd[RS];
             const bool tied = !rdrs && nd[RS] == nd[RB];
             for (int e = 0; e < MOS1_NE; e++) {
-                if (mos1_entry_tr_only(e) && !(caps && t.tran)) continue;  // exactly zero
-                if (mos1_entry_r_only(e) && !rdrs) continue;             // grd = grs = 0
-                if (tied && mos1t_merged(e)) continue;                   // merged in registers
+                if (mos1_entry_tr_only(e) && !(caps && (t.tran || t.ac))) continue;  // exactly zero
+                if (mos1_entry_r_only(e) && !rdrs) continue;                        // grd = grs = 0
+                if (tied && mos1t_merged(e)) continue;                              // merged in registers
+                if (t.ac && mos1_entry_col(e) < 0) continue;                        // no AC RHS
                 const int r = nd[mos1_entry_row(e)];
                 const int col = mos1_entry_col(e) < 0 ? -2 : nd[mos1_entry_col(e)];
                 if (mos1_entry_col(e) >= 0 && col < 0) continue;
@@ -584,6 +676,7 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
             break;
         }
         case S21_COMP_MOS0:
+            if (t.ac) fail(S21_ERR, "AC Not Implemented For This Component!");  // comps/mod.rs:86-88
             for (int e = 0; e < MOS0_NE; e++) {
                 const int r = nd[mos0_entry_row(e)];
                 const int col = mos0_entry_col(e) < 0 ? -2 : nd[mos0_entry_col(e)];
@@ -594,6 +687,7 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
             flags[ci] = c.model == 1 ? DEV_PMOS : 0;
             break;
         case S21_COMP_DIODE:
+            if (t.ac) fail(S21_ERR, "AC Not Implemented For This Component!");
             for (int e = 0; e < DIODE_NE; e++) {
                 const int r = nd[diode_entry_row(e)];
                 const int col = diode_entry_col(e) < 0 ? -2 : nd[diode_entry_col(e)];
@@ -604,12 +698,13 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
             break;
         default: {
             const int p = nd[0], q = nd[1], iv = nd[2];
-            if (c.type == S21_COMP_R || (c.type == S21_COMP_C && t.tran)) {
+            if (c.type == S21_COMP_I && t.ac) fail(S21_ERR, "AC Not Implemented For This Component!");
+            if (c.type == S21_COMP_R || (c.type == S21_COMP_C && (t.tran || t.ac))) {
                 add(0, p, p < 0 ? -1 : p);
                 add(1, q, q < 0 ? -1 : q);
                 add(2, p, q < 0 ? -1 : q);
                 add(3, q, p < 0 ? -1 : p);
-                if (c.type == S21_COMP_C) {
+                if (c.type == S21_COMP_C && !t.ac) {
                     add(4, p, -2);
                     add(5, q, -2);
                 }

@@ -7,6 +7,8 @@ Random draws use splitmix64 -> 53-bit uniform -> Box-Muller, seeded per workload
   chain_mc()   N instances of a 64-stage CMOS inverter chain, per-device VTO ~ N(base, 15 mV) and
                W ~ N(base, 3 %), input tied to 0.5 V
   ring_batch() ring oscillators of uneven size: k = 3 + 2 U{0..49} stages, C ~ U[5, 50] fF, VDD ~ U[0.8, 1.2]
+  amp_ladder() N instances of a two-stage common-source amplifier driving a 20-section RC ladder with
+               R ~ U[1k, 10k] and C ~ U[10f, 100f] per section and instance (AC sweep workload)
 """
 import numpy as np
 
@@ -108,3 +110,29 @@ def ring_batch(n_inst=1024, seed=0xC3):
     cap = 5e-15 + u[:, 1] * 45e-15
     vdd = 0.8 + u[:, 2] * 0.4
     return stages, cap, vdd
+
+
+def amp_ladder(n_inst=256, sections=20, seed=0xC5, vin=0.6, vdd=1.0, rd1=5e3, rd2=2e3):
+    """Two-stage common-source amplifier (NMOS M1: g = in, d = n1, RD1 from vdd; M2: g = n1, d = n2, RD2
+    from vdd) with an AC-driven input source (dc bias vin, acm = 1) and an RC ladder on n2.  Returns
+    (netlist, overrides): every ladder resistor's conductance and capacitor's value per instance."""
+    n = Netlist()
+    nm = n.mos1_model(**NMOS)
+    ni = n.mos1_inst(**NINST)
+    n.v("vin", vin, "in", "", acm=1.0)
+    n.v("vdd", vdd, "vdd", "")
+    n.r(1.0 / rd1, "vdd", "n1", name="rd1")
+    n.mos1("m1", "n1", "in", "", "", nm, ni)
+    n.r(1.0 / rd2, "vdd", "n2", name="rd2")
+    n.mos1("m2", "n2", "n1", "", "", nm, ni)
+    u = uniform(seed, 2 * sections * n_inst).reshape(n_inst, sections, 2)
+    over = []
+    prev = "n2"
+    for k in range(sections):
+        node = f"l{k + 1}"
+        ri = n.r(1.0 / 5.5e3, prev, node, name=f"rl{k + 1}")
+        ci = n.c(55e-15, node, "", name=f"cl{k + 1}")
+        over.append((ri, "value", 0, 1.0 / (1e3 + 9e3 * u[:, k, 0])))
+        over.append((ci, "value", 0, 10e-15 + 90e-15 * u[:, k, 1]))
+        prev = node
+    return n, over

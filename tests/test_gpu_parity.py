@@ -160,3 +160,52 @@ def test_generated_factorisation_transient_matches_golden(ipw):
         X = b.wave(0, inst)
         worst = max(float(np.max(np.abs(X[:, k] - np.array(g[nm])))) for k, nm in enumerate(names))
         assert worst <= 1e-6, (inst, worst)
+
+
+def test_ac_batch_matches_oracle_per_instance():
+    """C5-style sweep: two-stage CS amplifier with a per-instance RC ladder, 1 kHz to 100 GHz."""
+    n_inst = 24
+    nl, over = W.amp_ladder(n_inst)
+    cn = nl.to_c()
+    log("ac fp64", n_inst)
+    b = Batch(Group(cn, n_inst, over)).ac(1000, 100_000_000_000, 1000)
+    st, it = b.status()
+    assert np.all(st == 0), st
+    f = b.axis()
+    for i in [0, 5, 23]:
+        fo, Xo, _ = O.ac(materialize(nl, over, i).to_c(), 1000, 100_000_000_000, 1000, max_pts=1100)
+        assert np.array_equal(f, fo)
+        X = b.wave(0, i)
+        err = np.max(np.abs(X - Xo) / np.maximum(np.abs(Xo), 1e-9))
+        log("  inst", i, "max rel", err)
+        assert err < 1e-9, (i, err)
+
+
+def test_ac_mixed_close_to_fp64():
+    n_inst = 16
+    nl, over = W.amp_ladder(n_inst)
+    g = Group(nl.to_c(), n_inst, over)
+    a = Batch(g).ac(1000, 100_000_000_000, 200)
+    m = Batch(g, precision=MIXED).ac(1000, 100_000_000_000, 200)
+    assert np.all(m.status()[0] == 0)
+    worst = max(float(np.max(np.abs(m.wave(0, i) - a.wave(0, i)))) for i in range(n_inst))
+    log("ac mixed vs fp64", worst)
+    assert worst <= 1e-6
+
+
+def test_ac_byte_abi_matches_oracle():
+    from spice21_amd import api, protos
+
+    ckt = protos.Circuit(name="rc")
+    ckt.comps.append(protos.Instance(v=protos.Vsrc(name="vi", p="inp", n="", dc=0.0, acm=1.0)))
+    ckt.comps.append(protos.Instance(r=protos.Resistor(name="r1", p="inp", n="out", g=1e-3)))
+    ckt.comps.append(protos.Instance(c=protos.Capacitor(name="c1", p="out", n="", c=1e-9)))
+    res = api.ac(ckt, args=protos.AcOptions(fstart=1, fstop=10_000_000, npts=70))
+    nl = W.Netlist()
+    nl.v("vi", 0.0, "inp", "", acm=1.0)
+    nl.r(1e-3, "inp", "out")
+    nl.c(1e-9, "out", "")
+    f, X, _ = O.ac(nl.to_c(), 1, 10_000_000, 70)
+    out = np.array(res["out"])
+    assert out.shape == (len(f),)
+    assert np.max(np.abs(out - X[:, 2]) / np.abs(X[:, 2])) < 1e-9
