@@ -532,13 +532,12 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
 
     HIPCHECK(hipStreamSynchronize(b->stream));
     // Generated factorisation: for a batch of one pattern with enough instances to repay the compilation.
-    // Its launch shape is fixed here: one instance per wavefront asks the compiler for four resident
-    // wavefronts per SIMD, more instances per wavefront for two.
+    // Its launch shape (instances per wavefront) is fixed here.
     g.has_jit = false;
     const bool want = !ac && (b->jit > 0 || (b->jit == 0 && b->groups.size() == 1 && NI >= JIT_MIN_INST));
     if (want && t.sym_status == S21_OK && jit_ops(t) <= JIT_MAX_OPS) {
         g.jit_ipw = b->ipw > 0 ? b->ipw : auto_ipw(NI, device_simds());
-        g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED, g.jit_ipw == 1 ? 4 : 2));
+        g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED, 2));
         g.has_jit = true;
     }
     g.blob_bytes = blob_bytes;
@@ -589,20 +588,27 @@ static void launch_any(s21_batch* b, const GroupGPU* dg, int ng, int nb, size_t 
 
 // LDS bytes between the regions of consecutive instances: whole 256-byte rows plus an offset that
 // starts the segments of one half-wavefront on different LDS banks.
-static size_t inst_stride(size_t raw, int ipw) {
+static size_t inst_stride(size_t raw, int ipw, bool lane_per_instance = false) {
     const size_t L = WAVE / ipw;
-    return ((raw + 255) / 256) * 256 + (L * 8) % 256;
+    // The generated factorisation reads the same offset of every instance's region from one lane each:
+    // regions 8 bytes apart modulo 256 put those lanes on different banks.
+    return ((raw + 255) / 256) * 256 + (lane_per_instance ? 8 : (L * 8) % 256);
 }
+constexpr size_t JIT_STATIC_LDS = sizeof(int) * WAVE * MAX_WAVES;  // newton_wg's per-slot flags
 
 // Launch shape: instances per wavefront (the batch's setting, or auto_ipw) and wavefronts per workgroup
 // (the most resident instances per CU within the LDS).
 static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_inst, size_t max_block, int simds,
                         Args& A, size_t& stride) {
     int ipw = b->ipw;
-    if (b->groups.size() == 1 && b->groups[0].has_jit) ipw = b->groups[0].jit_ipw;
+    const bool jit = b->groups.size() == 1 && b->groups[0].has_jit;
+    if (jit) {
+        ipw = b->groups[0].jit_ipw;
+        blob += JIT_STATIC_LDS;
+    }
     if (ipw <= 0) ipw = auto_ipw(total_inst, simds);
     for (;; ipw /= 2) {
-        stride = inst_stride(raw, ipw);
+        stride = inst_stride(raw, ipw, jit);
         int best = 0;
         size_t best_res = 0;
         for (int k = 1; k <= MAX_WAVES; k++) {

@@ -8,7 +8,8 @@
 // l_ik = a_ik / u_kk, the row-k updates, c_i -= c_k l_ik skipped when c_k is zero; then rows from the last,
 // c_i = (c_i - sum u_ij c_j) / u_ii; sparse21/mod.rs:865-919, 946-979), compiles it together with the
 // kernel body of kernel.h through hiprtc, and keeps the module for the life of the process.  One lane per
-// instance runs the generated code; the values it needs stay in registers.
+// instance runs the generated code, every instance of a workgroup at once (newton_wg); the values it
+// needs stay in registers.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -71,18 +72,11 @@ std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
     }
     o << "}\n";
     o << "struct LuGen {\n"
+         "    static constexpr bool lane_per_instance = true;\n"
          "    template <typename V>\n"
-         "    static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {\n"
-         "        bool z = false;\n"
-         "        if (act && C.sl == 0) z = lu_gen<V>(lu, C.c);\n"
-         "        wave_sync();\n"
-         "        return z;\n"
-         "    }\n"
+         "    static __device__ bool factor_lane(LDS V* lu, LDS double* c) { return lu_gen<V>(lu, c); }\n"
          "    template <typename V>\n"
-         "    static __device__ void backward(const Ctx& C, bool act, LDS const V* lu) {\n"
-         "        if (act && C.sl == 0) bw_gen<V>(lu, C.c);\n"
-         "        wave_sync();\n"
-         "    }\n"
+         "    static __device__ void backward_lane(LDS const V* lu, LDS double* c) { bw_gen<V>(lu, c); }\n"
          "};\n}  // namespace s21\n";
     o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES, " << waves_per_simd
       << ") void s21_newton_jit("

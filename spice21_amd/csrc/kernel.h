@@ -545,8 +545,11 @@ __device__ inline void commit(const Ctx& C, bool act) {
 
 // One Newton solve (Solver::solve, analysis.rs:169-210) of every active segment of the wavefront.
 // `iters` accumulates loads.  Returns the segment's S21_* status (S21_OK for a lane that was inactive).
-// The LU policy supplies factor<V>(C, act, lu) -> zero pivot seen, and backward<V>(C, act, lu).
+// The LU policy supplies factor<V>(C, act, lu) -> zero pivot seen, and backward<V>(C, act, lu), run by
+// each instance's own lanes; or, with lane_per_instance, factor_lane / backward_lane run by one lane
+// per instance for every instance of the workgroup at once (newton_wg).
 struct LuLevels {
+    static constexpr bool lane_per_instance = false;
     template <typename V>
     static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {
         return factor_forward<V>(C, act, lu);
@@ -980,6 +983,86 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
     return st == RUNNING ? S21_CONVERGENCE : st;
 }
 
+// One Newton solve of every instance of the workgroup in lockstep, for an LU policy that factors one
+// lane per instance: the instances' own lanes assemble, test convergence and update; between two
+// barriers the first n_slots threads of the workgroup each factor and sweep one instance's matrix,
+// so one instruction stream serves all of them.  s_act: per-slot flag words in LDS.
+template <bool MIXED, class LU>
+__device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
+                         LDS int* s_act, int slot, int n_slots, LDS char* region0, int stride) {
+    const TopoGPU& T = C.T;
+    const bool prof = A.prof != 0;
+    long long t0 = 0, t1 = 0;
+    int st = act ? RUNNING : S21_OK;
+    bool prev_ok = true;  // dx starts at zero
+    for (int it = 0; it < A.max_iter; it++) {
+        if (!__syncthreads_or(act)) break;
+        if (act) iters++;
+        assemble(C, act, tran, A, prof ? pc : nullptr);
+        wave_sync();
+        if (prof) t0 = cycles();
+        const bool bad = seg_any(C, residual(C, act, A.iabstol));
+        wave_sync();
+        const bool conv = act && prev_ok && !bad;
+        if (wave_any(conv)) {
+            commit(C, conv);
+            wave_sync();
+        }
+        if (conv) {
+            st = S21_OK;
+            act = false;
+        }
+        if (act && T.sym_status != S21_OK) {
+            st = T.sym_status;
+            act = false;
+        }
+        if (MIXED)
+            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) C.lu32[p] = (float)C.a[p];
+        if (C.sl == 0) s_act[slot] = act ? 1 : 0;
+        if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
+        __syncthreads();
+        for (int sl = threadIdx.x; sl < n_slots; sl += blockDim.x) {
+            if (!s_act[sl]) continue;
+            LDS double* c = (LDS double*)(region0 + (size_t)sl * stride) + T.n;
+            LDS double* a = c + T.n;
+            bool z;
+            if (MIXED) {
+                LDS float* lu = (LDS float*)(a + T.nnz + T.n + T.n_hub + T.n_part);
+                z = LU::template factor_lane<float>(lu, c);
+                if (!z) LU::template backward_lane<float>(lu, c);
+            } else {
+                z = LU::template factor_lane<double>(a, c);
+                if (!z) LU::template backward_lane<double>(a, c);
+            }
+            s_act[sl] = z ? 2 : 1;
+        }
+        __syncthreads();
+        if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+        if (act && s_act[slot] == 2) {  // a zero pivot other than the last (sparse21/mod.rs:872)
+            st = S21_PIVOT;
+            act = false;
+        }
+        // clamp and update (analysis.rs:196-207)
+        double mx = 0.0;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            const double dx = fabs(C.c[C.I.ce2i[k]]);
+            mx = (dx > mx) ? dx : mx;
+        }
+        mx = seg_max(C, mx);
+        bool ok = true;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            double dx = C.c[C.I.ce2i[k]];
+            if (mx > 1.0) dx = dx * 1.0 / mx;
+            ok = ok && !(fabs(dx) > A.reltol);
+            C.x[k] = C.x[k] + dx;
+        }
+        prev_ok = !seg_any(C, !ok);
+        wave_sync();
+        if (prof) { t1 = cycles(); pc[5] += t1 - t0; }
+    }
+    return st == RUNNING ? S21_CONVERGENCE : st;
+}
+
 // Workgroup = A.waves wavefronts x A.ipw instances, sharing one LDS copy of the group's records.
 template <bool MIXED, class LU>
 __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_groups, const Args& A) {
@@ -1014,7 +1097,8 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
         inst = (b - G.blk0) * A.waves * A.ipw + slot;
         valid = inst < G.n_inst;
     }
-    if (!wave_any(valid)) return;
+    // A workgroup-synchronous solve needs every wavefront at every barrier, instance or not.
+    if (!LU::lane_per_instance && !wave_any(valid)) return;
     if (!valid) inst = 0;  // idle lanes touch no instance data
     C.inst = inst;
     LDS const u16* b16 = (LDS const u16*)smem_u4;
@@ -1090,15 +1174,25 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
     const long long tstart = A.prof ? cycles() : 0;
     long long iters = 0;
-    int st = newton<MIXED, LU>(C, valid, false, A, iters, pc);
+    LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
+    auto solve = [&](bool act, bool tran) {
+        if constexpr (LU::lane_per_instance) {
+            __shared__ int s_act[WAVE * MAX_WAVES];
+            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
+                                        A.inst_bytes);
+        } else {
+            return newton<MIXED, LU>(C, act, tran, A, iters, pc);
+        }
+    };
+    int st = solve(valid, false);
     if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
     if (A.mode == MODE_TRAN) {
         bool alive = valid && st == S21_OK;
         double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
         for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
         for (long long pt = 1; pt < A.npts; pt++) {  // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
-            if (!wave_any(alive)) break;
-            const int s2 = newton<MIXED, LU>(C, alive, true, A, iters, pc);
+            if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
+            const int s2 = solve(alive, true);
             if (alive) {
                 st = s2;
                 alive = s2 == S21_OK;

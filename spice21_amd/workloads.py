@@ -112,6 +112,22 @@ def ring_batch(n_inst=1024, seed=0xC3):
     return stages, cap, vdd
 
 
+def ring_groups(n_inst=1024, seed=0xC3):
+    """The uneven ring-oscillator batch as batch groups, one per stage count: a list of (netlist,
+    n_inst_k, overrides, instance ids) with every stage capacitor's value and the supply's value per
+    instance (node "1" is unknown 0 in every group, so one initial condition applies to all)."""
+    stages, cap, vdd = ring_batch(n_inst, seed)
+    groups = []
+    for k in sorted(set(int(v) for v in stages)):
+        ids = np.nonzero(stages == k)[0]
+        n = ring5(stages=k)
+        caps = [ci for ci, c in enumerate(n.comps) if c[0] == 1]  # S21_COMP_C
+        vsrc = [ci for ci, c in enumerate(n.comps) if c[0] == 3]  # S21_COMP_V
+        over = [(ci, "value", 0, cap[ids]) for ci in caps] + [(vsrc[0], "value", 0, vdd[ids])]
+        groups.append((n, len(ids), over, ids))
+    return groups
+
+
 def amp_ladder(n_inst=256, sections=20, seed=0xC5, vin=0.6, vdd=1.0, rd1=5e3, rd2=2e3):
     """Two-stage common-source amplifier (NMOS M1: g = in, d = n1, RD1 from vdd; M2: g = n1, d = n2, RD2
     from vdd) with an AC-driven input source (dc bias vin, acm = 1) and an RC ladder on n2.  Returns

@@ -209,3 +209,21 @@ def test_ac_byte_abi_matches_oracle():
     out = np.array(res["out"])
     assert out.shape == (len(f),)
     assert np.max(np.abs(out - X[:, 2]) / np.abs(X[:, 2])) < 1e-9
+
+
+def test_uneven_ring_batch_transient_matches_oracle():
+    """C3-style: several groups of different size in one launch, per-instance C and VDD, 300 steps."""
+    gs = W.ring_groups(48, seed=0xC3)[:4]
+    groups = [Group(n.to_c(), k, over) for n, k, over, _ in gs]
+    log("ring batch groups", [(g.cnet.n_vars, g.n_inst) for g in groups])
+    b = Batch(groups).tran(1e-12, 3e-10, [("1", 0.0)])
+    for gi, (n, k, over, _) in enumerate(gs):
+        st, it = b.status(gi)
+        assert np.all(st == 0), (gi, st)
+        for i in sorted({0, k - 1}):
+            t, X, names, _ = O.tran(materialize(n, over, i).to_c(), 1e-12, 3e-10, [("1", 0.0)])
+            Xg = b.wave(gi, i)
+            assert Xg.shape == X.shape
+            err = float(np.max(np.abs(Xg - X)))
+            log("  group", gi, "inst", i, "max |dv|", err)
+            assert err <= 1e-9, (gi, i, err)
