@@ -82,6 +82,54 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
 
 
+def extra_measurements():
+    """The metric's transient rate and the other workloads, by HIP-event kernel time (each analysis call
+    also rebuilds its host structures, which is not the kernel's throughput)."""
+    import numpy as np
+
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
+
+    out = {}
+    # batched transient of 1024 uneven ring oscillators
+    gs = W.ring_groups(1024)
+    groups = [Group(n.to_c(), k, over) for n, k, over, _ in gs]
+    rec, waves = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        b = Batch(groups, precision=prec).tran(1e-12, 5e-9, [("1", 0.0)])
+        ms = b.timing()[0]
+        npts = len(b.axis())
+        iters = sum(int(b.status(g)[1].sum()) for g in range(len(groups)))
+        failed = sum(int(np.sum(b.status(g)[0] != 0)) for g in range(len(groups)))
+        rec[name] = {"tran_pts_per_s": 1024 * npts / (ms * 1e-3), "instance_iters_per_s": iters / (ms * 1e-3),
+                     "kernel_ms": ms, "failed_instances": failed}
+        waves[name] = [b.wave(g, 0) for g in range(0, len(groups), 7)]
+    rec["mixed_vs_fp64_max_node_v"] = float(max(np.abs(a - c).max() for a, c in zip(waves["fp64"], waves["mixed"])))
+    out["tran_ring_batch"] = {"workload": "1024 ring oscillators of 3..101 stages (C, VDD per instance), "
+                                          "0-5 ns at 1 ps, 5000 points", **rec}
+    # batched AC sweep
+    nl, over = W.amp_ladder(256)
+    g = Group(nl.to_c(), 256, over)
+    rec, sw = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        b = Batch(g, precision=prec).ac(1000, 100_000_000_000, 1000)
+        ms = b.timing()[0]
+        nf = len(b.axis())
+        rec[name] = {"instance_freqs_per_s": 256 * nf / (ms * 1e-3), "kernel_ms": ms,
+                     "failed_instances": int(np.sum(b.status()[0] != 0))}
+        sw[name] = [b.wave(0, i) for i in range(0, 256, 37)]
+    rec["mixed_vs_fp64_max_abs"] = float(max(np.abs(a - c).max() for a, c in zip(sw["fp64"], sw["mixed"])))
+    out["ac_amp_ladder"] = {"workload": "256 two-stage CS amplifiers with a 20-section RC ladder, 1 kHz-100 GHz, "
+                                        "1002 frequencies", **rec}
+    # one 5-stage ring oscillator (latency)
+    b = Batch(Group(W.ring5().to_c())).tran(1e-12, 1e-8, [("1", 0.5)])
+    ms = b.timing()[0]
+    it = int(b.status()[1][0])
+    out["tran_single_ring5"] = {"workload": "5-stage CMOS ring, 0-10 ns at 1 ps", "points": len(b.axis()),
+                                "kernel_ms": ms, "us_per_newton_iter": ms * 1e3 / it, "newton_iters": it}
+    return out
+
+
 def time_batch(batch, steps, warmup):
     for _ in range(warmup):
         batch.op()
@@ -106,6 +154,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the instances-per-wavefront autotuning before the timed steps")
     ap.add_argument("--ipw", type=int, default=0, help="instances per wavefront (implies --no-autotune)")
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the transient / AC / single-ring measurements")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="'mixed' / 'fp64' time one path only (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -190,6 +240,9 @@ def main():
         d = np.abs(res["mixed"]["x"] - res["fp64"]["x"])
         line["mixed_vs_fp64"] = {"max_node_v": float(d[:, vmask].max()), "max_branch_i": float(d[:, imask].max()),
                                  "speedup": value / line["fp64"]["value"]}
+
+    if args.extra and args.precision == "both":
+        line["other_workloads"] = extra_measurements()
 
     if not args.no_cpu:
         from oracle import oracle_py as O  # the CPU restatement, timed as the baseline and used as the checker
