@@ -75,6 +75,11 @@ struct GroupHost {
     bool has_jit = false;
     int jit_ipw = 1;                        // instances per wavefront the generated kernel is built for
     std::string prepared;  // analysis the uploaded structure was built for ("" = none)
+    // Pivoting of the operating-point structure: by default the search on instance 0's matrix at x = 0;
+    // pivot_row / pivot_col give the order instead; pivot_xs / pivot_vds (iterates, with diode guesses in
+    // component order) add the reference's later searches on instance 0's matrices at those iterates.
+    std::vector<int> pivot_row, pivot_col;
+    std::vector<std::vector<double>> pivot_xs, pivot_vds;
 };
 
 }  // namespace s21
@@ -297,10 +302,24 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
         std::vector<double> re, im;
         host_assemble_ac(t, op0, grd0, grs0, lin0, ac->omega0, re, im);
         topo_pivot_cx(t, re, im);
+    } else if (!tr && !g.pivot_row.empty()) {
+        topo_pivot_forced(t, g.pivot_row, g.pivot_col);
     } else {
         std::vector<double> x0(t.n, 0.0), vals, rhs;
         host_assemble_op(t, h0, o, x0, vals, rhs);
-        topo_pivot(t, vals);
+        if (!tr && !g.pivot_xs.empty()) {
+            std::vector<std::vector<double>> later;
+            for (size_t k = 0; k < g.pivot_xs.size(); k++) {
+                HostInstance hk = h0;
+                hk.dio_vd = g.pivot_vds[k];
+                std::vector<double> v, r;
+                host_assemble_op(t, hk, o, g.pivot_xs[k], v, r);
+                later.push_back(std::move(v));
+            }
+            topo_pivot_sequence(t, vals, later);
+        } else {
+            topo_pivot(t, vals);
+        }
     }
     topo_schedule(t, has_caps);
 
@@ -719,6 +738,7 @@ static Args base_args(const s21_batch* b, int mode) {
     A.ipw = 1;
     A.inst_bytes = 0;
     A.capture = 0;
+    A.resume = 0;
     A.omega = nullptr;
     return A;
 }
@@ -749,6 +769,164 @@ static void group_from(GroupHost& g, const s21_group& sg) {
         x.field = o.field;
         x.vals.assign(o.vals, o.vals + sg.n_inst);
         g.overs.push_back(std::move(x));
+    }
+}
+
+// ---- zero-pivot recovery of the operating point ----
+// The reference searches its pivots again at every factorisation (Matrix::solve after each reset,
+// sparse21/mod.rs:647-674, 929-932); the engine keeps the first search's order, which is the order those
+// searches return for as long as it stays acceptable.  An instance whose kept order meets an exact zero
+// pivot (status S21_PIVOT) resumes that iteration in a launch of its own: a mixed-precision instance first
+// in FP64 on the same order, then on the order the reference's search picks at that iterate, searched from
+// the arrangement and fill its earlier factorisations left.  The resumed solve keeps its iteration count
+// and the limit of 100 (analysis.rs:173); each resumption uses at least one iteration.
+constexpr int MAX_RESUME = 8;
+
+struct Stalled {
+    int g = 0, i = 0;             // group, instance
+    std::vector<double> x;        // iterate of the interrupted iteration
+    std::vector<double> vd;       // per diode (component order): the guess that iteration's evaluation began from
+    long long iters = 0;          // loads before the interrupted iteration
+    bool fp64 = false;            // the interrupted solve ran in FP64
+    std::vector<int> row, col;    // the order that met the zero pivot
+    std::vector<std::vector<double>> xs, vds;  // iterates (and diode guesses) of the searches so far
+};
+
+static std::vector<double> diode_field(const GroupHost& g, int f) {
+    const Topology& t = g.topo;
+    const size_t nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE], s = (size_t)g.n_inst * nd;
+    std::vector<double> v(s);
+    if (s) HIPCHECK(hipMemcpy(v.data(), g.gg.d_state + f * s, s * sizeof(double), hipMemcpyDeviceToHost));
+    return v;
+}
+// Diode guesses of instance i in component order from a state field [n_inst][n_d] in device order.
+static std::vector<double> diodes_by_component(const GroupHost& g, int i, const std::vector<double>& field) {
+    const Topology& t = g.topo;
+    const int nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE];
+    std::vector<int> rank(t.comps.size(), -1);
+    int r = 0;
+    for (size_t c = 0; c < t.comps.size(); c++)
+        if (t.comps[c].type == S21_COMP_DIODE) rank[c] = r++;
+    std::vector<double> vd(r, 0.0);
+    for (int d = 0; d < nd; d++) vd[rank[t.dev_comp[t.cls_start[CLS_DIODE] + d]]] = field[(size_t)i * nd + d];
+    return vd;
+}
+
+static void recover_zero_pivots(s21_batch* b) {
+    std::vector<Stalled> todo;
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        GroupHost& g = b->groups[gi];
+        std::vector<int> st(g.n_inst);
+        HIPCHECK(hipMemcpy(st.data(), g.gg.status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<double> f3;
+        for (int i = 0; i < g.n_inst; i++) {
+            if (st[i] != S21_PIVOT) continue;
+            if (f3.empty()) f3 = diode_field(g, 3);
+            Stalled s;
+            s.g = (int)gi;
+            s.i = i;
+            s.x.resize(g.topo.n);
+            HIPCHECK(hipMemcpy(s.x.data(), g.gg.x + (size_t)i * g.topo.n, g.topo.n * sizeof(double),
+                               hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(&s.iters, g.gg.iters + i, sizeof(long long), hipMemcpyDeviceToHost));
+            s.iters -= 1;
+            s.vd = diodes_by_component(g, i, f3);
+            s.fp64 = b->precision != S21_MIXED;
+            s.row = g.topo.row_i2e;
+            s.col = g.topo.col_i2e;
+            todo.push_back(std::move(s));
+        }
+    }
+    if (todo.empty()) return;
+    fetch_results(b);  // host copies of every result, patched below
+    for (int round = 0; round < MAX_RESUME && !todo.empty(); round++) {
+        s21_batch sb;
+        sb.opts = b->opts;
+        sb.precision = S21_FP64;
+        sb.ipw = 1;
+        sb.jit = -1;
+        HIPCHECK(hipStreamCreateWithFlags(&sb.stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&sb.ev0));
+        HIPCHECK(hipEventCreate(&sb.ev1));
+        sb.groups.resize(todo.size());
+        for (size_t k = 0; k < todo.size(); k++) {
+            const Stalled& s = todo[k];
+            const GroupHost& g = b->groups[s.g];
+            GroupHost& h = sb.groups[k];
+            h.n_vars = g.n_vars;
+            h.names = g.names;
+            h.kinds = g.kinds;
+            h.comps = g.comps;
+            h.mos1_models = g.mos1_models;
+            h.mos1_insts = g.mos1_insts;
+            h.diode_models = g.diode_models;
+            h.diode_insts = g.diode_insts;
+            h.n_inst = 1;
+            for (const Override& o : g.overs) {
+                Override x = o;
+                x.vals.assign(1, o.vals[s.i]);
+                h.overs.push_back(std::move(x));
+            }
+            if (!s.fp64) {  // FP64 on the order that failed in FP32
+                h.pivot_row = s.row;
+                h.pivot_col = s.col;
+            } else {  // the searches so far, then the search at the interrupted iterate
+                h.pivot_xs = s.xs;
+                h.pivot_vds = s.vds;
+                h.pivot_xs.push_back(s.x);
+                h.pivot_vds.push_back(s.vd);
+            }
+        }
+        prepare_all(&sb, nullptr);
+        for (size_t k = 0; k < todo.size(); k++) {
+            const Stalled& s = todo[k];
+            GroupHost& h = sb.groups[k];
+            HIPCHECK(hipMemcpy(h.gg.x, s.x.data(), s.x.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(h.gg.iters, &s.iters, sizeof(long long), hipMemcpyHostToDevice));
+            const Topology& t = h.topo;
+            const int nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE];
+            std::vector<int> rank(t.comps.size(), -1);
+            int r = 0;
+            for (size_t c = 0; c < t.comps.size(); c++)
+                if (t.comps[c].type == S21_COMP_DIODE) rank[c] = r++;
+            std::vector<double> vd0(nd);
+            for (int d = 0; d < nd; d++) vd0[d] = s.vd[rank[t.dev_comp[t.cls_start[CLS_DIODE] + d]]];
+            if (nd) HIPCHECK(hipMemcpy(h.gg.d_state, vd0.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+        }
+        Args A = base_args(&sb, MODE_OP);
+        A.resume = 1;
+        run(&sb, A, nullptr, 1);
+        sb.fetched = false;
+        fetch_results(&sb);
+        std::vector<Stalled> next;
+        for (size_t k = 0; k < todo.size(); k++) {
+            Stalled& s = todo[k];
+            GroupHost& h = sb.groups[k];
+            GroupHost& g = b->groups[s.g];
+            const int n = g.topo.n;
+            if (h.status[0] == S21_PIVOT && round + 1 < MAX_RESUME) {
+                Stalled t2;
+                t2.g = s.g;
+                t2.i = s.i;
+                t2.x = h.x;
+                t2.vd = diodes_by_component(h, 0, diode_field(h, 3));
+                t2.iters = h.iters[0] - 1;
+                t2.fp64 = true;
+                t2.row = h.topo.row_i2e;
+                t2.col = h.topo.col_i2e;
+                t2.xs = h.pivot_xs;
+                t2.vds = h.pivot_vds;
+                next.push_back(std::move(t2));
+                continue;
+            }
+            std::copy(h.x.begin(), h.x.begin() + n, g.x.begin() + (size_t)s.i * n);
+            g.status[s.i] = h.status[0];
+            g.iters[s.i] = h.iters[0];
+            HIPCHECK(hipMemcpy(g.gg.x + (size_t)s.i * n, h.x.data(), n * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(g.gg.status + s.i, &h.status[0], sizeof(int), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(g.gg.iters + s.i, &h.iters[0], sizeof(long long), hipMemcpyHostToDevice));
+        }
+        todo.swap(next);
     }
 }
 
@@ -851,6 +1029,7 @@ int s21_batch_op(s21_batch* b) {
         Args A = base_args(b, MODE_OP);
         run(b, A, nullptr, 1);
         b->fetched = false;
+        recover_zero_pivots(b);
         b->npts = 0;
         b->axis.clear();
         b->last_mode = MODE_OP;

@@ -105,6 +105,12 @@ struct Topology {
 void topo_elements(Topology& t);
 void topo_pivot(Topology& t, const std::vector<double>& first_values);
 void topo_pivot_search(Topology& t, const std::vector<double>& first_values);
+// A given pivot order (internal index -> external row / column) and its fill pattern.
+// The reference's searches of several factorisations in turn: the first on first_values, each later one
+// on the next value set, from the arrangement and fill the previous search left; the last order is kept.
+void topo_pivot_sequence(Topology& t, const std::vector<double>& first_values,
+                         const std::vector<std::vector<double>>& later_values);
+void topo_pivot_forced(Topology& t, const std::vector<int>& row_i2e, const std::vector<int>& col_i2e);
 // The same on the complex matrix of the first AC iteration (Matrix<Complex<f64>>: magnitudes are norms).
 void topo_pivot_cx(Topology& t, const std::vector<double>& re, const std::vector<double>& im);
 void topo_schedule(Topology& t, const std::vector<uint8_t>& mos1_has_caps);
@@ -115,6 +121,7 @@ struct HostInstance {
     std::vector<Mos1Derived> m1;
     std::vector<DiodeP> dio;
     std::vector<double> lin_value;  // per linear component, operating-point value
+    std::vector<double> dio_vd;     // per diode, the guess its limiting starts from (empty: 0, as at x = 0)
 };
 void host_assemble_op(const Topology& t, const HostInstance& h, const s21_options& o, const std::vector<double>& x,
                       std::vector<double>& vals, std::vector<double>& rhs);

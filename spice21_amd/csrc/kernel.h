@@ -34,7 +34,7 @@ namespace s21 {
 // ============================================================================================
 // Device-side descriptors
 // ============================================================================================
-constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 3;
+constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 4;
 
 struct TopoGPU {
     int n, nnz, n_hub, n_part;
@@ -66,7 +66,8 @@ struct GroupGPU {
     int lin_istr[3];
     double* m1_state;  // [14][n_inst][n_m1]: committed op (7) then guess (7)
     double* acop;      // [9][n_inst][n_m1] small-signal operating point (capture), or nullptr
-    double* d_state;   // [3][n_inst][n_d]: guess vd, guess charge, committed charge
+    double* d_state;   // [4][n_inst][n_d]: guess vd, guess charge, committed charge, guess vd the last
+                       // evaluation started from (to resume an interrupted solve at that evaluation)
     double* c_q;            // [n_inst][n_lin] committed capacitor charge
     double* x;              // [n_inst][n]
     int* status;            // [n_inst]
@@ -89,6 +90,9 @@ struct Args {
     int ipw;         // instances per wavefront (1, 2, 4, 8 or 16)
     int inst_bytes;  // LDS bytes per instance
     int capture;     // operating point: store each Mos1's small-signal operating point for AC
+    int resume;      // the first solve continues an interrupted one: G.x holds its iterate, G.iters the
+                     // iterations it has used, and its first convergence test fails (the interrupted
+                     // iteration had passed that test already)
     const double* omega;  // AC: angular frequency of each point
 };
 
@@ -321,15 +325,17 @@ __device__ inline void eval_diode(const Ctx& C, int j, LDS const u16* dv, LDS co
     P.m = F(13);
     GLB double* st = glb(G.d_state);
     const double vr = xv(C.x, (int)dv[DR_] - 1), vn = xv(C.x, (int)dv[DN_] - 1);
+    const double vd0 = st[0 * s + o];
     DiodeGuess gs;
     Stamps<DIODE_NE> sv;
     if (tran)
-        diode_load<true>(P, vr, vn, A.gmin, A.tstep, st[0 * s + o], st[2 * s + o], sv, &gs);
+        diode_load<true>(P, vr, vn, A.gmin, A.tstep, vd0, st[2 * s + o], sv, &gs);
     else
-        diode_load<false>(P, vr, vn, A.gmin, A.tstep, st[0 * s + o], st[2 * s + o], sv, &gs);
+        diode_load<false>(P, vr, vn, A.gmin, A.tstep, vd0, st[2 * s + o], sv, &gs);
     apply<DIODE_NE, false>(C, code, sv.v, false);
     st[0 * s + o] = gs.vd;
     st[1 * s + o] = gs.charge;
+    st[3 * s + o] = vd0;
 }
 
 __device__ inline void eval_lin(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
@@ -345,7 +351,14 @@ __device__ inline void eval_lin(const Ctx& C, int j, LDS const u16* dv, LDS cons
         put(2, -val);
         put(3, -val);
     } else if (typ == S21_COMP_C) {
-        if (!tran) return;  // no stamps in the operating point (comps/mod.rs:205-209)
+        if (!tran) {  // no stamps in the operating point (comps/mod.rs:205-209)
+            // Hub cells are summed whether or not their device stamped, so this device's cells must hold
+            // zero rather than whatever the LDS held before the launch.
+#pragma unroll
+            for (int e = 0; e < 6; e++)
+                if (code[e] != 0xFFFF && (code[e] & 0x8000)) C.hub[code[e] & 0x7FFF] = 0.0;
+            return;
+        }
         cap_load_tran(val, xv(C.x, (int)dv[0] - 1) - xv(C.x, (int)dv[1] - 1), glb(G.c_q)[o], A.tstep, put);
     } else if (typ == S21_COMP_V) {
         put(0, 1.0);
@@ -545,6 +558,8 @@ __device__ inline void commit(const Ctx& C, bool act) {
 
 // One Newton solve (Solver::solve, analysis.rs:169-210) of every active segment of the wavefront.
 // `iters` accumulates loads.  Returns the segment's S21_* status (S21_OK for a lane that was inactive).
+// A fresh solve has it_left = max_iter and prev_ok = true (dx starts at zero); a resumed one has the
+// iterations it has left and prev_ok = false.
 // The LU policy supplies factor<V>(C, act, lu) -> zero pivot seen, and backward<V>(C, act, lu), run by
 // each instance's own lanes; or, with lane_per_instance, factor_lane / backward_lane run by one lane
 // per instance for every instance of the workgroup at once (newton_wg).
@@ -561,13 +576,14 @@ struct LuLevels {
 };
 
 template <bool MIXED, class LU>
-__device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc) {
+__device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
+                      int it_left, bool prev_ok) {
     const TopoGPU& T = C.T;
     const bool prof = A.prof != 0;
     long long t0 = 0, t1 = 0;
     int st = act ? RUNNING : S21_OK;
-    bool prev_ok = true;  // dx starts at zero
     for (int it = 0; it < A.max_iter; it++) {
+        if (it >= it_left) act = false;  // the segment's share of the iteration limit is used up
         if (!wave_any(act)) break;
         if (act) iters++;
         assemble(C, act, tran, A, prof ? pc : nullptr);
@@ -989,13 +1005,14 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
 // so one instruction stream serves all of them.  s_act: per-slot flag words in LDS.
 template <bool MIXED, class LU>
 __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
-                         LDS int* s_act, int slot, int n_slots, LDS char* region0, int stride) {
+                         int it_left, bool prev_ok, LDS int* s_act, int slot, int n_slots, LDS char* region0,
+                         int stride) {
     const TopoGPU& T = C.T;
     const bool prof = A.prof != 0;
     long long t0 = 0, t1 = 0;
     int st = act ? RUNNING : S21_OK;
-    bool prev_ok = true;  // dx starts at zero
     for (int it = 0; it < A.max_iter; it++) {
+        if (it >= it_left) act = false;
         if (!__syncthreads_or(act)) break;
         if (act) iters++;
         assemble(C, act, tran, A, prof ? pc : nullptr);
@@ -1174,17 +1191,22 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
     const long long tstart = A.prof ? cycles() : 0;
     long long iters = 0;
+    int it_left = A.max_iter;
+    if (A.resume && valid) {
+        iters = G.iters[inst];
+        it_left = A.max_iter - (int)iters;
+    }
     LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
-    auto solve = [&](bool act, bool tran) {
+    auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
         if constexpr (LU::lane_per_instance) {
             __shared__ int s_act[WAVE * MAX_WAVES];
-            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
-                                        A.inst_bytes);
+            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
+                                        A.waves * A.ipw, region0, A.inst_bytes);
         } else {
-            return newton<MIXED, LU>(C, act, tran, A, iters, pc);
+            return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
         }
     };
-    int st = solve(valid, false);
+    int st = solve(valid, false, it_left, !A.resume);
     if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
     if (A.mode == MODE_TRAN) {
         bool alive = valid && st == S21_OK;
@@ -1192,7 +1214,7 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
         for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
         for (long long pt = 1; pt < A.npts; pt++) {  // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
             if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
-            const int s2 = solve(alive, true);
+            const int s2 = solve(alive, true, A.max_iter, true);
             if (alive) {
                 st = s2;
                 alive = s2 == S21_OK;

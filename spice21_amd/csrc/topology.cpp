@@ -133,7 +133,8 @@ void host_assemble_op(const Topology& t, const HostInstance& h, const s21_option
             break;
         }
         case S21_COMP_DIODE: {
-            diode_load<false>(h.dio[id++], X(nd[DR_]), X(nd[DN_]), o.gmin, 0.0, 0.0, 0.0,
+            const double vd_guess = h.dio_vd.empty() ? 0.0 : h.dio_vd[id];
+            diode_load<false>(h.dio[id++], X(nd[DR_]), X(nd[DN_]), o.gmin, 0.0, vd_guess, 0.0,
                               [&](int e, double v) {
                                   const int r = nd[diode_entry_row(e)];
                                   if (diode_entry_col(e) < 0)
@@ -383,12 +384,14 @@ struct PivotState {
         c_e2i[c_i2e[b]] = b;
         std::swap(mc[a], mc[b]);
     }
-    void eliminate(int pivot, int n0, std::vector<int>& L, std::vector<int>& U) {  // sparse21/mod.rs:865-919
+    // `numeric` false: the fill pattern and Markowitz counts only (an order given in advance).
+    void eliminate(int pivot, int n0, std::vector<int>& L, std::vector<int>& U, bool numeric = true) {  // sparse21/mod.rs:865-919
         const T pv = val[pivot];
-        if (is_zero(pv)) fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
+        if (numeric && is_zero(pv)) fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
         col_from(n0, n0 + 1, L);
         row_from(n0, n0 + 1, U);
-        for (int l : L) val[l] = val[l] / pv;
+        if (numeric)
+            for (int l : L) val[l] = val[l] / pv;
         for (int u : U) {
             const int uc = ocol[u];
             for (int l : L) {
@@ -398,8 +401,10 @@ struct PivotState {
                     mr[irow(t)] += 1;  // fill-in while factoring bumps both counts (sparse21/mod.rs:350-353)
                     mc[icol(t)] += 1;
                 }
-                const T v = val[u] * val[l];
-                val[t] = val[t] - v;
+                if (numeric) {
+                    const T v = val[u] * val[l];
+                    val[t] = val[t] - v;
+                }
             }
             mc[icol(u)] -= 1;
         }
@@ -439,7 +444,9 @@ static void topo_identity_pattern(Topology& t) {
 }
 
 template <class T>
-static void pivot_search(Topology& t, const std::vector<T>& first_values);
+static void pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r = nullptr,
+                         const std::vector<int>* order_c = nullptr,
+                         const std::vector<std::vector<T>>* later_values = nullptr);
 
 void topo_pivot(Topology& t, const std::vector<double>& first_values) {
     t.sym_status = S21_OK;
@@ -469,8 +476,36 @@ void topo_pivot_cx(Topology& t, const std::vector<double>& re, const std::vector
 
 void topo_pivot_search(Topology& t, const std::vector<double>& first_values) { pivot_search<double>(t, first_values); }
 
+void topo_pivot_sequence(Topology& t, const std::vector<double>& first_values,
+                         const std::vector<std::vector<double>>& later_values) {
+    t.sym_status = S21_OK;
+    t.sym_error.clear();
+    try {
+        pivot_search<double>(t, first_values, nullptr, nullptr, &later_values);
+    } catch (Error& e) {
+        t.sym_status = e.code;
+        t.sym_error = e.what();
+        topo_identity_pattern(t);
+    }
+}
+
+void topo_pivot_forced(Topology& t, const std::vector<int>& row_i2e, const std::vector<int>& col_i2e) {
+    if ((int)row_i2e.size() != t.n || (int)col_i2e.size() != t.n) fail(S21_INVALID, "pivot order of the wrong size");
+    t.sym_status = S21_OK;
+    t.sym_error.clear();
+    std::vector<double> ones(t.erow.size(), 1.0);
+    pivot_search<double>(t, ones, &row_i2e, &col_i2e);
+}
+
+// With order_r / order_c (internal index -> external row / column), the pivots are those entries instead
+// of the search's choice and no values are computed: the fill pattern of a given order.  With
+// later_values, each set is searched in turn the way the reference's next factorisation searches: from
+// the arrangement and the elements (fill included) the previous one left, element values reset to zero
+// and reloaded (Matrix::reset / update / lu_factorize, sparse21/mod.rs:272-282, 647-674), Markowitz
+// counts restarting from each row's and column's element count (setup_factoring, :191-196).
 template <class T>
-static void pivot_search(Topology& t, const std::vector<T>& first_values) {
+static void pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r,
+                         const std::vector<int>* order_c, const std::vector<std::vector<T>>* later_values) {
     const int n = t.n;
     PivotState<T> s;
     s.n = n;
@@ -493,14 +528,32 @@ static void pivot_search(Topology& t, const std::vector<T>& first_values) {
     for (int k = 0; k < n; k++)
         if (s.rows[k].empty() || s.cols[k].empty()) fail(S21_SINGULAR, "Singular Matrix");
     std::vector<int> tmp, L, U;
-    for (int n0 = 0; n0 < n - 1; n0++) {
-        int pv = s.search_diagonal(n0, tmp);
-        if (pv < 0) pv = s.search_submatrix(n0, tmp);
-        if (pv < 0) pv = s.find_max(n0, tmp);
-        if (pv < 0) fail(S21_PIVOT, "Pivot Search Fail");
-        s.swap_rows(s.irow(pv), n0);
-        s.swap_cols(s.icol(pv), n0);
-        s.eliminate(pv, n0, L, U);
+    const bool forced = order_r != nullptr;
+    const size_t passes = 1 + (later_values && !forced ? later_values->size() : 0);
+    for (size_t pass = 0; pass < passes; pass++) {
+        if (pass > 0) {
+            const std::vector<T>& v = (*later_values)[pass - 1];
+            for (size_t e = 0; e < s.val.size(); e++) s.val[e] = e < v.size() ? v[e] : T{};
+            for (int k = 0; k < n; k++) {
+                s.mr[k] = (long)s.rows[s.r_i2e[k]].size();
+                s.mc[k] = (long)s.cols[s.c_i2e[k]].size();
+            }
+        }
+        for (int n0 = 0; n0 < n - 1; n0++) {
+            int pv;
+            if (forced) {
+                pv = s.lookup((*order_r)[n0], (*order_c)[n0]);
+                if (pv < 0) fail(S21_INVALID, "given pivot order names an absent entry");
+            } else {
+                pv = s.search_diagonal(n0, tmp);
+                if (pv < 0) pv = s.search_submatrix(n0, tmp);
+                if (pv < 0) pv = s.find_max(n0, tmp);
+                if (pv < 0) fail(S21_PIVOT, "Pivot Search Fail");
+            }
+            s.swap_rows(s.irow(pv), n0);
+            s.swap_cols(s.icol(pv), n0);
+            s.eliminate(pv, n0, L, U, !forced);
+        }
     }
     t.row_i2e = s.r_i2e;
     t.row_e2i = s.r_e2i;

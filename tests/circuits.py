@@ -55,3 +55,29 @@ GOLDEN_RUNS = {
 def materialize(nl, overrides, i):
     """Standalone netlist of instance i of a batch (see Netlist.instance)."""
     return nl.instance(overrides, i)
+
+
+def py_ring3(acm=0.0):
+    """The three-stage Mos1 ring of spice21py's test_mos1_ro (spice21py/spice21py/tests/test_spice21.py:
+    132-174) as a protobuf circuit: vss is a node of its own (driven by vvss), every stage's capacitor ends
+    on it, and the feedback source vvfb closes the loop from ring0 to ring3 (AC magnitude ``acm``)."""
+    from spice21_amd import protos as P
+    from spice21_amd.api import circuit
+
+    comps = []
+    for k in range(3):
+        inp, out = f"ring{k}", f"ring{k + 1}"
+        p = P.Mos(name=f"stg{k}p", model="pmos", params="default")
+        p.ports.g, p.ports.d, p.ports.s, p.ports.b = inp, out, "vdd", "vdd"
+        n = P.Mos(name=f"stg{k}n", model="nmos", params="default")
+        n.ports.g, n.ports.d, n.ports.s, n.ports.b = inp, out, "vss", "vss"
+        comps += [p, n, P.Capacitor(name=f"stg{k}c", p=out, n="vss", c=1e-15)]
+    return circuit(
+        P.Mos1Model(name="nmos", mos_type=P.MosType.NMOS),
+        P.Mos1Model(name="pmos", mos_type=P.MosType.PMOS),
+        P.Mos1InstParams(name="default"),
+        P.Vsrc(name="vvdd", p="vdd", n="vss", dc=1.0),
+        P.Vsrc(name="vvss", p="vss", dc=0),
+        P.Vsrc(name="vvfb", p="ring0", n="ring3", dc=0, acm=acm),
+        *comps,
+    )

@@ -12,7 +12,8 @@ import pytest
 from oracle import oracle_py as O
 from spice21_amd import workloads as W
 from spice21_amd.batch import FP64, MIXED, Batch, Group
-from tests.circuits import GOLDEN_RUNS, materialize, ro3
+from spice21_amd.elab import elaborate
+from tests.circuits import GOLDEN_RUNS, materialize, py_ring3, ro3
 from tests.test_symbolic import random_circuit
 
 pytestmark = pytest.mark.gpu
@@ -60,13 +61,14 @@ def test_dcop_matches_oracle(which):
     else:
         nl = ro3(which)
     cn = nl.to_c()
-    try:
-        ox, oit, _ = O.dcop(cn)
-    except O.OracleError:
-        pytest.skip("reference does not converge on this draw")
     log("dcop", which)
     b = Batch(Group(cn)).op()
     st, it = b.status()
+    try:
+        ox, oit, _ = O.dcop(cn)
+    except O.OracleError as e:  # the reference fails on this draw: so must the engine, the same way
+        assert e.desc == "Convergence Failed" and st[0] == 2, (which, e.desc, st[0])
+        return
     assert st[0] == 0
     x = b.x()[0]
     assert rel_err(x, ox) < 1e-9, (which, x, ox)
@@ -227,3 +229,102 @@ def test_uneven_ring_batch_transient_matches_oracle():
             err = float(np.max(np.abs(Xg - X)))
             log("  group", gi, "inst", i, "max |dv|", err)
             assert err <= 1e-9, (gi, i, err)
+
+
+S21_CONVERGENCE = 2
+
+
+def _random_batch_cases(n=60):
+    """Random netlists (NMOS/PMOS Mos1 with and without rd/rs, Mos0, diodes with and without rs, R, C, I,
+    V; tests/test_symbolic.py) with the oracle's dcop outcome for each."""
+    cases = []
+    for s in range(n):
+        cn = random_circuit(s).to_c()
+        try:
+            ox, oit, _ = O.dcop(cn)
+            cases.append((s, cn, ox, oit))
+        except O.OracleError as e:
+            assert e.desc == "Convergence Failed", (s, e.desc)
+            cases.append((s, cn, None, None))
+    return cases
+
+
+# Seeds whose kept pivot order meets an exact zero pivot in FP64 (seed 56: at the second iteration, with
+# a diode's 1.5e15 S conductance in the matrix; the reference searches again and moves on).  The engine
+# resumes such an instance on the order the reference's search picks there; the reference changes its
+# order twice more later in this solve, so the last digits follow another rounding path through
+# Jacobians of condition number up to 4e19, and the solution is held to 1e-9 of its norm instead.
+RESUMED_FP64 = {56}
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_random_circuits_dcop_outcome_matches_oracle(precision):
+    """60 random netlists as 60 groups of one launch.  Where the oracle converges the engine converges to
+    the same point (FP64: 1e-9 relative, same iteration count; mixed: 1e-6 V of it); where the oracle
+    reports "Convergence Failed" (analysis.rs:209) the instance's status is S21_CONVERGENCE."""
+    cases = _random_batch_cases()
+    log("random dcop", precision, "groups", len(cases), "oracle failures", sum(c[2] is None for c in cases))
+    b = Batch([Group(cn) for _, cn, _, _ in cases], precision=precision).op()
+    it_diff = []
+    for gi, (s, cn, ox, oit) in enumerate(cases):
+        st, it = b.status(gi)
+        if ox is None:
+            assert st[0] == S21_CONVERGENCE, (s, st[0])
+            continue
+        assert st[0] == 0, (s, st[0])
+        x = b.x(gi)[0]
+        if precision == FP64 and s in RESUMED_FP64:
+            log("  seed", s, "resumed: iterations", int(it[0]), "oracle", oit,
+                "max |dx| / max |x|", float(np.max(np.abs(x - ox)) / np.max(np.abs(ox))))
+            assert np.max(np.abs(x - ox)) <= 1e-9 * np.max(np.abs(ox)), (s, x, ox)
+        elif precision == FP64:
+            # 1e-9 relative, with a 1 fV / 1 fA floor for unknowns the residual test (1e-12) leaves at noise
+            assert np.all(np.abs(x - ox) <= 1e-9 * np.abs(ox) + 1e-15), (s, x, ox)
+            if it[0] != oit:
+                it_diff.append((s, int(it[0]), oit))
+        else:
+            v = [k for k in range(cn.n_vars) if cn.kinds[k] == 0]
+            assert np.max(np.abs(x[v] - ox[v])) <= 1e-6, s
+    assert not it_diff, it_diff
+
+
+def test_random_circuits_transient_matches_oracle():
+    """Twelve random netlists that the oracle can bias, 100 steps of 1 ps each, one group per netlist."""
+    cases = [c for c in _random_batch_cases() if c[2] is not None][:12]
+    b = Batch([Group(cn) for _, cn, _, _ in cases]).tran(1e-12, 1e-10)
+    for gi, (s, cn, _, _) in enumerate(cases):
+        t, X, names, _ = O.tran(cn, 1e-12, 1e-10)
+        st, _ = b.status(gi)
+        assert st[0] == 0, (s, st[0])
+        Xg = b.wave(gi, 0)
+        assert Xg.shape == X.shape
+        err = float(np.max(np.abs(Xg - X) / np.maximum(np.abs(X), 1.0)))
+        log("  seed", s, "n", cn.n_vars, "max dev", err)
+        assert err <= 1e-9, (s, err)
+
+
+def test_capacitor_hub_cells_read_zero_in_operating_points():
+    """A capacitor stamps nothing in an operating point (comps/mod.rs:205-209).  In the three-stage ring
+    every stage's capacitor ends on vss, a hub slot, so those capacitors own hub cells; in the t = 0
+    operating point of a transient and in the DC point of an AC sweep the cells must read zero, not what
+    an earlier launch left in the LDS.  A large batch runs first, on every CU, to leave such content."""
+    nl, over = W.chain_mc(4096)
+    Batch(Group(nl.to_c(), 4096, over)).op()
+    e = elaborate(py_ring3())
+    b = Batch(Group(e)).tran(1e-11, 5e-10, [("ring0", 0.0)])
+    assert b.structure()["n_contrib"] > 0  # vss is a hub slot of the transient's structure
+    assert b.status()[0][0] == 0
+    t, X, names, _ = O.tran(e, 1e-11, 5e-10, [("ring0", 0.0)])
+    err = float(np.max(np.abs(b.wave(0, 0) - X)))
+    log("ring tran after a large batch: max |dx|", err)
+    assert err <= 1e-9
+    Batch(Group(nl.to_c(), 4096, over)).op()
+    ea = elaborate(py_ring3(acm=1.0))
+    ba = Batch(Group(ea)).ac(1, 1_000_000_000, 20)
+    assert ba.status()[0][0] == 0
+    f, Xa, _ = O.ac(ea, 1, 1_000_000_000, 20)
+    assert np.array_equal(ba.axis(), f)
+    Xg = ba.wave(0, 0)
+    err = float(np.max(np.abs(Xg - Xa) / np.maximum(np.abs(Xa), 1e-12)))
+    log("ring ac after a large batch: max rel", err)
+    assert err <= 1e-9
