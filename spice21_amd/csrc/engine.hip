@@ -562,7 +562,7 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     g.blob_bytes = blob_bytes;
     const size_t vw = ac ? 2 : 1;  // values per entry: complex for AC
     g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) * vw +
-                   (b->precision == S21_MIXED ? (size_t)t.nnz * sizeof(float) * vw : 0);
+                   (b->precision == S21_MIXED ? (size_t)(t.nnz + t.n) * sizeof(float) * vw : 0);
     g.gg.acop = nullptr;
 }
 
@@ -635,7 +635,7 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
             const size_t bytes = blob + (size_t)k * ipw * stride;
             if (bytes > max_block) break;
             const size_t res = (LDS_PER_CU / bytes) * k * ipw;
-            if (res > best_res) {
+            if (res >= best_res) {  // at equal residency, larger workgroups share the generated LU's lanes better
                 best_res = res;
                 best = k;
             }

@@ -3,13 +3,12 @@
 // The level-scheduled factorisation walks 16-bit records in LDS, so every dependent step pays a record
 // load before its data load.  For a batch whose instances share one pattern, this file writes the
 // factorisation with the fused forward sweep and the backward sweep out as straight-line code -- every
-// LU position a constant offset -- in exactly the order of the records and with the same division
-// (ldiv) (IKJ by rows in pivot order:
+// LU position a constant offset -- in the order of the records and with the same division (ldiv)
+// (IKJ by rows in pivot order:
 // l_ik = a_ik / u_kk, the row-k updates, c_i -= c_k l_ik skipped when c_k is zero; then rows from the last,
 // c_i = (c_i - sum u_ij c_j) / u_ii; sparse21/mod.rs:865-919, 946-979), compiles it together with the
 // kernel body of kernel.h through hiprtc, and keeps the module for the life of the process.  One lane per
-// instance runs the generated code, every instance of a workgroup at once (newton_wg); the values it
-// needs stay in registers.
+// instance runs the generated code, every instance of a workgroup at once (newton_wg).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -41,11 +40,40 @@ int64_t jit_ops(const Topology& t) {
     return ops;
 }
 
-std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
+// Two shapes of the generated code, chosen by precision (both measured on the 64-stage chain batch):
+//
+// * FP64: straight-line statements on the LDS arrays, which the compiler schedules freely.
+// * FP32 (mixed precision): values in named locals, in steps -- a row of the factorisation (long rows in
+//   chunks of kChunk L entries) or of the backward sweep.  A step takes what the previous kCarry steps
+//   computed from their locals and loads everything else from the LDS; those loads are issued one step
+//   ahead, before the previous step's arithmetic, so their latency overlaps it.  A step stores what later
+//   code reads (a finished U row, the swept RHS).  A scheduling barrier between steps keeps the compiler
+//   from hoisting the loads of many steps at once, which would need more registers than a lane has.  It
+//   works on FP32 copies of A and of the residual; updates are fused multiply-adds and a zero c_k is not
+//   skipped: this factorisation only preconditions the FP64 Newton update.
+//   (With FP64 values, two registers each, this shape spills and runs slower than the straight one.)
+//
+// Both keep the records' operation order and division (ldiv); FP64 performs exactly their arithmetic.
+constexpr int kCarry = 1;
+constexpr int kChunk = 4;
+
+namespace {
+struct GenStep {
+    std::string loads, body;
+};
+// Emits steps with their loads one step ahead, a scheduling barrier after each.
+std::string emit_steps(const std::vector<GenStep>& st) {
     std::ostringstream o;
-    o << "#include \"kernel.h\"\nnamespace s21 {\n";
-    // factorisation + forward sweep
-    o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS double* __restrict__ c) {\n"
+    if (!st.empty()) o << st[0].loads;
+    for (size_t k = 0; k < st.size(); k++) {
+        if (k + 1 < st.size()) o << st[k + 1].loads;
+        o << st[k].body << "    __builtin_amdgcn_sched_barrier(0);\n";
+    }
+    return o.str();
+}
+
+void gen_straight(std::ostringstream& o, const Topology& t) {
+    o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
          "    V y, l, ck;\n";
     for (int i = 0; i < t.n; i++) {
         if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;  // no L entries
@@ -62,8 +90,7 @@ std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
     o << "    bool z = false;\n";
     for (int i = 0; i + 1 < t.n; i++) o << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
     o << "    return z;\n}\n";
-    // backward sweep
-    o << "template <typename V>\n__device__ inline void bw_gen(LDS const V* __restrict__ lu, LDS double* __restrict__ c) {\n"
+    o << "template <typename V>\n__device__ inline void bw_gen(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n"
          "    V y;\n";
     for (int i = t.n - 1; i >= 0; i--) {
         o << "    y = (V)c[" << i << "];";
@@ -71,12 +98,138 @@ std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
         o << " c[" << i << "] = (double)ldiv<V>(y, lu[" << t.diag[i] << "]);\n";
     }
     o << "}\n";
+}
+
+void gen_steps(std::ostringstream& o, const Topology& t) {
+    const int NEVER = -1000000;
+    // ---- factorisation + forward sweep ----
+    {
+        std::vector<GenStep> steps;
+        std::ostringstream pre;  // pivots of rows without L entries, checked up front
+        std::vector<int> def_step(t.nnz, NEVER), cdef_step(t.n, NEVER);  // step whose local holds the value
+        std::vector<char> used(t.nnz, 0), cused(t.n, 0);
+        int step = -1;
+        for (int i = 0; i < t.n; i++) {
+            const bool has_l = t.diag[i] >= 0 && t.diag[i] > t.rp[i];
+            if (!has_l) {
+                if (i + 1 < t.n && t.diag[i] >= 0) pre << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
+                continue;
+            }
+            const int row_first = step + 1;  // row i's own values, once loaded, stay in locals for the row
+            for (int q0 = t.rp[i]; q0 < t.diag[i]; q0 += kChunk) {
+                step++;
+                GenStep g;
+                std::ostringstream loads, body;
+                auto fresh = [&](int defd, bool own) { return own ? defd >= row_first : step - defd <= kCarry; };
+                auto use = [&](int pos) {
+                    const bool own = pos >= t.rp[i] && pos < t.rp[i + 1];
+                    if (!fresh(def_step[pos], own)) {
+                        loads << "    v" << pos << " = lu[" << pos << "];\n";
+                        def_step[pos] = step;
+                    }
+                    used[pos] = 1;
+                    return "v" + std::to_string(pos);
+                };
+                auto usec = [&](int k) {
+                    if (!fresh(cdef_step[k], k == i)) {
+                        loads << "    y" << k << " = c[" << k << "];\n";
+                        cdef_step[k] = step;
+                    }
+                    cused[k] = 1;
+                    return "y" + std::to_string(k);
+                };
+                const std::string yi = usec(i);
+                const int q1 = std::min(q0 + kChunk, t.diag[i]);
+                for (int q = q0; q < q1; q++) {
+                    const int k = t.ci[q];
+                    const std::string a = use(q), piv = use(t.diag[k]), yk = usec(k);
+                    body << "    l = ldiv<V>(" << a << ", " << piv << "); " << yi << " = __builtin_fmaf(-" << yk
+                         << ", l, " << yi << ");\n";
+                    for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++) {
+                        const std::string src = use(t.upd_src[x]), d = use(t.upd_dst[x]);
+                        body << "    " << d << " = __builtin_fmaf(-" << src << ", l, " << d << ");\n";
+                        def_step[t.upd_dst[x]] = step;
+                    }
+                }
+                cdef_step[i] = step;
+                if (q1 == t.diag[i]) {  // row finished: its U row and swept RHS
+                    for (int q = t.diag[i]; q < t.rp[i + 1]; q++)
+                        if (def_step[q] >= row_first) {
+                            body << "    lu[" << q << "] = v" << q << ";\n";
+                            def_step[q] = step;
+                        }
+                    body << "    c[" << i << "] = " << yi << ";\n";
+                    if (i + 1 < t.n) body << "    z = z || " << use(t.diag[i]) << " == V(0);\n";
+                }
+                g.loads = loads.str();
+                g.body = body.str();
+                steps.push_back(std::move(g));
+            }
+        }
+        o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
+             "    V l;\n    bool z = false;\n";
+        for (int q = 0; q < t.nnz; q++)
+            if (used[q]) o << "    V v" << q << ";\n";
+        for (int k = 0; k < t.n; k++)
+            if (cused[k]) o << "    V y" << k << ";\n";
+        o << pre.str() << emit_steps(steps) << "    return z;\n}\n";
+    }
+    // ---- backward sweep ----
+    {
+        std::vector<GenStep> steps;
+        std::vector<int> xdef(t.n, NEVER);
+        int step = -1;
+        for (int i = t.n - 1; i >= 0; i--) {
+            const int u0 = t.diag[i] + 1, u1 = t.rp[i + 1];
+            int q0 = u0;
+            do {
+                step++;
+                GenStep g;
+                std::ostringstream loads, body;
+                const int q1 = std::min(q0 + 2 * kChunk, u1);
+                if (q0 == u0) loads << "    y" << i << " = c[" << i << "];\n";
+                for (int q = q0; q < q1; q++) {
+                    const int j = t.ci[q];
+                    if (step - xdef[j] > kCarry) {
+                        loads << "    x" << j << " = c[" << j << "];\n";
+                        xdef[j] = step;
+                    }
+                    loads << "    u" << q << " = lu[" << q << "];\n";
+                    body << "    y" << i << " = __builtin_fmaf(-x" << j << ", u" << q << ", y" << i << ");\n";
+                }
+                if (q1 == u1) {
+                    loads << "    u" << t.diag[i] << " = lu[" << t.diag[i] << "];\n";
+                    body << "    x" << i << " = ldiv<V>(y" << i << ", u" << t.diag[i] << ");\n    c[" << i
+                         << "] = x" << i << ";\n";
+                    xdef[i] = step;
+                }
+                g.loads = loads.str();
+                g.body = body.str();
+                steps.push_back(std::move(g));
+                q0 = q1;
+            } while (q0 < u1);
+        }
+        o << "template <typename V>\n__device__ inline void bw_gen(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n";
+        for (int q = 0; q < t.nnz; q++) o << "    V u" << q << ";\n";
+        for (int k = 0; k < t.n; k++) o << "    V x" << k << ", y" << k << ";\n";
+        o << emit_steps(steps) << "}\n";
+    }
+}
+}  // namespace
+
+std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
+    std::ostringstream o;
+    o << "#include \"kernel.h\"\nnamespace s21 {\n";
+    if (mixed)
+        gen_steps(o, t);
+    else
+        gen_straight(o, t);
     o << "struct LuGen {\n"
          "    static constexpr bool lane_per_instance = true;\n"
          "    template <typename V>\n"
-         "    static __device__ bool factor_lane(LDS V* lu, LDS double* c) { return lu_gen<V>(lu, c); }\n"
+         "    static __device__ bool factor_lane(LDS V* lu, LDS V* c) { return lu_gen<V>(lu, c); }\n"
          "    template <typename V>\n"
-         "    static __device__ void backward_lane(LDS const V* lu, LDS double* c) { bw_gen<V>(lu, c); }\n"
+         "    static __device__ void backward_lane(LDS const V* lu, LDS V* c) { bw_gen<V>(lu, c); }\n"
          "};\n}  // namespace s21\n";
     o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES, " << waves_per_simd
       << ") void s21_newton_jit("

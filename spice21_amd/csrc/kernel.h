@@ -4,7 +4,7 @@
 // One persistent launch runs every instance's whole analysis: the operating point and, for a transient,
 // every time step.  A workgroup holds up to MAX_WAVES wavefronts; each wavefront holds `ipw` instances
 // of one group, each owning L = 64 / ipw consecutive lanes and one LDS region:
-//     x[n] | c[n] | a[nnz] | rhs[n] | hub[n_hub] | part[n_part] (| lu32[nnz] in mixed precision)
+//     x[n] | c[n] | a[nnz] | rhs[n] | hub[n_hub] | part[n_part] (| lu32[nnz] | c32[n] in mixed precision)
 // next to the workgroup's shared copy of the group's records.  Each Newton iteration (Solver::solve,
 // analysis.rs:169-210) runs these phases:
 //   1. device evaluation by colour classes: the devices of one colour share no matrix / RHS slot other
@@ -1033,8 +1033,10 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
             st = T.sym_status;
             act = false;
         }
-        if (MIXED)
+        if (MIXED) {  // FP32 copies of A and of the residual for the factorisation lanes
             for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) C.lu32[p] = (float)C.a[p];
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) C.lu32[T.nnz + k] = (float)C.c[k];
+        }
         if (C.sl == 0) s_act[slot] = act ? 1 : 0;
         if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
         __syncthreads();
@@ -1045,8 +1047,8 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
             bool z;
             if (MIXED) {
                 LDS float* lu = (LDS float*)(a + T.nnz + T.n + T.n_hub + T.n_part);
-                z = LU::template factor_lane<float>(lu, c);
-                if (!z) LU::template backward_lane<float>(lu, c);
+                z = LU::template factor_lane<float>(lu, lu + T.nnz);
+                if (!z) LU::template backward_lane<float>(lu, lu + T.nnz);
             } else {
                 z = LU::template factor_lane<double>(a, c);
                 if (!z) LU::template backward_lane<double>(a, c);
@@ -1058,6 +1060,10 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
         if (act && s_act[slot] == 2) {  // a zero pivot other than the last (sparse21/mod.rs:872)
             st = S21_PIVOT;
             act = false;
+        }
+        if (MIXED) {
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) C.c[k] = (double)C.lu32[T.nnz + k];
+            wave_sync();
         }
         // clamp and update (analysis.rs:196-207)
         double mx = 0.0;

@@ -120,16 +120,18 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
             gm = P.beta * vds * (1.0 + P.lambda * vds);
             gds = P.beta * ((vov - vds) * (1.0 + P.lambda * vds) + P.lambda * ((vov * vds) - (vds * vds) / 2.0));
         }
-        gmbs = (P.phi_t + vsb > 0.0) ? gm * P.gamma / 2.0 / sqrt(P.phi_t + vsb) : 0.0;
+        // With gamma = 0 the quotient is gm * 0 / 2 whatever the root: skip the root and the division.
+        if (P.phi_t + vsb > 0.0) gmbs = P.gamma == 0.0 ? gm * P.gamma / 2.0 : gm * P.gamma / 2.0 / sqrt(P.phi_t + vsb);
     }
     const double vt = P.vtherm;
     const double isat_bs = reversed ? P.isat_d : P.isat_s;
     const double isat_bd = reversed ? P.isat_s : P.isat_d;
-    const double ebs = exp(-vsb / vt);
+    // A junction at zero bias (source or drain on the bulk) has exp(-0 / vt) = 1 exactly.
+    const double ebs = vsb == 0.0 ? 1.0 : exp(-vsb / vt);
     const double ibs = isat_bs * (ebs - 1.0);
     const double gbs = (isat_bs / vt) * ebs + gmin;
     const double ibs_rhs = ibs + vsb * gbs;
-    const double ebd = exp(-vdb / vt);
+    const double ebd = vdb == 0.0 ? 1.0 : exp(-vdb / vt);
     const double ibd = isat_bd * (ebd - 1.0);
     const double gbd = (isat_bd / vt) * ebd + gmin;
     const double ibd_rhs = ibd + vdb * gbd;

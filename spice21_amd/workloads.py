@@ -152,3 +152,43 @@ def amp_ladder(n_inst=256, sections=20, seed=0xC5, vin=0.6, vdd=1.0, rd1=5e3, rd
         over.append((ci, "value", 0, 10e-15 + 90e-15 * u[:, k, 1]))
         prev = node
     return n, over
+
+
+def power_grid(size=200, n_inv=5000, n_diode=2000, pad_every=20, seed=0xC4):
+    """One large instance: a size x size resistor mesh (edge R ~ U[0.5, 2] ohm, 1 pF from every mesh node
+    to ground) with 1.0 V pads on every `pad_every`-th node of its boundary walk, `n_inv` CMOS inverters
+    (input and PMOS source/bulk at random mesh nodes, NMOS to ground, each driving a new output node) and
+    `n_diode` reverse-biased diodes (IS 1e-14, N 1; anode at ground, cathode at a random mesh node)."""
+    n = Netlist()
+    g = lambda r, c: f"g{r}_{c}"
+    nedge = 2 * size * (size - 1)
+    u = uniform(seed, nedge + 2 * n_inv + n_diode)
+    r_edge = 0.5 + 1.5 * u[:nedge]
+    pick = (u[nedge:] * (size * size)).astype(np.int64)
+    k = 0
+    for r in range(size):
+        for c in range(size - 1):
+            n.r(1.0 / r_edge[k], g(r, c), g(r, c + 1), name=f"rh{r}_{c}")
+            k += 1
+    for r in range(size - 1):
+        for c in range(size):
+            n.r(1.0 / r_edge[k], g(r, c), g(r + 1, c), name=f"rv{r}_{c}")
+            k += 1
+    for r in range(size):
+        for c in range(size):
+            n.c(1e-12, g(r, c), "", name=f"c{r}_{c}")
+    walk = ([(0, c) for c in range(size)] + [(r, size - 1) for r in range(1, size)] +
+            [(size - 1, c) for c in range(size - 2, -1, -1)] + [(r, 0) for r in range(size - 2, 0, -1)])
+    for j, (r, c) in enumerate(walk[::pad_every]):
+        n.v(f"vpad{j}", 1.0, g(r, c), "")
+    nm, pm, ni, pi = _models(n)
+    node = lambda p: g(int(p) // size, int(p) % size)
+    for j in range(n_inv):
+        a, s = node(pick[2 * j]), node(pick[2 * j + 1])
+        n.mos1(f"inv{j}.p", f"o{j}", a, s, s, pm, pi)
+        n.mos1(f"inv{j}.n", f"o{j}", a, "", "", nm, ni)
+    dm = n.diode_model(is_=1e-14, n=1.0)
+    di = n.diode_inst()
+    for j in range(n_diode):
+        n.diode(f"d{j}", "", node(pick[2 * n_inv + j]), dm, di)
+    return n
