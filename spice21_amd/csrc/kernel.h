@@ -223,8 +223,39 @@ __device__ inline void apply(const Ctx& C, LDS const u16* code, const double (&v
     }
 }
 
+// Only the listed entries (stamp indices fixed at compile time): the others' codes are 0xFFFF.
+template <int N, bool SIGMA>
+__device__ inline void apply_some(const Ctx& C, LDS const u16* code, const double (&v)[MOS1_NE], const int (&e)[N],
+                                  bool rev) {
+    constexpr int B = 8;
+#pragma unroll
+    for (int k0 = 0; k0 < N; k0 += B) {
+        int c[B];
+        double cur[B];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) c[k] = code[(SIGMA && rev) ? mos1_sigma(e[k0 + k]) : e[k0 + k]];
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k0 + k < N) cur[k] = c[k] < 0x8000 ? C.a[c[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            if (k0 + k >= N) continue;
+            if (c[k] < 0x8000)
+                C.a[c[k]] = cur[k] + v[e[k0 + k]];
+            else if (c[k] != 0xFFFF)
+                C.hub[c[k] & 0x7FFF] = v[e[k0 + k]];
+        }
+    }
+}
+// Entries a Mos1 without live charge terms can stamp (the host gives the charge-only ones no slot unless
+// the device has capacitance and the analysis is a transient); of those, the ones a tied Mos1 keeps
+// after its merges (it has no terminal resistances either).  Both lists are closed under the swap.
+constexpr int M1_DC_ENTRIES[20] = {0, 1, 2, 3, 4, 5, 7, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 25};
+constexpr int M1T_DC_ENTRIES[8] = {0, 1, 2, 3, 4, 5, 22, 23};
+
 // Tied Mos1: stamps to entries (the source/drain swap), coinciding entries merged, then one batch.
-__device__ inline void apply_tied(const Ctx& C, LDS const u16* code, const M1Stamps& sv) {
+__device__ inline void apply_tied(const Ctx& C, LDS const u16* code, const M1Stamps& sv, bool charges) {
     double v[MOS1_NE];
 #pragma unroll
     for (int e = 0; e < MOS1_NE; e++) v[e] = sv.rev ? sv.v[mos1_sigma(e)] : sv.v[e];
@@ -234,7 +265,10 @@ __device__ inline void apply_tied(const Ctx& C, LDS const u16* code, const M1Sta
     v[5] = v[5] + v[11];                     // (s,g)
     v[8] = v[8] + v[10];                     // (g,s)
     v[23] = v[23] + v[25];                   // rhs s
-    apply<MOS1_NE, false>(C, code, v, false);
+    if (charges)
+        apply<MOS1_NE, false>(C, code, v, false);
+    else
+        apply_some<8, false>(C, code, v, M1T_DC_ENTRIES, false);
 }
 
 __device__ inline int mos1_count(const TopoGPU& T) { return T.cls_start[CLS_MOS1T + 1] - T.cls_start[CLS_MOS1]; }
@@ -265,9 +299,9 @@ __device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS con
     if (!(dv[6] & DEV_CAPS)) {
         mos1_load<false>(P, nullptr, nullptr, vd, vg, vs, vb, A.gmin, A.tstep, sv, nullptr);
         if (tied)
-            apply_tied(C, code, sv);
+            apply_tied(C, code, sv, false);
         else
-            apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
+            apply_some<20, true>(C, code, sv.v, M1_DC_ENTRIES, sv.rev);
         return;
     }
     // gate capacitance: capacitances and operating-point state from global memory
@@ -290,7 +324,7 @@ __device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS con
     else
         mos1_load<false>(P, &K, &op, vd, vg, vs, vb, A.gmin, A.tstep, sv, &gs);
     if (tied)
-        apply_tied(C, code, sv);
+        apply_tied(C, code, sv, true);  // charge entries have slots (hub cells among them) in both solves
     else
         apply<MOS1_NE, true>(C, code, sv.v, sv.rev);
     st[7 * s + o] = gs.vgs;
