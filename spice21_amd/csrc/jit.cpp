@@ -49,8 +49,10 @@ int64_t jit_ops(const Topology& t) {
 //   ahead, before the previous step's arithmetic, so their latency overlaps it.  A step stores what later
 //   code reads (a finished U row, the swept RHS).  A scheduling barrier between steps keeps the compiler
 //   from hoisting the loads of many steps at once, which would need more registers than a lane has.  It
-//   works on FP32 copies of A and of the residual; updates are fused multiply-adds and a zero c_k is not
-//   skipped: this factorisation only preconditions the FP64 Newton update.
+//   works on FP32 copies of A and of the residual; updates are fused multiply-adds, a zero c_k is not
+//   skipped, and each pivot's reciprocal (the one ldiv multiplies by) is taken once, stored in place of
+//   the pivot and reused by every row and by the backward sweep: this factorisation only preconditions
+//   the FP64 Newton update.
 //   (With FP64 values, two registers each, this shape spills and runs slower than the straight one.)
 //
 // Both keep the records' operation order and division (ldiv); FP64 performs exactly their arithmetic.
@@ -105,14 +107,18 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
     // ---- factorisation + forward sweep ----
     {
         std::vector<GenStep> steps;
-        std::ostringstream pre;  // pivots of rows without L entries, checked up front
+        std::ostringstream pre;  // pivots of rows without L entries: checked and inverted up front
         std::vector<int> def_step(t.nnz, NEVER), cdef_step(t.n, NEVER);  // step whose local holds the value
         std::vector<char> used(t.nnz, 0), cused(t.n, 0);
         int step = -1;
         for (int i = 0; i < t.n; i++) {
             const bool has_l = t.diag[i] >= 0 && t.diag[i] > t.rp[i];
             if (!has_l) {
-                if (i + 1 < t.n && t.diag[i] >= 0) pre << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
+                if (t.diag[i] >= 0) {
+                    pre << "    l = lu[" << t.diag[i] << "];\n";
+                    if (i + 1 < t.n) pre << "    m = fminf(m, fabsf(l));\n";
+                    pre << "    lu[" << t.diag[i] << "] = __builtin_amdgcn_rcpf(l);\n";
+                }
                 continue;
             }
             const int row_first = step + 1;  // row i's own values, once loaded, stay in locals for the row
@@ -142,9 +148,9 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
                 const int q1 = std::min(q0 + kChunk, t.diag[i]);
                 for (int q = q0; q < q1; q++) {
                     const int k = t.ci[q];
-                    const std::string a = use(q), piv = use(t.diag[k]), yk = usec(k);
-                    body << "    l = ldiv<V>(" << a << ", " << piv << "); " << yi << " = __builtin_fmaf(-" << yk
-                         << ", l, " << yi << ");\n";
+                    const std::string a = use(q), rpiv = use(t.diag[k]), yk = usec(k);  // pivot reciprocal
+                    body << "    l = " << a << " * " << rpiv << "; " << yi << " = __builtin_fmaf(-" << yk << ", l, "
+                         << yi << ");\n";
                     for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++) {
                         const std::string src = use(t.upd_src[x]), d = use(t.upd_dst[x]);
                         body << "    " << d << " = __builtin_fmaf(-" << src << ", l, " << d << ");\n";
@@ -152,14 +158,17 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
                     }
                 }
                 cdef_step[i] = step;
-                if (q1 == t.diag[i]) {  // row finished: its U row and swept RHS
+                if (q1 == t.diag[i]) {  // row finished: its U row (the pivot inverted) and swept RHS
+                    const std::string d = use(t.diag[i]);
+                    if (i + 1 < t.n) body << "    m = fminf(m, fabsf(" << d << "));\n";
+                    body << "    " << d << " = __builtin_amdgcn_rcpf(" << d << ");\n";
+                    def_step[t.diag[i]] = step;
                     for (int q = t.diag[i]; q < t.rp[i + 1]; q++)
                         if (def_step[q] >= row_first) {
                             body << "    lu[" << q << "] = v" << q << ";\n";
                             def_step[q] = step;
                         }
                     body << "    c[" << i << "] = " << yi << ";\n";
-                    if (i + 1 < t.n) body << "    z = z || " << use(t.diag[i]) << " == V(0);\n";
                 }
                 g.loads = loads.str();
                 g.body = body.str();
@@ -167,12 +176,12 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
             }
         }
         o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
-             "    V l;\n    bool z = false;\n";
+             "    V l, m = __builtin_inff();\n";
         for (int q = 0; q < t.nnz; q++)
             if (used[q]) o << "    V v" << q << ";\n";
         for (int k = 0; k < t.n; k++)
             if (cused[k]) o << "    V y" << k << ";\n";
-        o << pre.str() << emit_steps(steps) << "    return z;\n}\n";
+        o << pre.str() << emit_steps(steps) << "    return m == V(0);\n}\n";
     }
     // ---- backward sweep ----
     {
@@ -199,8 +208,8 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
                 }
                 if (q1 == u1) {
                     loads << "    u" << t.diag[i] << " = lu[" << t.diag[i] << "];\n";
-                    body << "    x" << i << " = ldiv<V>(y" << i << ", u" << t.diag[i] << ");\n    c[" << i
-                         << "] = x" << i << ";\n";
+                    body << "    x" << i << " = y" << i << " * u" << t.diag[i] << ";\n    c[" << i << "] = x" << i
+                         << ";\n";
                     xdef[i] = step;
                 }
                 g.loads = loads.str();
