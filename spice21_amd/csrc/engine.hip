@@ -615,8 +615,10 @@ static size_t inst_stride(size_t raw, int ipw, bool lane_per_instance = false) {
 }
 constexpr size_t JIT_STATIC_LDS = sizeof(int) * WAVE * MAX_WAVES;  // newton_wg's per-slot flags
 
-// Launch shape: instances per wavefront (the batch's setting, or auto_ipw) and wavefronts per workgroup
-// (the most resident instances per CU within the LDS).
+// Launch shape: instances per wavefront (the batch's setting, or auto_ipw) and wavefronts per workgroup:
+// the most that still leave a workgroup for every CU (or as many workgroups as one wavefront per
+// workgroup gives, for a smaller batch) and fit in the LDS -- a larger workgroup shares the generated
+// factorisation's lanes among more instances, a smaller one spreads the batch over more CUs.
 static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_inst, size_t max_block, int simds,
                         Args& A, size_t& stride) {
     int ipw = b->ipw;
@@ -626,19 +628,20 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
         blob += JIT_STATIC_LDS;
     }
     if (ipw <= 0) ipw = auto_ipw(total_inst, simds);
+    const long cus = std::max(simds / 4, 1);
     for (;; ipw /= 2) {
         stride = inst_stride(raw, ipw, jit);
+        auto workgroups = [&](int k) {
+            long n = 0;
+            for (const GroupHost& g : b->groups) n += (g.n_inst + k * ipw - 1) / (k * ipw);
+            return n;
+        };
+        const long target = std::min(cus, workgroups(1));
         int best = 0;
-        size_t best_res = 0;
         for (int k = 1; k <= MAX_WAVES; k++) {
-            if (k > 1 && (size_t)(k - 1) * ipw >= (size_t)total_inst) break;  // nothing left to fill
-            const size_t bytes = blob + (size_t)k * ipw * stride;
-            if (bytes > max_block) break;
-            const size_t res = (LDS_PER_CU / bytes) * k * ipw;
-            if (res >= best_res) {  // at equal residency, larger workgroups share the generated LU's lanes better
-                best_res = res;
-                best = k;
-            }
+            if (blob + (size_t)k * ipw * stride > max_block) break;
+            if (k > 1 && workgroups(k) < target) break;
+            best = k;
         }
         if (best > 0) {
             A.ipw = ipw;
