@@ -75,11 +75,11 @@ struct GroupHost {
     bool has_jit = false;
     int jit_ipw = 1;                        // instances per wavefront the generated kernel is built for
     std::string prepared;  // analysis the uploaded structure was built for ("" = none)
-    // Pivoting of the operating-point structure: by default the search on instance 0's matrix at x = 0;
-    // pivot_row / pivot_col give the order instead; pivot_xs / pivot_vds (iterates, with diode guesses in
-    // component order) add the reference's later searches on instance 0's matrices at those iterates.
+    // Pivoting: by default the search on instance 0's first matrix (x = 0); pivot_row / pivot_col give
+    // the order instead; pivot_later (matrix values in element order) adds the reference's later
+    // searches on those matrices.
     std::vector<int> pivot_row, pivot_col;
-    std::vector<std::vector<double>> pivot_xs, pivot_vds;
+    std::vector<std::vector<double>> pivot_later;
 };
 
 }  // namespace s21
@@ -302,24 +302,15 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
         std::vector<double> re, im;
         host_assemble_ac(t, op0, grd0, grs0, lin0, ac->omega0, re, im);
         topo_pivot_cx(t, re, im);
-    } else if (!tr && !g.pivot_row.empty()) {
+    } else if (!g.pivot_row.empty()) {
         topo_pivot_forced(t, g.pivot_row, g.pivot_col);
     } else {
         std::vector<double> x0(t.n, 0.0), vals, rhs;
         host_assemble_op(t, h0, o, x0, vals, rhs);
-        if (!tr && !g.pivot_xs.empty()) {
-            std::vector<std::vector<double>> later;
-            for (size_t k = 0; k < g.pivot_xs.size(); k++) {
-                HostInstance hk = h0;
-                hk.dio_vd = g.pivot_vds[k];
-                std::vector<double> v, r;
-                host_assemble_op(t, hk, o, g.pivot_xs[k], v, r);
-                later.push_back(std::move(v));
-            }
-            topo_pivot_sequence(t, vals, later);
-        } else {
+        if (!g.pivot_later.empty())
+            topo_pivot_sequence(t, vals, g.pivot_later);
+        else
             topo_pivot(t, vals);
-        }
     }
     topo_schedule(t, has_caps);
 
@@ -536,9 +527,10 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     // ---- outputs ----
     const size_t s_x = pad((size_t)NI * t.n), s_st = ((NI * sizeof(int) + 255) / 256) * 256,
                  s_it = ((NI * sizeof(long long) + 255) / 256) * 256, s_dbg = pad((size_t)t.nnz + t.n),
-                 s_prof = b->profile ? ((NI * NPROF * sizeof(long long) + 255) / 256) * 256 : 0;
+                 s_prof = b->profile ? ((NI * NPROF * sizeof(long long) + 255) / 256) * 256 : 0,
+                 s_ra = ((NI * 2 * sizeof(int) + 255) / 256) * 256;
     g.d_out.reset(new DevMem());
-    g.d_out->alloc(s_x + s_st + s_it + s_dbg + s_prof + 256);
+    g.d_out->alloc(s_x + s_st + s_it + s_dbg + s_prof + s_ra + 256);
     HIPCHECK(hipMemsetAsync(g.d_out->p, 0, g.d_out->bytes, b->stream));
     char* op = (char*)g.d_out->p;
     g.gg.x = (double*)op;
@@ -546,6 +538,7 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     g.gg.iters = (long long*)(op + s_x + s_st);
     g.gg.dbg = (double*)(op + s_x + s_st + s_it);
     g.gg.prof = b->profile ? (long long*)(op + s_x + s_st + s_it + s_dbg) : nullptr;
+    g.gg.resume_at = (int*)(op + s_x + s_st + s_it + s_dbg + s_prof);
     g.gg.n_inst = NI;
     g.gg.wave = nullptr;
 
@@ -742,6 +735,7 @@ static Args base_args(const s21_batch* b, int mode) {
     A.inst_bytes = 0;
     A.capture = 0;
     A.resume = 0;
+    A.asm_tran = 0;
     A.omega = nullptr;
     return A;
 }
@@ -775,73 +769,132 @@ static void group_from(GroupHost& g, const s21_group& sg) {
     }
 }
 
-// ---- zero-pivot recovery of the operating point ----
+// ---- zero-pivot recovery ----
 // The reference searches its pivots again at every factorisation (Matrix::solve after each reset,
 // sparse21/mod.rs:647-674, 929-932); the engine keeps the first search's order, which is the order those
 // searches return for as long as it stays acceptable.  An instance whose kept order meets an exact zero
-// pivot (status S21_PIVOT) resumes that iteration in a launch of its own: a mixed-precision instance first
-// in FP64 on the same order, then on the order the reference's search picks at that iterate, searched from
-// the arrangement and fill its earlier factorisations left.  The resumed solve keeps its iteration count
-// and the limit of 100 (analysis.rs:173); each resumption uses at least one iteration.
+// pivot (status S21_PIVOT) resumes the interrupted iteration -- of the operating point or of a transient
+// time point -- in a launch of its own: a mixed-precision instance first in FP64 on the same order, then
+// on the order the reference's search picks on that iteration's matrix (assembled on the GPU from the
+// instance's state), searched from the arrangement and fill its earlier factorisations left.  The
+// resumed solve keeps its iteration count and the limit of 100 (analysis.rs:173); each resumption uses
+// at least one iteration.
 constexpr int MAX_RESUME = 8;
 
+// An instance's device state, by component (the device order of a structure depends on its colouring).
+struct InstState {
+    std::vector<std::vector<double>> m1, d;  // per Mos1 / diode component: its state fields
+    std::vector<double> q;                    // per linear component: committed capacitor charge
+};
+static void dev_maps(const Topology& t, std::vector<int>& m1, std::vector<int>& d, std::vector<int>& lin) {
+    m1.assign(t.dev_comp.begin() + t.cls_start[CLS_MOS1], t.dev_comp.begin() + t.cls_start[CLS_MOS1T + 1]);
+    d.assign(t.dev_comp.begin() + t.cls_start[CLS_DIODE], t.dev_comp.begin() + t.cls_start[CLS_DIODE + 1]);
+    lin.assign(t.dev_comp.begin() + t.cls_start[CLS_LIN], t.dev_comp.begin() + t.cls_start[CLS_LIN + 1]);
+}
+// State of instance i; the diode guess is the one the last evaluation started from (field 3).
+static InstState read_state(const GroupHost& g, int i) {
+    std::vector<int> m1, d, lin;
+    dev_maps(g.topo, m1, d, lin);
+    const size_t NI = g.n_inst, nm = m1.size(), nd = d.size(), nl = lin.size();
+    InstState S;
+    const size_t ncomp = g.topo.comps.size();
+    S.m1.assign(ncomp, {});
+    S.d.assign(ncomp, {});
+    S.q.assign(ncomp, 0.0);
+    std::vector<double> tmp;
+    auto rd = [&](const double* base, size_t nf, size_t ndev, std::vector<std::vector<double>>& out,
+                  const std::vector<int>& comp) {
+        tmp.resize(nf * NI * ndev);
+        if (!tmp.empty())
+            HIPCHECK(hipMemcpy(tmp.data(), base, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ndev; k++) {
+            std::vector<double>& v = out[comp[k]];
+            v.resize(nf);
+            for (size_t f = 0; f < nf; f++) v[f] = tmp[f * NI * ndev + i * ndev + k];
+        }
+    };
+    rd(g.gg.m1_state, 2 * M1_STATE_F, nm, S.m1, m1);
+    rd(g.gg.d_state, D_STATE_F, nd, S.d, d);
+    for (size_t k = 0; k < nd; k++) S.d[d[k]][0] = S.d[d[k]][3];
+    if (nl) {
+        tmp.resize(NI * nl);
+        HIPCHECK(hipMemcpy(tmp.data(), g.gg.c_q, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nl; k++) S.q[lin[k]] = tmp[i * nl + k];
+    }
+    return S;
+}
+// State of a one-instance group.
+static void write_state(GroupHost& h, const InstState& S) {
+    std::vector<int> m1, d, lin;
+    dev_maps(h.topo, m1, d, lin);
+    const size_t nm = m1.size(), nd = d.size(), nl = lin.size();
+    std::vector<double> tmp;
+    auto wr = [&](double* base, size_t nf, size_t ndev, const std::vector<std::vector<double>>& in,
+                  const std::vector<int>& comp) {
+        tmp.assign(nf * ndev, 0.0);
+        for (size_t k = 0; k < ndev; k++)
+            for (size_t f = 0; f < nf && f < in[comp[k]].size(); f++) tmp[f * ndev + k] = in[comp[k]][f];
+        if (!tmp.empty())
+            HIPCHECK(hipMemcpy(base, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    };
+    wr(h.gg.m1_state, 2 * M1_STATE_F, nm, S.m1, m1);
+    wr(h.gg.d_state, D_STATE_F, nd, S.d, d);
+    if (nl) {
+        tmp.assign(nl, 0.0);
+        for (size_t k = 0; k < nl; k++) tmp[k] = S.q[lin[k]];
+        HIPCHECK(hipMemcpy(h.gg.c_q, tmp.data(), nl * sizeof(double), hipMemcpyHostToDevice));
+    }
+}
+
 struct Stalled {
-    int g = 0, i = 0;             // group, instance
-    std::vector<double> x;        // iterate of the interrupted iteration
-    std::vector<double> vd;       // per diode (component order): the guess that iteration's evaluation began from
-    long long iters = 0;          // loads before the interrupted iteration
-    bool fp64 = false;            // the interrupted solve ran in FP64
-    std::vector<int> row, col;    // the order that met the zero pivot
-    std::vector<std::vector<double>> xs, vds;  // iterates (and diode guesses) of the searches so far
+    int g = 0, i = 0;           // group, instance
+    std::vector<double> x;      // iterate of the interrupted iteration
+    InstState state;            // state before that iteration
+    long long iters = 0;        // loads before the interrupted iteration
+    int pt = 0, used = 0;       // its time point, and the iterations its solve had used before it
+    bool fp64 = false;          // the interrupted solve ran in FP64
+    std::vector<int> row, col;  // the order that met the zero pivot
+    std::vector<std::vector<double>> later;  // matrices of the searches after the first
 };
 
-static std::vector<double> diode_field(const GroupHost& g, int f) {
-    const Topology& t = g.topo;
-    const size_t nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE], s = (size_t)g.n_inst * nd;
-    std::vector<double> v(s);
-    if (s) HIPCHECK(hipMemcpy(v.data(), g.gg.d_state + f * s, s * sizeof(double), hipMemcpyDeviceToHost));
-    return v;
-}
-// Diode guesses of instance i in component order from a state field [n_inst][n_d] in device order.
-static std::vector<double> diodes_by_component(const GroupHost& g, int i, const std::vector<double>& field) {
-    const Topology& t = g.topo;
-    const int nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE];
-    std::vector<int> rank(t.comps.size(), -1);
-    int r = 0;
-    for (size_t c = 0; c < t.comps.size(); c++)
-        if (t.comps[c].type == S21_COMP_DIODE) rank[c] = r++;
-    std::vector<double> vd(r, 0.0);
-    for (int d = 0; d < nd; d++) vd[rank[t.dev_comp[t.cls_start[CLS_DIODE] + d]]] = field[(size_t)i * nd + d];
-    return vd;
+static Stalled stalled_from(const GroupHost& g, int gi, int i, bool fp64) {
+    Stalled s;
+    s.g = gi;
+    s.i = i;
+    s.x.resize(g.topo.n);
+    HIPCHECK(hipMemcpy(s.x.data(), g.gg.x + (size_t)i * g.topo.n, g.topo.n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&s.iters, g.gg.iters + i, sizeof(long long), hipMemcpyDeviceToHost));
+    int ra[2];
+    HIPCHECK(hipMemcpy(ra, g.gg.resume_at + 2 * i, sizeof(ra), hipMemcpyDeviceToHost));
+    s.iters -= 1;
+    s.pt = ra[0];
+    s.used = ra[1] - 1;
+    s.state = read_state(g, i);
+    s.fp64 = fp64;
+    s.row = g.topo.row_i2e;
+    s.col = g.topo.col_i2e;
+    return s;
 }
 
-static void recover_zero_pivots(s21_batch* b) {
+// Device offset of group gi's waveform in the batch's wave buffer (the layout run() gives it).
+static size_t wave_offset(const s21_batch* b, int gi, int64_t npts) {
+    size_t off = 0;
+    for (int k = 0; k < gi; k++) off += (size_t)b->groups[k].n_inst * (size_t)npts * b->groups[k].topo.n;
+    return off;
+}
+
+static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_t npts) {
     std::vector<Stalled> todo;
     for (size_t gi = 0; gi < b->groups.size(); gi++) {
         GroupHost& g = b->groups[gi];
         std::vector<int> st(g.n_inst);
         HIPCHECK(hipMemcpy(st.data(), g.gg.status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<double> f3;
-        for (int i = 0; i < g.n_inst; i++) {
-            if (st[i] != S21_PIVOT) continue;
-            if (f3.empty()) f3 = diode_field(g, 3);
-            Stalled s;
-            s.g = (int)gi;
-            s.i = i;
-            s.x.resize(g.topo.n);
-            HIPCHECK(hipMemcpy(s.x.data(), g.gg.x + (size_t)i * g.topo.n, g.topo.n * sizeof(double),
-                               hipMemcpyDeviceToHost));
-            HIPCHECK(hipMemcpy(&s.iters, g.gg.iters + i, sizeof(long long), hipMemcpyDeviceToHost));
-            s.iters -= 1;
-            s.vd = diodes_by_component(g, i, f3);
-            s.fp64 = b->precision != S21_MIXED;
-            s.row = g.topo.row_i2e;
-            s.col = g.topo.col_i2e;
-            todo.push_back(std::move(s));
-        }
+        for (int i = 0; i < g.n_inst; i++)
+            if (st[i] == S21_PIVOT) todo.push_back(stalled_from(g, (int)gi, i, b->precision != S21_MIXED));
     }
     if (todo.empty()) return;
     fetch_results(b);  // host copies of every result, patched below
+    const int mode = tr ? MODE_TRAN : MODE_OP;
     for (int round = 0; round < MAX_RESUME && !todo.empty(); round++) {
         s21_batch sb;
         sb.opts = b->opts;
@@ -870,35 +923,70 @@ static void recover_zero_pivots(s21_batch* b) {
                 x.vals.assign(1, o.vals[s.i]);
                 h.overs.push_back(std::move(x));
             }
-            if (!s.fp64) {  // FP64 on the order that failed in FP32
-                h.pivot_row = s.row;
-                h.pivot_col = s.col;
-            } else {  // the searches so far, then the search at the interrupted iterate
-                h.pivot_xs = s.xs;
-                h.pivot_vds = s.vds;
-                h.pivot_xs.push_back(s.x);
-                h.pivot_vds.push_back(s.vd);
-            }
+            h.pivot_row = s.row;  // the order that failed: FP64 retry, or the structure for the assembly below
+            h.pivot_col = s.col;
         }
-        prepare_all(&sb, nullptr);
-        for (size_t k = 0; k < todo.size(); k++) {
+        auto load_inputs = [&](size_t k) {
             const Stalled& s = todo[k];
             GroupHost& h = sb.groups[k];
             HIPCHECK(hipMemcpy(h.gg.x, s.x.data(), s.x.size() * sizeof(double), hipMemcpyHostToDevice));
             HIPCHECK(hipMemcpy(h.gg.iters, &s.iters, sizeof(long long), hipMemcpyHostToDevice));
+            const int ra[2] = {s.pt, s.used};
+            HIPCHECK(hipMemcpy(h.gg.resume_at, ra, sizeof(ra), hipMemcpyHostToDevice));
+            write_state(h, s.state);
+        };
+        prepare_all(&sb, tr);
+        // Instances that already ran in FP64 need a new order: the interrupted iteration's matrix, as the
+        // kernel assembles it from the iterate and the state, searched after the earlier ones.
+        bool rebuilt = false;
+        for (size_t k = 0; k < todo.size(); k++) {
+            Stalled& s = todo[k];
+            if (!s.fp64) continue;
+            GroupHost& h = sb.groups[k];
+            load_inputs(k);
+            Args A = base_args(&sb, MODE_ASSEMBLE);
+            A.asm_tran = s.pt > 0;
+            A.tstep = tr ? tr->tstep : 0.0;
+            A.dbg_inst = 0;
+            A.waves = 1;
+            A.ipw = 1;
+            const size_t stride = inst_stride(h.inst_bytes, 1);
+            A.inst_bytes = (int)stride;
+            GroupGPU gg = h.gg;
+            gg.blk0 = 0;
+            if (!sb.d_groups) sb.d_groups.reset(new DevMem());
+            if (sb.d_groups->bytes < sizeof(GroupGPU)) sb.d_groups->alloc(sizeof(GroupGPU));
+            HIPCHECK(hipMemcpy(sb.d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice));
+            launch_any(&sb, (const GroupGPU*)sb.d_groups->p, 1, 1, h.blob_bytes + stride, A);
+            HIPCHECK(hipStreamSynchronize(sb.stream));
             const Topology& t = h.topo;
-            const int nd = t.cls_start[CLS_DIODE + 1] - t.cls_start[CLS_DIODE];
-            std::vector<int> rank(t.comps.size(), -1);
-            int r = 0;
-            for (size_t c = 0; c < t.comps.size(); c++)
-                if (t.comps[c].type == S21_COMP_DIODE) rank[c] = r++;
-            std::vector<double> vd0(nd);
-            for (int d = 0; d < nd; d++) vd0[d] = s.vd[rank[t.dev_comp[t.cls_start[CLS_DIODE] + d]]];
-            if (nd) HIPCHECK(hipMemcpy(h.gg.d_state, vd0.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+            std::vector<double> dbg(t.nnz + t.n), vals(t.erow.size());
+            HIPCHECK(hipMemcpy(dbg.data(), h.gg.dbg, dbg.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t e = 0; e < vals.size(); e++) vals[e] = dbg[t.elem_pos[e]];
+            s.later.push_back(std::move(vals));
+            h.pivot_row.clear();
+            h.pivot_col.clear();
+            h.pivot_later = s.later;
+            h.prepared.clear();
+            rebuilt = true;
         }
-        Args A = base_args(&sb, MODE_OP);
+        if (rebuilt) prepare_all(&sb, tr);
+        for (size_t k = 0; k < todo.size(); k++) load_inputs(k);
+        std::unique_ptr<DevMem> wave;
+        if (tr) {
+            size_t total = 0;
+            for (GroupHost& h : sb.groups) total += (size_t)npts * h.topo.n;
+            wave.reset(new DevMem());
+            wave->alloc(std::max<size_t>(total, 1) * sizeof(double));
+        }
+        Args A = base_args(&sb, mode);
         A.resume = 1;
-        run(&sb, A, nullptr, 1);
+        if (tr) {
+            A.tstep = tr->tstep;
+            A.tstop = tr->tstop;
+            A.npts = npts;
+        }
+        run(&sb, A, tr ? (double*)wave->p : nullptr, tr ? npts : 1);
         sb.fetched = false;
         fetch_results(&sb);
         std::vector<Stalled> next;
@@ -907,18 +995,17 @@ static void recover_zero_pivots(s21_batch* b) {
             GroupHost& h = sb.groups[k];
             GroupHost& g = b->groups[s.g];
             const int n = g.topo.n;
+            auto copy_wave = [&](int64_t p1) {  // points [s.pt, p1) of this round into the batch's waveform
+                if (!tr || p1 <= s.pt) return;
+                double* dst = (double*)b->d_wave->p + wave_offset(b, s.g, npts) + ((size_t)s.i * npts + s.pt) * n;
+                const double* src = (const double*)wave->p + wave_offset(&sb, (int)k, npts) + (size_t)s.pt * n;
+                HIPCHECK(hipMemcpy(dst, src, (size_t)(p1 - s.pt) * n * sizeof(double), hipMemcpyDeviceToDevice));
+            };
             if (h.status[0] == S21_PIVOT && round + 1 < MAX_RESUME) {
-                Stalled t2;
-                t2.g = s.g;
+                Stalled t2 = stalled_from(h, s.g, 0, true);
                 t2.i = s.i;
-                t2.x = h.x;
-                t2.vd = diodes_by_component(h, 0, diode_field(h, 3));
-                t2.iters = h.iters[0] - 1;
-                t2.fp64 = true;
-                t2.row = h.topo.row_i2e;
-                t2.col = h.topo.col_i2e;
-                t2.xs = h.pivot_xs;
-                t2.vds = h.pivot_vds;
+                t2.later = s.later;
+                copy_wave(t2.pt);
                 next.push_back(std::move(t2));
                 continue;
             }
@@ -928,6 +1015,7 @@ static void recover_zero_pivots(s21_batch* b) {
             HIPCHECK(hipMemcpy(g.gg.x + (size_t)s.i * n, h.x.data(), n * sizeof(double), hipMemcpyHostToDevice));
             HIPCHECK(hipMemcpy(g.gg.status + s.i, &h.status[0], sizeof(int), hipMemcpyHostToDevice));
             HIPCHECK(hipMemcpy(g.gg.iters + s.i, &h.iters[0], sizeof(long long), hipMemcpyHostToDevice));
+            copy_wave(npts);
         }
         todo.swap(next);
     }
@@ -1032,7 +1120,7 @@ int s21_batch_op(s21_batch* b) {
         Args A = base_args(b, MODE_OP);
         run(b, A, nullptr, 1);
         b->fetched = false;
-        recover_zero_pivots(b);
+        recover_zero_pivots(b, nullptr, 1);
         b->npts = 0;
         b->axis.clear();
         b->last_mode = MODE_OP;
@@ -1057,6 +1145,7 @@ int s21_batch_tran(s21_batch* b, const s21_tran_options* tr) {
         A.npts = npts;
         run(b, A, (double*)b->d_wave->p, npts);
         b->fetched = false;
+        recover_zero_pivots(b, tr, npts);
         b->npts = npts;
         b->axis.resize(npts);
         double t = 0.0;

@@ -75,6 +75,8 @@ struct GroupGPU {
     double* wave;           // [n_inst][npts][n] (transient) or nullptr
     double* dbg;            // assembly dump [nnz + n] (assemble mode)
     long long* prof;        // [n_inst][8] phase cycle counters (profiling mode) or nullptr
+    int* resume_at;         // [n_inst][2]: time point of the last solve and the iterations it had used
+                            // before its current one (written at exit; read by a resuming launch)
 };
 
 enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2, MODE_AC = 3 };
@@ -90,9 +92,11 @@ struct Args {
     int ipw;         // instances per wavefront (1, 2, 4, 8 or 16)
     int inst_bytes;  // LDS bytes per instance
     int capture;     // operating point: store each Mos1's small-signal operating point for AC
-    int resume;      // the first solve continues an interrupted one: G.x holds its iterate, G.iters the
-                     // iterations it has used, and its first convergence test fails (the interrupted
-                     // iteration had passed that test already)
+    int resume;      // each instance continues an interrupted solve at G.resume_at's time point: G.x holds
+                     // the iterate, G.iters the iterations so far, G.resume_at those of that solve, the
+                     // state arrays the state before the interrupted iteration; its first convergence
+                     // test fails (the interrupted iteration had passed that test already)
+    int asm_tran;    // assemble mode: the transient matrix (companions from the state arrays)
     const double* omega;  // AC: angular frequency of each point
 };
 
@@ -1222,7 +1226,7 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     wave_sync();
 
     if (A.mode == MODE_ASSEMBLE) {
-        assemble(C, valid, false, A);
+        assemble(C, valid, A.asm_tran != 0, A);
         wave_sync();
         for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
         return;
@@ -1231,11 +1235,14 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
     const long long tstart = A.prof ? cycles() : 0;
     long long iters = 0;
-    int it_left = A.max_iter;
+    int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
     if (A.resume && valid) {
         iters = G.iters[inst];
-        it_left = A.max_iter - (int)iters;
+        pt0 = G.resume_at[2 * inst];
+        used0 = G.resume_at[2 * inst + 1];
     }
+    int last_pt = 0;
+    long long last_start = 0;  // iteration count when the last solve began
     LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
     auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
         if constexpr (LU::lane_per_instance) {
@@ -1246,20 +1253,28 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
             return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
         }
     };
-    int st = solve(valid, false, it_left, !A.resume);
+    last_start = iters - used0;
+    const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
+    int st = solve(valid && op_here, false, A.max_iter - (A.resume ? used0 : 0), !A.resume);
     if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
     if (A.mode == MODE_TRAN) {
         bool alive = valid && st == S21_OK;
         double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
-        for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
+        for (int k = alive && op_here ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
         for (long long pt = 1; pt < A.npts; pt++) {  // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
             if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
-            const int s2 = solve(alive, true, A.max_iter, true);
-            if (alive) {
+            const bool here = alive && pt >= pt0;   // a resumed instance joins at its own point
+            const bool cont = A.resume && pt == pt0;  // ... with the solve it continues
+            if (here) {
+                last_pt = (int)pt;
+                last_start = cont ? iters - used0 : iters;
+            }
+            const int s2 = solve(here, true, cont ? A.max_iter - used0 : A.max_iter, !cont);
+            if (here) {
                 st = s2;
                 alive = s2 == S21_OK;
             }
-            for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
+            for (int k = here && alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
         }
     }
     if (!valid) return;
@@ -1267,6 +1282,10 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     if (C.sl == 0) {
         G.status[inst] = st;
         G.iters[inst] = iters;
+        if (G.resume_at) {
+            G.resume_at[2 * inst] = last_pt;
+            G.resume_at[2 * inst + 1] = (int)(iters - last_start);
+        }
         if (A.prof && G.prof) {
             pc[7] = cycles() - tstart;
             for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];

@@ -298,19 +298,30 @@ def test_random_circuits_dcop_outcome_matches_oracle(precision):
     assert not it_diff, it_diff
 
 
-def test_random_circuits_transient_matches_oracle():
-    """Twelve random netlists that the oracle can bias, 100 steps of 1 ps each, one group per netlist."""
-    cases = [c for c in _random_batch_cases() if c[2] is not None][:12]
-    b = Batch([Group(cn) for _, cn, _, _ in cases]).tran(1e-12, 1e-10)
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_random_circuits_transient_matches_oracle(precision):
+    """The 36 random netlists the oracle can bias, 100 steps of 1 ps each, one group per netlist: FP64 to
+    1e-9 of the oracle (relative, 1 V / 1 A floor), mixed precision to 1e-6 V."""
+    cases = [c for c in _random_batch_cases() if c[2] is not None]
+    b = Batch([Group(cn) for _, cn, _, _ in cases], precision=precision).tran(1e-12, 1e-10)
     for gi, (s, cn, _, _) in enumerate(cases):
-        t, X, names, _ = O.tran(cn, 1e-12, 1e-10)
+        try:
+            t, X, names, _ = O.tran(cn, 1e-12, 1e-10)
+        except O.OracleError as e:
+            st, _ = b.status(gi)
+            assert st[0] != 0, (s, e.desc)
+            continue
         st, _ = b.status(gi)
         assert st[0] == 0, (s, st[0])
         Xg = b.wave(gi, 0)
         assert Xg.shape == X.shape
-        err = float(np.max(np.abs(Xg - X) / np.maximum(np.abs(X), 1.0)))
-        log("  seed", s, "n", cn.n_vars, "max dev", err)
-        assert err <= 1e-9, (s, err)
+        if precision == FP64:
+            err = float(np.max(np.abs(Xg - X) / np.maximum(np.abs(X), 1.0)))
+            assert err <= 1e-9, (s, err)
+        else:
+            v = [k for k in range(cn.n_vars) if cn.kinds[k] == 0]
+            err = float(np.max(np.abs(Xg[:, v] - X[:, v])))
+            assert err <= 1e-6, (s, err)
 
 
 def test_capacitor_hub_cells_read_zero_in_operating_points():
