@@ -81,3 +81,40 @@ def py_ring3(acm=0.0):
         P.Vsrc(name="vvfb", p="ring0", n="ring3", dc=0, acm=acm),
         *comps,
     )
+
+
+def ring_caps(stages=5, tox=1e-8, overlap=2e-10, load=10e-15):
+    """The 5-stage ring of the single-ring workload with gate capacitance: both Mos1 models set tox (Meyer
+    gate charge, comps/mos.rs:725-806) and the gate-source/drain/bulk overlap capacitances, and each stage
+    node has a 10 fF load to ground."""
+    n = Netlist(signals=[str(k + 1) for k in range(stages)] + ["vdd"])
+    caps = dict(tox=tox, cgso=overlap, cgdo=overlap, cgbo=overlap)
+    nm = n.mos1_model(vt0=0.4, kp=100e-6, **caps)
+    pm = n.mos1_model(mos_type=1, vt0=0.4, kp=50e-6, **caps)
+    ni = n.mos1_inst(w=1e-6, l=0.1e-6)
+    pi = n.mos1_inst(w=2e-6, l=0.1e-6)
+    n.v("vdd", 1.0, "vdd", "")
+    for k in range(stages):
+        a, b = str(k + 1), str((k + 1) % stages + 1)
+        n.mos1(f"x{k}.p", b, a, "vdd", "vdd", pm, pi)
+        n.mos1(f"x{k}.n", b, a, "", "", nm, ni)
+        n.c(load, b, "")
+    return n
+
+
+def cs_amp_caps(n_stages=2, tox=1e-8, overlap=2e-10):
+    """A common-source amplifier chain with gate capacitance (Mos1 tox and overlaps), resistive loads from
+    a 1 V supply and an AC-driven 0.6 V input bias: Mos1::load_ac's capacitive stamps in the AC sweep."""
+    n = Netlist()
+    nm = n.mos1_model(vt0=0.4, kp=100e-6, tox=tox, cgso=overlap, cgdo=overlap, cgbo=overlap)
+    ni = n.mos1_inst(w=1e-6, l=0.1e-6)
+    n.v("vin", 0.6, "in", "", acm=1.0)
+    n.v("vdd", 1.0, "vdd", "")
+    prev = "in"
+    for k in range(n_stages):
+        out = f"o{k + 1}"
+        n.r(1.0 / 5e3, "vdd", out)
+        n.mos1(f"m{k + 1}", out, prev, "", "", nm, ni)
+        n.c(20e-15, out, "")
+        prev = out
+    return n

@@ -349,3 +349,63 @@ def test_capacitor_hub_cells_read_zero_in_operating_points():
     err = float(np.max(np.abs(Xg - Xa) / np.maximum(np.abs(Xa), 1e-12)))
     log("ring ac after a large batch: max rel", err)
     assert err <= 1e-9
+
+
+def test_mos1_gate_capacitance_transient_matches_oracle():
+    """Mos1 with tox and overlap capacitances (the Meyer charge path, its committed operating point and
+    the reference's tr.gs quirk) in a 500-point ring transient, one instance and a batch of 32 whose
+    overlap capacitance differs per instance: FP64 to 1e-9 of the oracle, mixed precision to 1e-6 V.
+    Node 1 starts at 0 V: from the ring's symmetric operating point (every node at 0.5 V) the symmetry
+    breaking amplifies any last-bit difference about e-fold every ten points."""
+    from tests.circuits import ring_caps
+
+    nl = ring_caps()
+    cn = nl.to_c()
+    t, X, names, _ = O.tran(cn, 1e-12, 5e-10, [("1", 0.0)])
+    b = Batch(Group(cn)).tran(1e-12, 5e-10, [("1", 0.0)])
+    assert b.status()[0][0] == 0
+    assert np.array_equal(b.axis(), t)
+    err = float(np.max(np.abs(b.wave(0, 0) - X)))
+    log("ring with gate capacitance, FP64: max |dx|", err)
+    assert err <= 1e-9
+    m = Batch(Group(cn), precision=MIXED).tran(1e-12, 5e-10, [("1", 0.0)])
+    assert m.status()[0][0] == 0
+    assert float(np.max(np.abs(m.wave(0, 0)[:, :5] - X[:, :5]))) <= 1e-6
+    # a batch: every Mos1's gate-drain overlap varies per instance (the gate-source one never reaches a
+    # stamp: the reference's gd integration overwrites tr.gs, comps/mos.rs:788, 801)
+    n_inst = 32
+    mos = [ci for ci, c in enumerate(nl.comps) if c[0] == 4]  # S21_COMP_MOS1
+    vals = 2e-10 * (1.0 + 0.5 * np.linspace(-1.0, 1.0, n_inst))
+    over = [(ci, "model", "cgdo", vals) for ci in mos]
+    bb = Batch(Group(cn, n_inst, over)).tran(1e-12, 5e-10, [("1", 0.0)])
+    st, _ = bb.status()
+    worst, failed = 0.0, []
+    for i in range(n_inst):  # the reference fails to converge on some of these; so must the engine
+        try:
+            ti, Xi, _, _ = O.tran(materialize(nl, over, i).to_c(), 1e-12, 5e-10, [("1", 0.0)])
+        except O.OracleError as e:
+            assert e.desc == "Convergence Failed" and st[i] == 2, (i, e.desc, st[i])
+            failed.append(i)
+            continue
+        assert st[i] == 0, (i, st[i])
+        worst = max(worst, float(np.max(np.abs(bb.wave(0, i) - Xi))))
+    log("  batch: max |dx|", worst, "instances the oracle cannot simulate", failed)
+    assert worst <= 1e-9
+    ok = [i for i in range(n_inst) if i not in failed]
+    assert float(np.max(np.abs(bb.wave(0, ok[0]) - bb.wave(0, ok[-1])))) > 0.1  # the overrides matter
+
+
+def test_mos1_gate_capacitance_ac_matches_oracle():
+    """The AC sweep of an amplifier chain whose Mos1 have gate capacitance (the jwC stamps of
+    Mos1::load_ac from the captured Meyer capacitances): FP64 to 1e-9 of the oracle."""
+    from tests.circuits import cs_amp_caps
+
+    cn = cs_amp_caps().to_c()
+    f, X, _ = O.ac(cn, 1000, 100_000_000_000, 40)
+    b = Batch(Group(cn)).ac(1000, 100_000_000_000, 40)
+    assert b.status()[0][0] == 0
+    assert np.array_equal(b.axis(), f)
+    Xg = b.wave(0, 0)
+    err = float(np.max(np.abs(Xg - X) / np.maximum(np.abs(X), 1e-12)))
+    log("amplifier with gate capacitance, AC: max rel", err)
+    assert err <= 1e-9
