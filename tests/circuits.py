@@ -118,3 +118,18 @@ def cs_amp_caps(n_stages=2, tox=1e-8, overlap=2e-10):
         n.c(20e-15, out, "")
         prev = out
     return n
+
+
+def diode_rc(cj0=1e-12, tt=0.0, forward=False):
+    """A 1 V source charging node a through 1 kohm against 1 pF and a diode with junction capacitance cj0
+    (and transit time tt), reverse-biased by default (cathode on a) or forward-biased (anode on a)."""
+    n = Netlist()
+    n.v("v1", 1.0, "in", "")
+    n.r(1e-3, "in", "a")
+    dm = n.diode_model(cj0=cj0, tt=tt)
+    if forward:
+        n.diode("d1", "a", "", dm, n.diode_inst())
+    else:
+        n.diode("d1", "", "a", dm, n.diode_inst())
+    n.c(1e-12, "a", "")
+    return n

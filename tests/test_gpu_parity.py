@@ -409,3 +409,25 @@ def test_mos1_gate_capacitance_ac_matches_oracle():
     err = float(np.max(np.abs(Xg - X) / np.maximum(np.abs(X), 1e-12)))
     log("amplifier with gate capacitance, AC: max rel", err)
     assert err <= 1e-9
+
+
+def test_diode_charge_transient_outcome_matches_oracle():
+    """Diode junction charge in a transient (node a charged through 1 kohm from 0 V): reverse-biased the
+    oracle converges at every point and the engine matches it to 1e-9; forward-biased the reference's
+    depletion-capacitance formula (comps/diode.rs:311-318) keeps Newton from converging, and the engine
+    reports the same "Convergence Failed"."""
+    from tests.circuits import diode_rc
+
+    cn = diode_rc().to_c()
+    t, X, names, it = O.tran(cn, 1e-11, 2e-9, [("a", 0.0)])
+    b = Batch(Group(cn)).tran(1e-11, 2e-9, [("a", 0.0)])
+    st, its = b.status()
+    assert st[0] == 0 and np.array_equal(b.axis(), t)
+    err = float(np.max(np.abs(b.wave(0, 0) - X)))
+    log("reverse-biased diode charge: max |dx|", err, "iterations", int(its[0]), "oracle", it)
+    assert err <= 1e-9
+    for forward, tt in ((True, 0.0), (True, 1e-10)):
+        cf = diode_rc(tt=tt, forward=forward).to_c()
+        with pytest.raises(O.OracleError, match="Convergence Failed"):
+            O.tran(cf, 1e-11, 2e-9, [("a", 0.0)])
+        assert Batch(Group(cf)).tran(1e-11, 2e-9, [("a", 0.0)]).status()[0][0] == 2
