@@ -27,7 +27,8 @@ namespace s21 {
     } while (0)
 
 constexpr int JIT_MIN_INST = 512;      // instances that repay a run-time compilation
-constexpr int64_t JIT_MAX_OPS = 40000;  // largest generated factorisation
+constexpr int64_t JIT_MAX_OPS = 40000;         // largest generated factorisation of one pattern
+constexpr int64_t JIT_MAX_TOTAL_OPS = 200000;  // ... and of all the patterns of one batch
 
 struct DevMem {
     void* p = nullptr;
@@ -71,9 +72,6 @@ struct GroupHost {
     std::vector<int> status;
     std::vector<long long> iters;
     size_t blob_bytes = 0, inst_bytes = 0;  // record blob; LDS bytes one instance needs
-    JitKernel jit;                          // kernel with this pattern's factorisation written out
-    bool has_jit = false;
-    int jit_ipw = 1;                        // instances per wavefront the generated kernel is built for
     std::string prepared;  // analysis the uploaded structure was built for ("" = none)
     // Pivoting: by default the search on instance 0's first matrix (x = 0); pivot_row / pivot_col give
     // the order instead; pivot_later (matrix values in element order) adds the reference's later
@@ -98,6 +96,9 @@ struct s21_batch {
     int ipw = 0;  // instances per wavefront; 0 = chosen per launch
     int jit = 0;  // generated factorisation: 0 = for large single-pattern batches, 1 = always, -1 = never
     bool last_jit = false;
+    // kernel with every group's factorisation written out (generated at the last preparation), or none
+    s21::JitKernel jitk;
+    bool has_jit = false;
     std::vector<double> axis;
     std::unique_ptr<s21::DevMem> d_wave, d_groups;
     int last_mode = -1;
@@ -543,15 +544,6 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     g.gg.wave = nullptr;
 
     HIPCHECK(hipStreamSynchronize(b->stream));
-    // Generated factorisation: for a batch of one pattern with enough instances to repay the compilation.
-    // Its launch shape (instances per wavefront) is fixed here.
-    g.has_jit = false;
-    const bool want = !ac && (b->jit > 0 || (b->jit == 0 && b->groups.size() == 1 && NI >= JIT_MIN_INST));
-    if (want && t.sym_status == S21_OK && jit_ops(t) <= JIT_MAX_OPS) {
-        g.jit_ipw = b->ipw > 0 ? b->ipw : auto_ipw(NI, device_simds());
-        g.jit = jit_compile(jit_source(t, b->precision == S21_MIXED, 2));
-        g.has_jit = true;
-    }
     g.blob_bytes = blob_bytes;
     const size_t vw = ac ? 2 : 1;  // values per entry: complex for AC
     g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) * vw +
@@ -568,8 +560,40 @@ static std::string prep_key(const s21_tran_options* tr) {
 }
 
 // Prepare every group for an analysis, or only reset the Newton state when the structure is current.
+// Generated factorisation: for a batch with enough instances to repay the compilation (or when asked),
+// one kernel with every group's pattern written out -- groups of one pattern share its functions.
+static void prepare_jit(s21_batch* b) {
+    b->has_jit = false;
+    int64_t total = 0, ops = 0;
+    for (const GroupHost& g : b->groups) total += g.n_inst;
+    if (b->jit < 0 || (b->jit == 0 && total < JIT_MIN_INST)) return;
+    std::vector<const Topology*> pats;
+    std::vector<int> group_pattern;
+    std::unordered_map<std::string, int> seen;  // pattern -> index, keyed by what its code is written from
+    for (const GroupHost& g : b->groups) {
+        const Topology& t = g.topo;
+        if (t.sym_status != S21_OK) return;
+        std::string key = std::to_string(t.n) + ":" + std::to_string(t.nnz);
+        for (const std::vector<int>* v : {&t.rp, &t.ci, &t.diag, &t.upd_ptr, &t.upd_src, &t.upd_dst})
+            key.append((const char*)v->data(), v->size() * sizeof(int)).push_back('|');
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            const int64_t k = jit_ops(t);
+            if (k > JIT_MAX_OPS) return;
+            ops += k;
+            it = seen.emplace(key, (int)pats.size()).first;
+            pats.push_back(&t);
+        }
+        group_pattern.push_back(it->second);
+    }
+    if (ops > JIT_MAX_TOTAL_OPS) return;
+    b->jitk = jit_compile(jit_source(pats, group_pattern, b->precision == S21_MIXED, 2));
+    b->has_jit = true;
+}
+
 static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
     const std::string key = prep_key(tr);
+    bool changed = false;
     for (GroupHost& g : b->groups) {
         if (g.prepared == key) {
             HIPCHECK(hipMemsetAsync(g.d_state->p, 0, g.d_state->bytes, b->stream));
@@ -578,8 +602,10 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
             g.prepared.clear();
             prepare_group(b, g, tr);
             g.prepared = key;
+            changed = true;
         }
     }
+    if (changed) prepare_jit(b);
     HIPCHECK(hipStreamSynchronize(b->stream));
 }
 
@@ -615,11 +641,8 @@ constexpr size_t JIT_STATIC_LDS = sizeof(int) * WAVE * MAX_WAVES;  // newton_wg'
 static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_inst, size_t max_block, int simds,
                         Args& A, size_t& stride) {
     int ipw = b->ipw;
-    const bool jit = b->groups.size() == 1 && b->groups[0].has_jit;
-    if (jit) {
-        ipw = b->groups[0].jit_ipw;
-        blob += JIT_STATIC_LDS;
-    }
+    const bool jit = b->has_jit;
+    if (jit) blob += JIT_STATIC_LDS;
     if (ipw <= 0) ipw = auto_ipw(total_inst, simds);
     const long cus = std::max(simds / 4, 1);
     for (;; ipw /= 2) {
@@ -680,13 +703,13 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     if (b->d_groups->bytes < hg.size() * sizeof(GroupGPU)) b->d_groups->alloc(hg.size() * sizeof(GroupGPU));
     HIPCHECK(hipMemcpyAsync(b->d_groups->p, hg.data(), hg.size() * sizeof(GroupGPU), hipMemcpyHostToDevice,
                             b->stream));
-    const bool jit = b->groups.size() == 1 && b->groups[0].has_jit && A.mode != MODE_ASSEMBLE;
+    const bool jit = b->has_jit && A.mode != MODE_ASSEMBLE;
     HIPCHECK(hipEventRecord(b->ev0, b->stream));
     if (jit) {
         const GroupGPU* dg = (const GroupGPU*)b->d_groups->p;
         int ng = (int)hg.size();
         void* args[] = {(void*)&dg, (void*)&ng, (void*)&A};
-        HIPCHECK(hipModuleLaunchKernel(b->groups[0].jit.fn, blk, 1, 1, WAVE * A.waves, 1, 1, (unsigned)smem, b->stream,
+        HIPCHECK(hipModuleLaunchKernel(b->jitk.fn, blk, 1, 1, WAVE * A.waves, 1, 1, (unsigned)smem, b->stream,
                                        args, nullptr));
     } else {
         launch_any(b, (const GroupGPU*)b->d_groups->p, (int)hg.size(), blk, smem, A);
@@ -1214,6 +1237,7 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
             prepare_group(b, g, nullptr, &acin[gi]);
             g.prepared = "ac";
         }
+        b->has_jit = false;  // the frequency loop runs the precompiled complex kernel
         std::vector<double> omega(F);
         for (int64_t p = 0; p < F; p++) omega[p] = 2.0 * M_PI * freqs[p];  // AcState::omega, analysis.rs:801
         DevMem d_omega;
@@ -1310,9 +1334,6 @@ int s21_batch_set_ipw(s21_batch* b, int32_t ipw) {
         if (!b) fail(S21_INVALID, "null batch");
         if (ipw != 0 && ipw != 1 && ipw != 2 && ipw != 4 && ipw != 8 && ipw != 16)
             fail(S21_INVALID, "instances per wavefront must be 0, 1, 2, 4, 8 or 16");
-        if (ipw != b->ipw)
-            for (GroupHost& g : b->groups)
-                if (g.has_jit) g.prepared.clear();  // the generated kernel is built for one shape
         b->ipw = ipw;
     });
 }

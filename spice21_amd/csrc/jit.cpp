@@ -74,8 +74,10 @@ std::string emit_steps(const std::vector<GenStep>& st) {
     return o.str();
 }
 
-void gen_straight(std::ostringstream& o, const Topology& t) {
-    o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
+// `fn` opens each generated function: its qualifiers and the name prefix (lu_gen / bw_gen follow it).
+void gen_straight(std::ostringstream& o, const Topology& t, const std::string& fn, const std::string& sfx) {
+    o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
+      << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
          "    V y, l, ck;\n";
     for (int i = 0; i < t.n; i++) {
         if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;  // no L entries
@@ -92,7 +94,8 @@ void gen_straight(std::ostringstream& o, const Topology& t) {
     o << "    bool z = false;\n";
     for (int i = 0; i + 1 < t.n; i++) o << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
     o << "    return z;\n}\n";
-    o << "template <typename V>\n__device__ inline void bw_gen(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n"
+    o << "template <typename V>\n" << fn << " void bw_gen" << sfx
+      << "(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n"
          "    V y;\n";
     for (int i = t.n - 1; i >= 0; i--) {
         o << "    y = (V)c[" << i << "];";
@@ -102,7 +105,7 @@ void gen_straight(std::ostringstream& o, const Topology& t) {
     o << "}\n";
 }
 
-void gen_steps(std::ostringstream& o, const Topology& t) {
+void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, const std::string& sfx) {
     const int NEVER = -1000000;
     // ---- factorisation + forward sweep ----
     {
@@ -175,7 +178,8 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
                 steps.push_back(std::move(g));
             }
         }
-        o << "template <typename V>\n__device__ inline bool lu_gen(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
+        o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
+          << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
              "    V l, m = __builtin_inff();\n";
         for (int q = 0; q < t.nnz; q++)
             if (used[q]) o << "    V v" << q << ";\n";
@@ -218,7 +222,8 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
                 q0 = q1;
             } while (q0 < u1);
         }
-        o << "template <typename V>\n__device__ inline void bw_gen(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n";
+        o << "template <typename V>\n" << fn << " void bw_gen" << sfx
+          << "(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n";
         for (int q = 0; q < t.nnz; q++) o << "    V u" << q << ";\n";
         for (int k = 0; k < t.n; k++) o << "    V x" << k << ", y" << k << ";\n";
         o << emit_steps(steps) << "}\n";
@@ -226,19 +231,46 @@ void gen_steps(std::ostringstream& o, const Topology& t) {
 }
 }  // namespace
 
-std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
+std::string jit_source(const std::vector<const Topology*>& ts, const std::vector<int>& group_pattern, bool mixed,
+                       int waves_per_simd) {
     std::ostringstream o;
     o << "#include \"kernel.h\"\nnamespace s21 {\n";
-    if (mixed)
-        gen_steps(o, t);
-    else
-        gen_straight(o, t);
+    // One pattern: the functions inline into the kernel.  Several: each stays a function of its own, so
+    // the kernel's registers are not allocated for all of them at once.
+    const std::string fn = ts.size() == 1 ? "__device__ inline" : "__device__ __noinline__";
+    for (size_t p = 0; p < ts.size(); p++) {
+        if (mixed)
+            gen_steps(o, *ts[p], fn, "_" + std::to_string(p));
+        else
+            gen_straight(o, *ts[p], fn, "_" + std::to_string(p));
+    }
+    // group of the batch -> its pattern's functions
+    auto dispatch = [&](const char* ret, const char* name) {
+        std::ostringstream d;
+        if (ts.size() == 1) {
+            d << "        " << ret << name << "_0<V>(lu, c);\n";
+            return d.str();
+        }
+        d << "        switch (g) {\n";
+        for (size_t p = 0; p < ts.size(); p++) {
+            for (size_t g = 0; g < group_pattern.size(); g++)
+                if (group_pattern[g] == (int)p) d << "        case " << g << ":\n";
+            d << "            " << ret << name << "_" << p << "<V>(lu, c);" << (ret[0] ? "" : " break;") << "\n";
+        }
+        d << "        }\n";
+        if (ret[0]) d << "        return true;\n";
+        return d.str();
+    };
     o << "struct LuGen {\n"
          "    static constexpr bool lane_per_instance = true;\n"
          "    template <typename V>\n"
-         "    static __device__ bool factor_lane(LDS V* lu, LDS V* c) { return lu_gen<V>(lu, c); }\n"
+         "    static __device__ bool factor_lane(int g, LDS V* lu, LDS V* c) {\n"
+      << dispatch("return ", "lu_gen")
+      << "    }\n"
          "    template <typename V>\n"
-         "    static __device__ void backward_lane(LDS const V* lu, LDS V* c) { bw_gen<V>(lu, c); }\n"
+         "    static __device__ void backward_lane(int g, LDS const V* lu, LDS V* c) {\n"
+      << dispatch("", "bw_gen")
+      << "    }\n"
          "};\n}  // namespace s21\n";
     o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES, " << waves_per_simd
       << ") void s21_newton_jit("
@@ -246,6 +278,10 @@ std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
          "    s21::newton_body<"
       << (mixed ? "true" : "false") << ", s21::LuGen>(groups, n_groups, A);\n}\n";
     return o.str();
+}
+
+std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
+    return jit_source(std::vector<const Topology*>{&t}, std::vector<int>{0}, mixed, waves_per_simd);
 }
 
 namespace {

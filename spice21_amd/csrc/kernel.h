@@ -1044,7 +1044,7 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
 template <bool MIXED, class LU>
 __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
                          int it_left, bool prev_ok, LDS int* s_act, int slot, int n_slots, LDS char* region0,
-                         int stride) {
+                         int stride, int gidx) {
     const TopoGPU& T = C.T;
     const bool prof = A.prof != 0;
     long long t0 = 0, t1 = 0;
@@ -1085,11 +1085,11 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
             bool z;
             if (MIXED) {
                 LDS float* lu = (LDS float*)(a + T.nnz + T.n + T.n_hub + T.n_part);
-                z = LU::template factor_lane<float>(lu, lu + T.nnz);
-                if (!z) LU::template backward_lane<float>(lu, lu + T.nnz);
+                z = LU::template factor_lane<float>(gidx, lu, lu + T.nnz);
+                if (!z) LU::template backward_lane<float>(gidx, lu, lu + T.nnz);
             } else {
-                z = LU::template factor_lane<double>(a, c);
-                if (!z) LU::template backward_lane<double>(a, c);
+                z = LU::template factor_lane<double>(gidx, a, c);
+                if (!z) LU::template backward_lane<double>(gidx, a, c);
             }
             s_act[sl] = z ? 2 : 1;
         }
@@ -1248,7 +1248,7 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
         if constexpr (LU::lane_per_instance) {
             __shared__ int s_act[WAVE * MAX_WAVES];
             return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
-                                        A.waves * A.ipw, region0, A.inst_bytes);
+                                        A.waves * A.ipw, region0, A.inst_bytes, g);
         } else {
             return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
         }

@@ -431,3 +431,31 @@ def test_diode_charge_transient_outcome_matches_oracle():
         with pytest.raises(O.OracleError, match="Convergence Failed"):
             O.tran(cf, 1e-11, 2e-9, [("a", 0.0)])
         assert Batch(Group(cf)).tran(1e-11, 2e-9, [("a", 0.0)]).status()[0][0] == 2
+
+
+def test_generated_factorisation_several_patterns():
+    """One generated kernel for a batch of several patterns (each group's factorisation chosen by its
+    index): FP64 bit for bit the record-driven results on the 60 random netlists (one group each), and
+    the uneven ring batch's transient against the oracle."""
+    cases = _random_batch_cases()
+    groups = [Group(cn) for _, cn, _, _ in cases]
+    a = Batch(groups).set_jit(-1).op()
+    b = Batch(groups).set_jit(1).op()
+    assert a.config()[2] is False and b.config()[2] is True
+    for gi in range(len(groups)):
+        sa, ia = a.status(gi)
+        sb, ib = b.status(gi)
+        assert sa[0] == sb[0] and ia[0] == ib[0], (gi, sa[0], sb[0], ia[0], ib[0])
+        if sa[0] == 0:
+            assert np.array_equal(a.x(gi), b.x(gi)), gi
+    gs = W.ring_groups(48, seed=0xC3)[:6]
+    rg = [Group(n.to_c(), k, over) for n, k, over, _ in gs]
+    for prec, tol in ((FP64, 1e-9), (MIXED, 1e-6)):
+        r = Batch(rg, precision=prec).set_jit(1).tran(1e-12, 3e-10, [("1", 0.0)])
+        assert r.config()[2] is True
+        for gi, (n, k, over, _) in enumerate(gs):
+            st, _ = r.status(gi)
+            assert np.all(st == 0), (gi, st)
+            t, X, names, _ = O.tran(materialize(n, over, k - 1).to_c(), 1e-12, 3e-10, [("1", 0.0)])
+            err = float(np.max(np.abs(r.wave(gi, k - 1)[:, :4] - X[:, :4])))
+            assert err <= tol, (prec, gi, err)
