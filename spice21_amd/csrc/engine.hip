@@ -216,7 +216,7 @@ static void inst_params(const GroupHost& g, const std::vector<const Override*>* 
 }
 
 static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_options* tr, HostPrep& hp,
-                         const AcInput* ac = nullptr) {
+                         const AcInput* ac = nullptr, int hub_contrib = HUB_CONTRIB) {
     Topology& t = g.topo;
     t = Topology();
     build_comps(g, tr, t);
@@ -313,7 +313,7 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
         else
             topo_pivot(t, vals);
     }
-    topo_schedule(t, has_caps);
+    topo_schedule(t, has_caps, hub_contrib);
 
     // Per-instance parameters, SoA [field][instance][class-local device] in the kernel's device order.
     const int n_m1 = t.cls_start[CLS_MOS1T + 1] - t.cls_start[CLS_MOS1];  // both Mos1 classes
@@ -407,9 +407,15 @@ static int auto_ipw(int total_inst, int simds) {
 }
 
 // Derive per-instance parameters, build the topology and upload everything for one analysis.
+// Batches of at most this many instances are latency-bound (one wavefront each at most): every shared
+// slot becomes a hub so that each device class is evaluated in one round instead of one per colour.
+constexpr int LATENCY_INST = 64;
+
 static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr, const AcInput* ac = nullptr) {
     HostPrep hp;
-    host_prepare(b->opts, g, tr, hp, ac);
+    int64_t total = 0;
+    for (const GroupHost& x : b->groups) total += x.n_inst;
+    host_prepare(b->opts, g, tr, hp, ac, total <= LATENCY_INST ? 1 : HUB_CONTRIB);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
 

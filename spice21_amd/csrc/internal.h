@@ -113,7 +113,9 @@ void topo_pivot_sequence(Topology& t, const std::vector<double>& first_values,
 void topo_pivot_forced(Topology& t, const std::vector<int>& row_i2e, const std::vector<int>& col_i2e);
 // The same on the complex matrix of the first AC iteration (Matrix<Complex<f64>>: magnitudes are norms).
 void topo_pivot_cx(Topology& t, const std::vector<double>& re, const std::vector<double>& im);
-void topo_schedule(Topology& t, const std::vector<uint8_t>& mos1_has_caps);
+// hub_contrib: slots with more contributing components are hubs (1: every shared slot, so each device
+// class evaluates in one colour round -- for a latency-bound launch of few instances).
+void topo_schedule(Topology& t, const std::vector<uint8_t>& mos1_has_caps, int hub_contrib = HUB_CONTRIB);
 void topo_records(Topology& t);
 
 // Reference-order OP assembly on the host (analysis.rs:153-168) at voltages x for the pivot search.

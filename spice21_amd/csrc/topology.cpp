@@ -586,7 +586,7 @@ static void pivot_search(Topology& t, const std::vector<T>& first_values, const 
     t.nnz = (int)t.ci.size();
 }
 
-void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
+void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps, int hub_contrib) {
     const int n = t.n;
     // IKJ update lists: for L entry (i,k), subtract l_ik * u_kj from (i,j) for each U entry (k,j).
     // upd_ptr is indexed by LU position; only L positions own updates.
@@ -778,7 +778,7 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
     }
     const int nslot = t.nnz + n;
     if (nslot >= 0x8000) fail(S21_INVALID, "circuit too large for the batched kernel (slot index)");
-    // Hub slots: more than HUB_CONTRIB distinct contributing components ...
+    // Hub slots: more than hub_contrib distinct contributing components ...
     std::vector<int> ncontrib(nslot, 0), lastc(nslot, -1);
     for (int ci = 0; ci < ncomp; ci++)
         for (auto& es : live[ci])
@@ -787,7 +787,7 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps) {
                 ncontrib[es.second]++;
             }
     std::vector<uint8_t> hub(nslot, 0);
-    for (int sl = 0; sl < nslot; sl++) hub[sl] = ncontrib[sl] > HUB_CONTRIB;
+    for (int sl = 0; sl < nslot; sl++) hub[sl] = ncontrib[sl] > hub_contrib;
     // ... and every slot one component stamps twice (the kernel applies a device's stamps as a batch)
     std::fill(lastc.begin(), lastc.end(), -1);
     for (int ci = 0; ci < ncomp; ci++)
