@@ -38,7 +38,8 @@ enum {
     S21_DECODE = 5,       /* "Decode Error: ..."         proto.rs:15-19                          */
     S21_INVALID = 6,      /* malformed argument (null pointer, bad index, size beyond limits)    */
     S21_DEVICE = 7,       /* HIP runtime failure or no gfx950 device                             */
-    S21_NONFINITE = 8     /* a NaN/Inf reached the Newton update (per-instance status only)      */
+    S21_NONFINITE = 8     /* reserved, never reported: a NaN passes the convergence test here as in
+                             the reference (analysis.rs:331-345 compares with >, false for NaN)    */
 };
 
 /* ---- flattened netlist (the output of elaboration, elab.rs:244-265) ----------------------- */

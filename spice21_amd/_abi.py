@@ -18,7 +18,7 @@ STATUS_NAMES = {
     S21_DECODE: "Decode Error",
     S21_INVALID: "invalid argument",
     S21_DEVICE: "HIP device error",
-    S21_NONFINITE: "non-finite Newton update",
+    S21_NONFINITE: "non-finite Newton update (reserved)",
 }
 
 # variable and component kinds (analysis.rs:33-38, comps/mod.rs:60-72)

@@ -918,6 +918,7 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
         GroupHost& g = b->groups[gi];
         std::vector<int> st(g.n_inst);
         HIPCHECK(hipMemcpy(st.data(), g.gg.status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+        if (g.topo.sym_status != S21_OK) continue;  // the search itself failed: nothing to resume on
         for (int i = 0; i < g.n_inst; i++)
             if (st[i] == S21_PIVOT) todo.push_back(stalled_from(g, (int)gi, i, b->precision != S21_MIXED));
     }

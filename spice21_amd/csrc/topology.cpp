@@ -603,7 +603,12 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps, int hub_con
             const int k = t.ci[q];
             for (int u = t.diag[k] + 1; u < t.rp[k + 1]; u++) {
                 int dst = posmap[t.ci[u]];
-                if (dst < 0) fail(S21_ERR, "internal: fill pattern not closed");
+                if (dst < 0) {
+                    // the unpivoted pattern of a failed search has no fill; it is never factored (the
+                    // kernel reports the search's failure when an instance needs a solve)
+                    if (t.sym_status != S21_OK) continue;
+                    fail(S21_ERR, "internal: fill pattern not closed");
+                }
                 t.upd_src.push_back(u);
                 t.upd_dst.push_back(dst);
             }

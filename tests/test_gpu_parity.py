@@ -459,3 +459,41 @@ def test_generated_factorisation_several_patterns():
             t, X, names, _ = O.tran(materialize(n, over, k - 1).to_c(), 1e-12, 3e-10, [("1", 0.0)])
             err = float(np.max(np.abs(r.wave(gi, k - 1)[:, :4] - X[:, :4])))
             assert err <= tol, (prec, gi, err)
+
+
+def _floating_cap():
+    n = W.Netlist()
+    n.v("v1", 1.0, "in", "")
+    n.r(1e-3, "in", "a")
+    n.c(1e-12, "a", "b")  # b has no DC path: its matrix entries exist but are zero in the operating point
+    return n
+
+
+def _nan_resistor():
+    n = W.Netlist()
+    n.v("v1", 1.0, "in", "")
+    n.r(float("nan"), "in", "a")
+    n.r(1e-3, "a", "")
+    return n
+
+
+def test_failure_outcomes_match_oracle_and_stay_per_instance():
+    """Failures the reference reports the engine reports per instance, leaving the rest of the batch
+    alone: a node reached only through a capacitor meets a zero pivot in the operating point (the
+    oracle's "Assert Neq Failed: zero pivot", sparse21/mod.rs:872 -> S21_PIVOT); a NaN conductance makes
+    every unknown NaN yet passes the convergence test, whose > comparisons are false for NaN, in the
+    oracle and the engine alike (analysis.rs:331-345)."""
+    fl, nn = _floating_cap().to_c(), _nan_resistor().to_c()
+    with pytest.raises(O.OracleError) as e:
+        O.dcop(fl)
+    assert e.value.code == 4
+    ox, oit, _ = O.dcop(nn)
+    assert np.all(np.isnan(ox))
+    chain = W.chain(64)[0].to_c()
+    cx, cit, _ = O.dcop(chain)
+    b = Batch([Group(fl), Group(chain), Group(nn)]).op()
+    assert b.status(0)[0][0] == 4
+    st, it = b.status(1)
+    assert st[0] == 0 and it[0] == cit and rel_err(b.x(1)[0], cx) < 1e-9
+    st, it = b.status(2)
+    assert st[0] == 0 and it[0] == oit and np.all(np.isnan(b.x(2)[0]))
