@@ -225,6 +225,15 @@ S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_
 S21_API int s21_batch_set_ipw(s21_batch* b, int32_t ipw);
 S21_API int s21_batch_set_jit(s21_batch* b, int32_t mode);
 S21_API int s21_batch_config(const s21_batch* b, int32_t* ipw, int32_t* waves, int32_t* jit);
+/* AC sweep shape (no counterpart in the reference, which sweeps one instance's frequencies in sequence,
+ * each solve starting from the previous frequency's solution, analysis.rs:797-818): each instance's
+ * frequencies are cut into `runs` contiguous runs solved in parallel, each run's first frequency
+ * started from x = 0 as the reference starts the sweep's first.  The small-signal system does not
+ * depend on x, so every frequency converges to the same solution up to rounding; only the Newton
+ * iteration counts of the runs' first frequencies differ.  1 = the reference's sequence of starts;
+ * 0 (default) = chosen per sweep to fill the device.  s21_batch_last_ac_runs reports the last sweep's. */
+S21_API int s21_batch_set_ac_runs(s21_batch* b, int32_t runs);
+S21_API int32_t s21_batch_last_ac_runs(const s21_batch* b);
 /* Per-phase device cycle counters (s_memtime) of the Newton kernel, for profiling: while on, each
  * instance accumulates [device evaluation, hub gather, residual, factor (+ forward sweep), backward
  * sweep, update, convergence/commit, total] cycles. */

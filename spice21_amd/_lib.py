@@ -67,6 +67,8 @@ def lib():
     _sig(L, "s21_jit_build", C.c_int, C.POINTER(A.s21_netlist), C.POINTER(A.s21_options),
          C.POINTER(A.s21_tran_options), i32, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
     _sig(L, "s21_batch_config", C.c_int, vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32))
+    _sig(L, "s21_batch_set_ac_runs", C.c_int, vp, i32)
+    _sig(L, "s21_batch_last_ac_runs", i32, vp)
     _sig(L, "s21_batch_profile", C.c_int, vp, i32, P(i64))
     _sig(L, "s21_batch_structure", C.c_int, vp, i32, P(i64), P(i64), P(i64), P(i64))
     _sig(L, "s21_batch_assemble", C.c_int, vp, i32, i32, P(d), P(i32), P(i32), P(d), P(d), P(i64))

@@ -183,6 +183,16 @@ class Batch:
         check(lib().s21_batch_set_jit(self.h, int(mode)))
         return self
 
+    def set_ac_runs(self, runs):
+        """AC: each instance's frequencies cut into this many contiguous runs solved in parallel, each
+        started from x = 0 (1 = the reference's sequence of warm starts; 0 = chosen per sweep)."""
+        check(lib().s21_batch_set_ac_runs(self.h, int(runs)))
+        return self
+
+    def ac_runs(self):
+        """Runs of frequencies per instance of the last AC sweep."""
+        return int(lib().s21_batch_last_ac_runs(self.h))
+
     def autotune(self, run, candidates=(1, 2, 4), repeats=2):
         """Times ``run(self)`` (an analysis call) at each instances-per-wavefront candidate and keeps the
         fastest; returns {ipw: kernel milliseconds}.  Each candidate runs once untimed first, which also

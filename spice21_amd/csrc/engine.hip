@@ -67,6 +67,7 @@ struct GroupHost {
     Topology topo;
     std::unique_ptr<DevMem> d_topo, d_par, d_state, d_out;
     std::unique_ptr<DevMem> d_acop;  // Mos1 small-signal operating points captured for AC
+    std::unique_ptr<DevMem> d_ac;    // AC: per-frequency records, runs' last iterates, runs' outcomes
     GroupGPU gg{};
     std::vector<double> x;
     std::vector<int> status;
@@ -95,6 +96,8 @@ struct s21_batch {
     int last_waves = 1, last_ipw = 1;
     int ipw = 0;  // instances per wavefront; 0 = chosen per launch
     int jit = 0;  // generated factorisation: 0 = for large single-pattern batches, 1 = always, -1 = never
+    int ac_runs = 0;  // AC: runs of frequencies per instance; 0 = chosen per sweep
+    int last_ac_runs = 0;
     bool last_jit = false;
     // kernel with every group's factorisation written out (generated at the last preparation), or none
     s21::JitKernel jitk;
@@ -554,6 +557,9 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     const size_t vw = ac ? 2 : 1;  // values per entry: complex for AC
     g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) * vw +
                    (b->precision == S21_MIXED ? (size_t)(t.nnz + t.n) * sizeof(float) * vw : 0);
+    if (ac)  // the instance's AC parameters, cached in LDS by the kernel
+        g.inst_bytes += (size_t)(M1_CORE_F * (t.cls_start[CLS_MOS1T + 1] - t.cls_start[CLS_MOS1]) +
+                                 (t.cls_start[CLS_LIN + 1] - t.cls_start[CLS_LIN])) * sizeof(double);
     g.gg.acop = nullptr;
 }
 
@@ -655,7 +661,7 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
         stride = inst_stride(raw, ipw, jit);
         auto workgroups = [&](int k) {
             long n = 0;
-            for (const GroupHost& g : b->groups) n += (g.n_inst + k * ipw - 1) / (k * ipw);
+            for (const GroupHost& g : b->groups) n += ((long)g.n_inst * ac_units(A) + k * ipw - 1) / (k * ipw);
             return n;
         };
         const long target = std::min(cus, workgroups(1));
@@ -688,7 +694,7 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
         hg.push_back(gg);
         blob = std::max(blob, g.blob_bytes);
         raw = std::max(raw, g.inst_bytes);
-        total += g.n_inst;
+        total += g.n_inst * ac_units(A0);
     }
     int dev = 0, max_block = 0, cus = 0;
     HIPCHECK(hipGetDevice(&dev));
@@ -703,7 +709,7 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     int blk = 0;
     for (GroupGPU& gg : hg) {
         gg.blk0 = blk;
-        blk += (gg.n_inst + per_block - 1) / per_block;
+        blk += (gg.n_inst * ac_units(A) + per_block - 1) / per_block;
     }
     if (!b->d_groups) b->d_groups.reset(new DevMem());
     if (b->d_groups->bytes < hg.size() * sizeof(GroupGPU)) b->d_groups->alloc(hg.size() * sizeof(GroupGPU));
@@ -766,6 +772,8 @@ static Args base_args(const s21_batch* b, int mode) {
     A.resume = 0;
     A.asm_tran = 0;
     A.omega = nullptr;
+    A.ac_runs = 1;
+    A.ac_pass = 0;
     return A;
 }
 
@@ -1245,6 +1253,35 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
             g.prepared = "ac";
         }
         b->has_jit = false;  // the frequency loop runs the precompiled complex kernel
+
+        // Runs of frequencies.  The small-signal system does not depend on x (every load_ac stamp comes
+        // from the captured operating point), so a frequency whose Newton loop takes at least one full
+        // step ends at that frequency's unique solution (up to rounding) whatever its start.  The warm
+        // start matters only where the reference's test passes at once: a frequency whose residual at
+        // the previous solution is already below 1e-9 keeps that solution.  Each instance's sweep is cut
+        // into contiguous runs, solved in parallel from x = 0 (first pass); each run is then re-solved
+        // from the previous run's end until both paths take a full step at the same frequency, after
+        // which they agree (second pass); a run whose re-solve never meets its first pass is continued
+        // in order (third pass).  The result is the sequential sweep's, and so are the iteration counts.
+        int runs = b->ac_runs;
+        if (runs <= 0) {  // about eight wavefronts per CU in the first pass
+            int dev = 0, cus = 0;
+            HIPCHECK(hipGetDevice(&dev));
+            HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            int64_t inst = 0;
+            for (const GroupHost& g : b->groups) inst += g.n_inst;
+            runs = (int)std::min<int64_t>((32 * (int64_t)std::max(cus, 1) + inst - 1) / inst, std::max<int64_t>(F / 16, 1));
+        }
+        runs = (int)std::max<int64_t>(std::min<int64_t>(runs, F), 1);
+        for (GroupHost& g : b->groups) {
+            const size_t ni = (size_t)g.n_inst, n2 = 2 * (size_t)g.topo.n;
+            if (!g.d_ac) g.d_ac.reset(new DevMem());
+            g.d_ac->alloc(ni * runs * n2 * sizeof(double) + ni * (size_t)F * sizeof(int) + ni * runs * sizeof(int));
+            g.gg.ac_end = (double*)g.d_ac->p;
+            g.gg.ac_rec = (int*)(g.gg.ac_end + ni * runs * n2);
+            g.gg.ac_state = g.gg.ac_rec + ni * (size_t)F;
+            HIPCHECK(hipMemset(g.gg.ac_rec, 0, ni * (size_t)F * sizeof(int)));
+        }
         std::vector<double> omega(F);
         for (int64_t p = 0; p < F; p++) omega[p] = 2.0 * M_PI * freqs[p];  // AcState::omega, analysis.rs:801
         DevMem d_omega;
@@ -1257,7 +1294,62 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
         Args B = base_args(b, MODE_AC);
         B.npts = F;
         B.omega = (const double*)d_omega.p;
-        run(b, B, (double*)b->d_wave->p, 2 * F);
+        B.ac_runs = runs;
+        double ms = 0.0;
+        int launches = 0;
+        auto pass = [&](int k) {
+            B.ac_pass = k;
+            run(b, B, (double*)b->d_wave->p, 2 * F);
+            ms += b->last_ms;
+            launches++;
+        };
+        pass(0);
+        if (runs > 1) {
+            pass(1);
+            // Third pass from the first run (in frequency order) that neither met its first pass nor failed.
+            bool third = false;
+            for (GroupHost& g : b->groups) {
+                std::vector<int> state((size_t)g.n_inst * runs);
+                HIPCHECK(hipMemcpy(state.data(), g.gg.ac_state, state.size() * sizeof(int), hipMemcpyDeviceToHost));
+                for (int i = 0; i < g.n_inst; i++) {
+                    int* st = state.data() + (size_t)i * runs;
+                    int from = 0;
+                    for (int r = 1; r < runs; r++) {
+                        if (st[r] == AC_MET) continue;
+                        if (st[r] == AC_UNMET && r + 1 < runs) from = r + 1;  // run r is the sweep's own
+                        break;
+                    }
+                    st[0] = from;
+                    third = third || from > 0;
+                }
+                HIPCHECK(hipMemcpy(g.gg.ac_state, state.data(), state.size() * sizeof(int), hipMemcpyHostToDevice));
+            }
+            if (third) pass(2);
+        }
+        b->last_ms = ms;
+        b->kernel_avg_ms = ms / launches;
+        b->launches = launches;
+        // Each instance's status and iteration count, from its frequencies in order: the sweep stops at
+        // the first that fails (analysis.rs:801).
+        for (GroupHost& g : b->groups) {
+            std::vector<int> rec((size_t)g.n_inst * F);
+            HIPCHECK(hipMemcpy(rec.data(), g.gg.ac_rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost));
+            std::vector<long long> it1(g.n_inst, 0);
+            std::vector<int> st1(g.n_inst, S21_OK);
+            for (int i = 0; i < g.n_inst; i++)
+                for (int64_t p = 0; p < F; p++) {
+                    const int r = rec[(size_t)i * F + p];
+                    if (r == 0) fail(S21_ERR, "internal: AC frequency left unsolved");
+                    it1[i] += r & 0xFF;
+                    if ((r >> 8) != S21_OK) {
+                        st1[i] = r >> 8;
+                        break;
+                    }
+                }
+            HIPCHECK(hipMemcpy(g.gg.status, st1.data(), g.n_inst * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(g.gg.iters, it1.data(), g.n_inst * sizeof(long long), hipMemcpyHostToDevice));
+        }
+        b->last_ac_runs = runs;
         b->fetched = false;
         b->npts = F;
         b->axis = freqs;
@@ -1353,6 +1445,14 @@ int s21_batch_set_jit(s21_batch* b, int32_t mode) {
         b->jit = mode;
     });
 }
+int s21_batch_set_ac_runs(s21_batch* b, int32_t runs) {
+    return guard([&] {
+        if (!b) fail(S21_INVALID, "null batch");
+        if (runs < 0) fail(S21_INVALID, "runs of frequencies must be 0 (chosen per sweep) or positive");
+        b->ac_runs = runs;
+    });
+}
+int32_t s21_batch_last_ac_runs(const s21_batch* b) { return b ? b->last_ac_runs : 0; }
 int s21_batch_config(const s21_batch* b, int32_t* ipw, int32_t* waves, int32_t* jit) {
     return guard([&] {
         if (!b) fail(S21_INVALID, "null batch");

@@ -77,6 +77,10 @@ struct GroupGPU {
     long long* prof;        // [n_inst][8] phase cycle counters (profiling mode) or nullptr
     int* resume_at;         // [n_inst][2]: time point of the last solve and the iterations it had used
                             // before its current one (written at exit; read by a resuming launch)
+    int* ac_rec;            // AC: [n_inst][npts] per frequency: status << 8 | Newton iterations (0 = not solved)
+    double* ac_end;         // AC: [n_inst][ac_runs][n] complex: each run's last iterate (first pass)
+    int* ac_state;          // AC: [n_inst][ac_runs] outcome of each run's re-solve from the previous
+                            // run's end (second pass); entry 0: run to stitch from in order (third pass)
 };
 
 enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2, MODE_AC = 3 };
@@ -98,7 +102,17 @@ struct Args {
                      // test fails (the interrupted iteration had passed that test already)
     int asm_tran;    // assemble mode: the transient matrix (companions from the state arrays)
     const double* omega;  // AC: angular frequency of each point
+    int ac_runs;          // AC: each instance's frequencies split into this many contiguous runs (1 outside AC)
+    int ac_pass;          // AC: 0 = every run from x = 0; 1 = each run re-solved from the previous run's
+                          // end until it meets the first pass; 2 = in order from G.ac_state's run
 };
+
+// AC launch units per instance: one per run, one per run boundary, or one.
+__host__ __device__ inline int ac_units(const Args& A) {
+    if (A.mode != MODE_AC) return 1;
+    return A.ac_pass == 0 ? A.ac_runs : A.ac_pass == 1 ? A.ac_runs - 1 : 1;
+}
+enum { AC_UNMET = 0, AC_MET = 1, AC_FAILED = 2 };
 
 // Address spaces: pointers read out of descriptors carry no address space, so they are re-qualified in
 // device code to keep LDS accesses on ds_* and global accesses on global_* instructions.
@@ -748,6 +762,8 @@ __device__ inline zd zwide(V a) { return zd{(double)a.re, (double)a.im}; }
 struct AcCtx {
     LDS double *x, *c, *a, *rhs, *hub, *part;
     LDS float* lu32;
+    LDS double* par;  // the instance's AC parameters: M1_CORE_F fields of each Mos1, then each linear value
+    int nm;           // Mos1 devices
 };
 template <typename R>
 __device__ inline Z<R> ldz(LDS const R* p, int i) {
@@ -798,9 +814,7 @@ struct AcM1Stamps {
 
 __device__ inline void eval_ac_mos1(const Ctx& C, const AcCtx& Z_, int j, LDS const u16* code, double omega,
                                     bool tied) {
-    const GroupGPU& G = *C.G;
-    GLB const double* par = glb(G.par);
-    auto F = [&](int f) { return par[G.m1_off[f] + (long long)C.inst * G.m1_istr[f] + j]; };
+    auto F = [&](int f) { return Z_.par[f * Z_.nm + j]; };
     const Mos1AcOp op{F(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7), F(8)};
     AcM1Stamps sv;
 #pragma unroll
@@ -832,9 +846,8 @@ __device__ inline void eval_ac_mos1(const Ctx& C, const AcCtx& Z_, int j, LDS co
 
 __device__ inline void eval_ac_lin(const Ctx& C, const AcCtx& Z_, int j, LDS const u16* dv, LDS const u16* code,
                                    double omega) {
-    const GroupGPU& G = *C.G;
     const int typ = dv[6] >> 4;
-    const double val = glb(G.par)[G.lin_off[2] + (long long)C.inst * G.lin_istr[2] + j];
+    const double val = Z_.par[M1_CORE_F * Z_.nm + j];
     double r[6] = {0, 0, 0, 0, 0, 0}, i[6] = {0, 0, 0, 0, 0, 0};
     if (typ == S21_COMP_R) {  // comps/mod.rs:311-322
         r[0] = val;
@@ -1155,8 +1168,9 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
         inst = A.dbg_inst;
         valid = slot == 0;
     } else {
-        inst = (b - G.blk0) * A.waves * A.ipw + slot;
-        valid = inst < G.n_inst;
+        inst = (b - G.blk0) * A.waves * A.ipw + slot;  // AC: the run's index, instance-major
+        valid = inst < G.n_inst * ac_units(A);
+        if (A.mode == MODE_AC) inst /= ac_units(A);
     }
     // A workgroup-synchronous solve needs every wavefront at every barrier, instance or not.
     if (!LU::lane_per_instance && !wave_any(valid)) return;
@@ -1192,25 +1206,131 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
         Z_.hub = Z_.rhs + 2 * T.n;
         Z_.part = Z_.hub + 2 * T.n_hub;
         Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
-        C.x = Z_.x;
-        for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});  // analysis.rs:215-233
-        wave_sync();
-        long long iters = 0;
-        bool alive = valid;
-        int st = S21_OK;
-        for (long long pt = 0; pt < A.npts; pt++) {
-            if (!wave_any(alive)) break;
-            const int s2 = newton_ac<MIXED>(C, Z_, alive, A.omega[pt], iters);
-            if (alive) {
-                st = s2;
-                alive = s2 == S21_OK;
+        Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
+        Z_.nm = mos1_count(T);
+        {  // parameters read every iteration: LDS instead of global-memory latency per colour
+            const int nl = T.n_cls[CLS_LIN];
+            GLB const double* par = glb(G.par);
+            for (int q = valid ? C.sl : M1_CORE_F * Z_.nm; q < M1_CORE_F * Z_.nm; q += C.L) {
+                const int f = q / Z_.nm, j = q - f * Z_.nm;
+                Z_.par[q] = par[G.m1_off[f] + (long long)inst * G.m1_istr[f] + j];
             }
-            zd* wv = (zd*)G.wave + ((size_t)inst * (size_t)A.npts + pt) * T.n;
-            for (int k = alive ? C.sl : T.n; k < T.n; k += C.L) wv[k] = ldz(Z_.x, k);
+            for (int j = valid ? C.sl : nl; j < nl; j += C.L)
+                Z_.par[M1_CORE_F * Z_.nm + j] = par[G.lin_off[2] + (long long)inst * G.lin_istr[2] + j];
+            wave_sync();
         }
-        if (valid && C.sl == 0) {
-            G.status[inst] = st;
-            G.iters[inst] = iters;
+        C.x = Z_.x;
+        const int R = A.ac_runs;
+        const long long F = A.npts;
+        const int u = valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0;
+        auto first_pt = [&](int r) { return F * r / R; };
+        GLB int* rec = glb(G.ac_rec) + (size_t)inst * F;
+        GLB double* wv = glb(G.wave) + (size_t)inst * F * 2 * T.n;
+        GLB double* ends = glb(G.ac_end) + (size_t)inst * R * 2 * T.n;
+        auto load_x = [&](bool act, GLB const double* src) {
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{src[2 * k], src[2 * k + 1]});
+            wave_sync();
+        };
+        auto store_x = [&](bool act, GLB double* dst) {
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+                const zd v = ldz(Z_.x, k);
+                dst[2 * k] = v.re;
+                dst[2 * k + 1] = v.im;
+            }
+        };
+        long long iters = 0;
+        // Newton at frequency pt from the current x; records the point and returns its status.
+        auto point = [&](bool act, long long pt, int& its) {
+            const long long i0 = iters;
+            const int s2 = newton_ac<MIXED>(C, Z_, act, act ? A.omega[pt] : 0.0, iters);
+            its = (int)(iters - i0);
+            if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
+            if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
+            return s2;
+        };
+        // Any frequency of [p, e) failed or unsolved (the lanes of the segment share the scan).
+        auto failed_in = [&](long long p, long long e) {
+            bool f = false;
+            for (long long q = p + C.sl; q < e; q += C.L) f = f || rec[q] == 0 || (rec[q] >> 8) != S21_OK;
+            return seg_any(C, f);
+        };
+        // A frequency solved (at least one full Newton step) on both paths: both hold its unique solution.
+        auto meets = [&](int its, int old) { return its >= 2 && old != 0 && (old >> 8) == S21_OK && (old & 0xFF) >= 2; };
+        if (A.ac_pass == 0) {  // run u from x = 0 (analysis.rs:215-233)
+            bool alive = valid;
+            for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});
+            wave_sync();
+            const long long p0 = first_pt(u), p1 = first_pt(u + 1);
+            for (long long pt = p0; pt < p1; pt++) {
+                if (!wave_any(alive)) break;
+                int its;
+                const int s2 = point(alive, pt, its);
+                if (alive) alive = s2 == S21_OK;
+            }
+            store_x(valid, ends + (size_t)u * 2 * T.n);
+        } else if (A.ac_pass == 1) {  // run r from the first pass's end of run r - 1, until the paths meet
+            const int r = u + 1;
+            bool live = valid;
+            load_x(live, ends + (size_t)(r - 1) * 2 * T.n);
+            int state = AC_UNMET;
+            const long long p0 = first_pt(r), p1 = first_pt(r + 1);
+            for (long long pt = p0; pt < p1; pt++) {
+                if (!wave_any(live)) break;
+                const int old = live ? rec[pt] : 0;
+                int its;
+                const int s2 = point(live, pt, its);
+                if (live && s2 != S21_OK) {
+                    state = AC_FAILED;
+                    live = false;
+                } else if (live && meets(its, old)) {
+                    state = AC_MET;
+                    live = false;
+                }
+            }
+            if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
+        } else {  // in order from run rs, jumping to the next run wherever the paths meet
+            const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
+            bool live = valid && rs > 0;
+            long long k = live ? first_pt(rs) : 1;
+            load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
+            while (wave_any(live)) {
+                const int old = live ? rec[k] : 0;
+                int its;
+                const int s2 = point(live, k, its);
+                if (!live) continue;
+                long long next = k + 1;
+                if (s2 != S21_OK) {
+                    live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
+                } else if (meets(its, old)) {
+                    int rr = (int)((k * R) / F);
+                    while (first_pt(rr + 1) <= k) rr++;
+                    while (first_pt(rr) > k) rr--;
+                    // The rest of run rr is stored data of a path through frequency k's solution: the sweep's
+                    // (a failure there is the sweep's).
+                    bool bad = failed_in(k + 1, first_pt(rr + 1));
+                    int q = rr + 1;
+                    // Run rr ends as the first pass ended it unless its re-solve never met that pass; then
+                    // every following run whose re-solve met is the sweep's, and so is a run whose re-solve
+                    // ran to its end (its start was the sweep's) or failed (the sweep fails there).
+                    if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
+                        while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
+                            bad = failed_in(first_pt(q), first_pt(q + 1));
+                            q++;
+                        }
+                        if (!bad && q < R) {
+                            if (G.ac_state[(size_t)inst * R + q] == AC_FAILED)
+                                bad = true;
+                            else
+                                q++;
+                        }
+                    }
+                    next = first_pt(q);
+                    if (bad) live = false;
+                    if (live && next < F) load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
+                }
+                if (next >= F) live = false;
+                k = next;
+            }
         }
         return;
     }

@@ -193,6 +193,44 @@ def test_ac_batch_matches_oracle_per_instance():
         assert err < 1e-9, (i, err)
 
 
+def test_ac_runs_of_frequencies_match_one_sequential_sweep():
+    """Each instance's sweep cut into runs solved in parallel and stitched: the same solutions and the
+    same Newton iteration counts as one sequential sweep, which repeats the reference's warm starts (its
+    counts equal the oracle's).  Without the stitching the runs' cold starts differ by up to 6e-4
+    relative on this sweep: a frequency whose residual at the previous solution is below 1e-9 keeps that
+    solution in the reference, so its result depends on where the sweep came from."""
+    n_inst = 12
+    nl, over = W.amp_ladder(n_inst)
+    g = Group(nl.to_c(), n_inst, over)
+    log("ac runs: sequential")
+    seq = Batch(g).set_ac_runs(1).ac(1000, 100_000_000_000, 300)
+    assert seq.ac_runs() == 1
+    st, it = seq.status()
+    assert np.all(st == 0), st
+    for i in [0, 7]:
+        ci = materialize(nl, over, i).to_c()
+        fo, Xo, ito = O.ac(ci, 1000, 100_000_000_000, 300, max_pts=400)
+        ito -= O.dcop(ci)[1]  # the oracle counts the operating point's iterations too
+        assert it[i] == ito, (i, it[i], ito)
+        err = np.max(np.abs(seq.wave(0, i) - Xo) / np.maximum(np.abs(Xo), 1e-9))
+        assert err < 1e-9, (i, err)
+    nf = len(seq.axis())
+    for runs in (0, 7, nf + 5):
+        log("ac runs:", runs)
+        par = Batch(g).set_ac_runs(runs).ac(1000, 100_000_000_000, 300)
+        assert par.ac_runs() == (min(runs, nf) if runs else par.ac_runs()) and par.ac_runs() > 1
+        st2, it2 = par.status()
+        assert np.all(st2 == 0), st2
+        log("  iterations", int(it2.sum()), "sequential", int(it.sum()))
+        assert np.array_equal(it2, it)
+        worst = 0.0
+        for i in range(n_inst):
+            a = seq.wave(0, i)
+            worst = max(worst, float(np.max(np.abs(par.wave(0, i) - a) / np.maximum(np.abs(a), 1e-9))))
+        log("  max rel vs sequential", worst)
+        assert worst < 1e-9, (runs, worst)
+
+
 def test_ac_mixed_close_to_fp64():
     n_inst = 16
     nl, over = W.amp_ladder(n_inst)
