@@ -211,6 +211,10 @@ S21_API int s21_symbolic(const s21_netlist* net, const s21_options* opts, const 
  * for gfx950 without loading it; reports its statements and code-object size. */
 S21_API int s21_jit_build(const s21_netlist* net, const s21_options* opts, const s21_tran_options* tran,
                           int32_t precision, int64_t* n_ops, int64_t* code_bytes);
+/* Host only (no GPU): the AC kernel's generated complex factorisation, for the pattern searched on the
+ * matrix at fstart with every Mos1 at a zero small-signal operating point; compiled, not loaded. */
+S21_API int s21_jit_build_ac(const s21_netlist* net, const s21_options* opts, double fstart, int32_t precision,
+                             int64_t* n_ops, int64_t* code_bytes);
 /* Pivot order chosen by the symbolic phase of `group` for its last analysis: internal-to-
  * external row and column maps (AxisMapping.i2e, sparse21/mod.rs:126-162), each [n_vars]. */
 S21_API int s21_batch_pivots(const s21_batch* b, int32_t group, int32_t* row_i2e, int32_t* col_i2e);
@@ -219,9 +223,10 @@ S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_
                         int64_t* levels, int64_t* n_contrib);
 /* Launch shape (no counterpart in the reference, whose solver is single-threaded): instances per
  * wavefront (1, 2, 4, 8 or 16; 0 = chosen per launch from the batch size), and whether the kernel
- * runs a factorisation generated and compiled at run time for the group's frozen pattern (1 = always,
- * -1 = never, 0 = for a single-pattern batch of at least 512 instances).  s21_batch_config reports the
- * last launch: instances per wavefront, wavefronts per workgroup, generated factorisation used. */
+ * runs a factorisation generated and compiled at run time for the groups' frozen patterns (1 = always,
+ * -1 = never, 0 = for a batch of at least 512 instances, or an AC sweep of at least 100,000
+ * instance-frequencies).  s21_batch_config reports the last launch: instances per wavefront,
+ * wavefronts per workgroup, generated factorisation used. */
 S21_API int s21_batch_set_ipw(s21_batch* b, int32_t ipw);
 S21_API int s21_batch_set_jit(s21_batch* b, int32_t mode);
 S21_API int s21_batch_config(const s21_batch* b, int32_t* ipw, int32_t* waves, int32_t* jit);

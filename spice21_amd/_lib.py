@@ -66,6 +66,8 @@ def lib():
     _sig(L, "s21_batch_set_jit", C.c_int, vp, i32)
     _sig(L, "s21_jit_build", C.c_int, C.POINTER(A.s21_netlist), C.POINTER(A.s21_options),
          C.POINTER(A.s21_tran_options), i32, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+    _sig(L, "s21_jit_build_ac", C.c_int, C.POINTER(A.s21_netlist), C.POINTER(A.s21_options), d, i32,
+         C.POINTER(C.c_int64), C.POINTER(C.c_int64))
     _sig(L, "s21_batch_config", C.c_int, vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32))
     _sig(L, "s21_batch_set_ac_runs", C.c_int, vp, i32)
     _sig(L, "s21_batch_last_ac_runs", i32, vp)

@@ -27,6 +27,7 @@ namespace s21 {
     } while (0)
 
 constexpr int JIT_MIN_INST = 512;      // instances that repay a run-time compilation
+constexpr int64_t JIT_MIN_AC_POINTS = 100000;  // AC: instance-frequencies that repay one
 constexpr int64_t JIT_MAX_OPS = 40000;         // largest generated factorisation of one pattern
 constexpr int64_t JIT_MAX_TOTAL_OPS = 200000;  // ... and of all the patterns of one batch
 
@@ -574,11 +575,11 @@ static std::string prep_key(const s21_tran_options* tr) {
 // Prepare every group for an analysis, or only reset the Newton state when the structure is current.
 // Generated factorisation: for a batch with enough instances to repay the compilation (or when asked),
 // one kernel with every group's pattern written out -- groups of one pattern share its functions.
-static void prepare_jit(s21_batch* b) {
+static void prepare_jit(s21_batch* b, bool ac = false, int64_t n_freq = 0) {
     b->has_jit = false;
     int64_t total = 0, ops = 0;
     for (const GroupHost& g : b->groups) total += g.n_inst;
-    if (b->jit < 0 || (b->jit == 0 && total < JIT_MIN_INST)) return;
+    if (b->jit < 0 || (b->jit == 0 && (ac ? total * n_freq < JIT_MIN_AC_POINTS : total < JIT_MIN_INST))) return;
     std::vector<const Topology*> pats;
     std::vector<int> group_pattern;
     std::unordered_map<std::string, int> seen;  // pattern -> index, keyed by what its code is written from
@@ -599,7 +600,7 @@ static void prepare_jit(s21_batch* b) {
         group_pattern.push_back(it->second);
     }
     if (ops > JIT_MAX_TOTAL_OPS) return;
-    b->jitk = jit_compile(jit_source(pats, group_pattern, b->precision == S21_MIXED, 2));
+    b->jitk = jit_compile(jit_source(pats, group_pattern, b->precision == S21_MIXED, 2, ac));
     b->has_jit = true;
 }
 
@@ -1128,6 +1129,34 @@ int s21_jit_build(const s21_netlist* net, const s21_options* opts, const s21_tra
     });
 }
 
+int s21_jit_build_ac(const s21_netlist* net, const s21_options* opts, double fstart, int32_t precision,
+                     int64_t* n_ops, int64_t* code_bytes) {
+    return guard([&] {
+        if (!net) fail(S21_INVALID, "null netlist");
+        GroupHost g;
+        s21_group sg{net, 1, 0, nullptr};
+        group_from(g, sg);
+        s21_options o;
+        if (opts)
+            o = *opts;
+        else
+            s21_default_options(&o);
+        AcInput in;
+        in.omega0 = 2.0 * M_PI * fstart;
+        for (size_t i = 0; i < g.comps.size(); i++)
+            if (g.comps[i].type == S21_COMP_MOS1) in.dc_comp.push_back((int)i);
+        in.acop.assign((size_t)M1_ACOP_F * in.dc_comp.size(), 0.0);
+        HostPrep hp;
+        host_prepare(o, g, nullptr, hp, &in);
+        const Topology& t = g.topo;
+        if (t.sym_status != S21_OK) fail(t.sym_status, t.sym_error);
+        if (n_ops) *n_ops = jit_ops(t);
+        const std::vector<char> code =
+            jit_code(jit_source(std::vector<const Topology*>{&t}, std::vector<int>{0}, precision == S21_MIXED, 2, true));
+        if (code_bytes) *code_bytes = (int64_t)code.size();
+    });
+}
+
 int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_options* opts, int32_t precision,
                      s21_batch** out) {
     return guard([&] {
@@ -1252,7 +1281,7 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
             prepare_group(b, g, nullptr, &acin[gi]);
             g.prepared = "ac";
         }
-        b->has_jit = false;  // the frequency loop runs the precompiled complex kernel
+        prepare_jit(b, true, F);  // the complex factorisation written out, or the precompiled kernel
 
         // Runs of frequencies.  The small-signal system does not depend on x (every load_ac stamp comes
         // from the captured operating point), so a frequency whose Newton loop takes at least one full

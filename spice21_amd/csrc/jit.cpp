@@ -229,17 +229,55 @@ void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, 
         o << emit_steps(steps) << "}\n";
     }
 }
+// The complex factorisation and sweeps of an AC pattern, straight-line on the interleaved (re, im)
+// arrays, with the record-driven code's operations (factor_forward_ac / backward_ac in kernel.h): the
+// num-complex naive division, c_k skipped when exactly zero.  R is the component type (double, or float
+// for the mixed-precision copies, whose division multiplies by the norm's reciprocal).
+void gen_cx(std::ostringstream& o, const Topology& t, const std::string& fn, const std::string& sfx) {
+    o << "template <typename R>\n" << fn << " bool lu_gen" << sfx
+      << "(LDS R* __restrict__ lu, LDS R* __restrict__ c) {\n"
+         "    using V = Z<R>;\n"
+         "    V y, l, ck;\n";
+    for (int i = 0; i < t.n; i++) {
+        if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;  // no L entries
+        o << "    y = ldz(c, " << i << ");\n";
+        for (int q = t.rp[i]; q < t.diag[i]; q++) {
+            const int k = t.ci[q];
+            o << "    l = zdiv(ldz(lu, " << q << "), ldz(lu, " << t.diag[k] << ")); stz(lu, " << q << ", l); ck = ldz(c, "
+              << k << "); if (!zzero(ck)) y = y - ck * l;\n";
+            for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++)
+                o << "    stz(lu, " << t.upd_dst[x] << ", ldz(lu, " << t.upd_dst[x] << ") - ldz(lu, " << t.upd_src[x]
+                  << ") * l);\n";
+        }
+        o << "    stz(c, " << i << ", y);\n";
+    }
+    o << "    bool z = false;\n";
+    for (int i = 0; i + 1 < t.n; i++) o << "    z = z || zzero(ldz(lu, " << t.diag[i] << "));\n";
+    o << "    return z;\n}\n";
+    o << "template <typename R>\n" << fn << " void bw_gen" << sfx
+      << "(LDS const R* __restrict__ lu, LDS R* __restrict__ c) {\n"
+         "    using V = Z<R>;\n"
+         "    V y;\n";
+    for (int i = t.n - 1; i >= 0; i--) {
+        o << "    y = ldz(c, " << i << ");";
+        for (int q = t.diag[i] + 1; q < t.rp[i + 1]; q++) o << " y = y - ldz(c, " << t.ci[q] << ") * ldz(lu, " << q << ");";
+        o << " stz(c, " << i << ", zdiv(y, ldz(lu, " << t.diag[i] << ")));\n";
+    }
+    o << "}\n";
+}
 }  // namespace
 
 std::string jit_source(const std::vector<const Topology*>& ts, const std::vector<int>& group_pattern, bool mixed,
-                       int waves_per_simd) {
+                       int waves_per_simd, bool complex) {
     std::ostringstream o;
     o << "#include \"kernel.h\"\nnamespace s21 {\n";
     // One pattern: the functions inline into the kernel.  Several: each stays a function of its own, so
     // the kernel's registers are not allocated for all of them at once.
     const std::string fn = ts.size() == 1 ? "__device__ inline" : "__device__ __noinline__";
     for (size_t p = 0; p < ts.size(); p++) {
-        if (mixed)
+        if (complex)
+            gen_cx(o, *ts[p], fn, "_" + std::to_string(p));
+        else if (mixed)
             gen_steps(o, *ts[p], fn, "_" + std::to_string(p));
         else
             gen_straight(o, *ts[p], fn, "_" + std::to_string(p));
@@ -263,6 +301,8 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
     };
     o << "struct LuGen {\n"
          "    static constexpr bool lane_per_instance = true;\n"
+         "    static constexpr int kind = "
+      << (complex ? 2 : 1) << ";\n"
          "    template <typename V>\n"
          "    static __device__ bool factor_lane(int g, LDS V* lu, LDS V* c) {\n"
       << dispatch("return ", "lu_gen")
@@ -281,7 +321,7 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
 }
 
 std::string jit_source(const Topology& t, bool mixed, int waves_per_simd) {
-    return jit_source(std::vector<const Topology*>{&t}, std::vector<int>{0}, mixed, waves_per_simd);
+    return jit_source(std::vector<const Topology*>{&t}, std::vector<int>{0}, mixed, waves_per_simd, false);
 }
 
 namespace {

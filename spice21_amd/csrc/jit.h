@@ -19,9 +19,10 @@ int64_t jit_ops(const Topology& t);
 // for at least `waves_per_simd` resident wavefronts per SIMD (its register budget).
 std::string jit_source(const Topology& t, bool mixed, int waves_per_simd);
 // The same for a batch of several patterns: ts[p] for pattern p, group_pattern[g] the pattern of group g
-// (its factorisation is chosen by the group index at run time).
+// (its factorisation is chosen by the group index at run time).  complex: the patterns are AC patterns
+// and the kernel runs AC launches only (otherwise it runs operating points and transients).
 std::string jit_source(const std::vector<const Topology*>& ts, const std::vector<int>& group_pattern, bool mixed,
-                       int waves_per_simd);
+                       int waves_per_simd, bool complex);
 // Code object of a source for gfx950 (no device needed).
 std::vector<char> jit_code(const std::string& src);
 // Compiled kernel for a source (compiled once per process and source).

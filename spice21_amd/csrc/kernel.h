@@ -617,6 +617,9 @@ __device__ inline void commit(const Ctx& C, bool act) {
 // per instance for every instance of the workgroup at once (newton_wg).
 struct LuLevels {
     static constexpr bool lane_per_instance = false;
+    // Analyses a policy serves: 0 every one; a generated policy 1 (operating points and transients of its
+    // real patterns) or 2 (AC launches of its complex patterns).
+    static constexpr int kind = 0;
     template <typename V>
     static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {
         return factor_forward<V>(C, act, lu);
@@ -1050,6 +1053,80 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
     return st == RUNNING ? S21_CONVERGENCE : st;
 }
 
+// The complex Newton of newton_ac for every instance of the workgroup in lockstep, with a factorisation
+// run by one lane per instance (LU::factor_lane / backward_lane on the interleaved complex arrays; the
+// mixed-precision copies of A and of the residual are complex64).
+template <bool MIXED, class LU>
+__device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omega, long long& iters, LDS int* s_act,
+                            int slot, int n_slots, LDS char* region0, int stride, int gidx) {
+    const TopoGPU& T = C.T;
+    int st = act ? RUNNING : S21_OK;
+    bool prev_ok = true;  // dx starts at zero
+    for (int it = 0; it < 20; it++) {
+        if (!__syncthreads_or(act)) break;
+        if (act) iters++;
+        assemble_ac(C, Z_, act, omega);
+        wave_sync();
+        const bool bad = seg_any(C, residual_ac(C, Z_, act));
+        wave_sync();
+        const bool conv = act && prev_ok && !bad;
+        if (conv) {
+            st = S21_OK;
+            act = false;
+        }
+        if (act && T.sym_status != S21_OK) {
+            st = T.sym_status;
+            act = false;
+        }
+        if (MIXED) {
+            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.lu32 + 2 * T.nnz, k, zcvt<zf>(ldz(Z_.c, k)));
+        }
+        if (C.sl == 0) s_act[slot] = act ? 1 : 0;
+        __syncthreads();
+        for (int sl = threadIdx.x; sl < n_slots; sl += blockDim.x) {
+            if (!s_act[sl]) continue;
+            LDS double* c = (LDS double*)(region0 + (size_t)sl * stride) + 2 * T.n;
+            LDS double* a = c + 2 * T.n;
+            bool z;
+            if (MIXED) {
+                LDS float* lu = (LDS float*)(a + 2 * (T.nnz + T.n + T.n_hub + T.n_part));
+                z = LU::template factor_lane<float>(gidx, lu, lu + 2 * T.nnz);
+                if (!z) LU::template backward_lane<float>(gidx, lu, lu + 2 * T.nnz);
+            } else {
+                z = LU::template factor_lane<double>(gidx, a, c);
+                if (!z) LU::template backward_lane<double>(gidx, a, c);
+            }
+            s_act[sl] = z ? 2 : 1;
+        }
+        __syncthreads();
+        if (act && s_act[slot] == 2) {
+            st = S21_PIVOT;
+            act = false;
+        }
+        if (MIXED) {
+            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.c, k, zwide(ldz(Z_.lu32 + 2 * T.nnz, k)));
+            wave_sync();
+        }
+        double mx = 0.0;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            const double d = znorm(ldz(Z_.c, C.I.ce2i[k]));
+            mx = (d > mx) ? d : mx;
+        }
+        mx = seg_max(C, mx);
+        bool ok = true;
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            zd dx = ldz(Z_.c, C.I.ce2i[k]);
+            if (mx > 1.0) dx = zd{dx.re * 1.0 / mx, dx.im * 1.0 / mx};
+            ok = ok && znorm(dx) < 1e-3;
+            stz(Z_.x, k, ldz(Z_.x, k) + dx);
+        }
+        prev_ok = !seg_any(C, !ok);
+        wave_sync();
+    }
+    return st == RUNNING ? S21_CONVERGENCE : st;
+}
+
 // One Newton solve of every instance of the workgroup in lockstep, for an LU policy that factors one
 // lane per instance: the instances' own lanes assemble, test convergence and update; between two
 // barriers the first n_slots threads of the workgroup each factor and sweep one instance's matrix,
@@ -1197,218 +1274,244 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     C.I.dev = b16 + T.o_dev;
     C.I.code = b16 + T.o_code;
     LDS double* dbase = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) + (size_t)slot * A.inst_bytes);
-    if (A.mode == MODE_AC) {  // complex: x | c | A | RHS | hub | partials (| complex-float LU)
-        AcCtx Z_;
-        Z_.x = dbase;
-        Z_.c = Z_.x + 2 * T.n;
-        Z_.a = Z_.c + 2 * T.n;
-        Z_.rhs = Z_.a + 2 * T.nnz;
-        Z_.hub = Z_.rhs + 2 * T.n;
-        Z_.part = Z_.hub + 2 * T.n_hub;
-        Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
-        Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
-        Z_.nm = mos1_count(T);
-        {  // parameters read every iteration: LDS instead of global-memory latency per colour
-            const int nl = T.n_cls[CLS_LIN];
-            GLB const double* par = glb(G.par);
-            for (int q = valid ? C.sl : M1_CORE_F * Z_.nm; q < M1_CORE_F * Z_.nm; q += C.L) {
-                const int f = q / Z_.nm, j = q - f * Z_.nm;
-                Z_.par[q] = par[G.m1_off[f] + (long long)inst * G.m1_istr[f] + j];
-            }
-            for (int j = valid ? C.sl : nl; j < nl; j += C.L)
-                Z_.par[M1_CORE_F * Z_.nm + j] = par[G.lin_off[2] + (long long)inst * G.lin_istr[2] + j];
-            wave_sync();
-        }
-        C.x = Z_.x;
-        const int R = A.ac_runs;
-        const long long F = A.npts;
-        const int u = valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0;
-        auto first_pt = [&](int r) { return F * r / R; };
-        GLB int* rec = glb(G.ac_rec) + (size_t)inst * F;
-        GLB double* wv = glb(G.wave) + (size_t)inst * F * 2 * T.n;
-        GLB double* ends = glb(G.ac_end) + (size_t)inst * R * 2 * T.n;
-        auto load_x = [&](bool act, GLB const double* src) {
-            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{src[2 * k], src[2 * k + 1]});
-            wave_sync();
-        };
-        auto store_x = [&](bool act, GLB double* dst) {
-            for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
-                const zd v = ldz(Z_.x, k);
-                dst[2 * k] = v.re;
-                dst[2 * k + 1] = v.im;
-            }
-        };
-        long long iters = 0;
-        // Newton at frequency pt from the current x; records the point and returns its status.
-        auto point = [&](bool act, long long pt, int& its) {
-            const long long i0 = iters;
-            const int s2 = newton_ac<MIXED>(C, Z_, act, act ? A.omega[pt] : 0.0, iters);
-            its = (int)(iters - i0);
-            if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
-            if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
-            return s2;
-        };
-        // Any frequency of [p, e) failed or unsolved (the lanes of the segment share the scan).
-        auto failed_in = [&](long long p, long long e) {
-            bool f = false;
-            for (long long q = p + C.sl; q < e; q += C.L) f = f || rec[q] == 0 || (rec[q] >> 8) != S21_OK;
-            return seg_any(C, f);
-        };
-        // A frequency solved (at least one full Newton step) on both paths: both hold its unique solution.
-        auto meets = [&](int its, int old) { return its >= 2 && old != 0 && (old >> 8) == S21_OK && (old & 0xFF) >= 2; };
-        if (A.ac_pass == 0) {  // run u from x = 0 (analysis.rs:215-233)
-            bool alive = valid;
-            for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});
-            wave_sync();
-            const long long p0 = first_pt(u), p1 = first_pt(u + 1);
-            for (long long pt = p0; pt < p1; pt++) {
-                if (!wave_any(alive)) break;
-                int its;
-                const int s2 = point(alive, pt, its);
-                if (alive) alive = s2 == S21_OK;
-            }
-            store_x(valid, ends + (size_t)u * 2 * T.n);
-        } else if (A.ac_pass == 1) {  // run r from the first pass's end of run r - 1, until the paths meet
-            const int r = u + 1;
-            bool live = valid;
-            load_x(live, ends + (size_t)(r - 1) * 2 * T.n);
-            int state = AC_UNMET;
-            const long long p0 = first_pt(r), p1 = first_pt(r + 1);
-            for (long long pt = p0; pt < p1; pt++) {
-                if (!wave_any(live)) break;
-                const int old = live ? rec[pt] : 0;
-                int its;
-                const int s2 = point(live, pt, its);
-                if (live && s2 != S21_OK) {
-                    state = AC_FAILED;
-                    live = false;
-                } else if (live && meets(its, old)) {
-                    state = AC_MET;
-                    live = false;
+    if (A.mode == MODE_AC) {  // complex: x | c | A | RHS | hub | partials (| complex-float LU) | parameters
+        if constexpr (LU::kind != 1) {
+            AcCtx Z_;
+            Z_.x = dbase;
+            Z_.c = Z_.x + 2 * T.n;
+            Z_.a = Z_.c + 2 * T.n;
+            Z_.rhs = Z_.a + 2 * T.nnz;
+            Z_.hub = Z_.rhs + 2 * T.n;
+            Z_.part = Z_.hub + 2 * T.n_hub;
+            Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
+            Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
+            Z_.nm = mos1_count(T);
+            {  // parameters read every iteration: LDS instead of global-memory latency per colour
+                const int nl = T.n_cls[CLS_LIN];
+                GLB const double* par = glb(G.par);
+                for (int q = valid ? C.sl : M1_CORE_F * Z_.nm; q < M1_CORE_F * Z_.nm; q += C.L) {
+                    const int f = q / Z_.nm, j = q - f * Z_.nm;
+                    Z_.par[q] = par[G.m1_off[f] + (long long)inst * G.m1_istr[f] + j];
                 }
+                for (int j = valid ? C.sl : nl; j < nl; j += C.L)
+                    Z_.par[M1_CORE_F * Z_.nm + j] = par[G.lin_off[2] + (long long)inst * G.lin_istr[2] + j];
+                wave_sync();
             }
-            if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
-        } else {  // in order from run rs, jumping to the next run wherever the paths meet
-            const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
-            bool live = valid && rs > 0;
-            long long k = live ? first_pt(rs) : 1;
-            load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
-            while (wave_any(live)) {
-                const int old = live ? rec[k] : 0;
-                int its;
-                const int s2 = point(live, k, its);
-                if (!live) continue;
-                long long next = k + 1;
-                if (s2 != S21_OK) {
-                    live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
-                } else if (meets(its, old)) {
-                    int rr = (int)((k * R) / F);
-                    while (first_pt(rr + 1) <= k) rr++;
-                    while (first_pt(rr) > k) rr--;
-                    // The rest of run rr is stored data of a path through frequency k's solution: the sweep's
-                    // (a failure there is the sweep's).
-                    bool bad = failed_in(k + 1, first_pt(rr + 1));
-                    int q = rr + 1;
-                    // Run rr ends as the first pass ended it unless its re-solve never met that pass; then
-                    // every following run whose re-solve met is the sweep's, and so is a run whose re-solve
-                    // ran to its end (its start was the sweep's) or failed (the sweep fails there).
-                    if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
-                        while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
-                            bad = failed_in(first_pt(q), first_pt(q + 1));
-                            q++;
-                        }
-                        if (!bad && q < R) {
-                            if (G.ac_state[(size_t)inst * R + q] == AC_FAILED)
-                                bad = true;
-                            else
-                                q++;
-                        }
+            C.x = Z_.x;
+            const int R = A.ac_runs;
+            const long long F = A.npts;
+            const int u = valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0;
+            auto first_pt = [&](int r) { return F * r / R; };
+            GLB int* rec = glb(G.ac_rec) + (size_t)inst * F;
+            GLB double* wv = glb(G.wave) + (size_t)inst * F * 2 * T.n;
+            GLB double* ends = glb(G.ac_end) + (size_t)inst * R * 2 * T.n;
+            auto load_x = [&](bool act, GLB const double* src) {
+                for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{src[2 * k], src[2 * k + 1]});
+                wave_sync();
+            };
+            auto store_x = [&](bool act, GLB double* dst) {
+                for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+                    const zd v = ldz(Z_.x, k);
+                    dst[2 * k] = v.re;
+                    dst[2 * k + 1] = v.im;
+                }
+            };
+            // Loops run in step across the workgroup when one lane per instance factors (its barriers).
+            auto any = [&](bool p) {
+                if constexpr (LU::lane_per_instance)
+                    return __syncthreads_or(p) != 0;
+                else
+                    return wave_any(p);
+            };
+            long long iters = 0;
+            // Newton at frequency pt from the current x; records the point and returns its status.
+            auto point = [&](bool act, long long pt, int& its) {
+                const long long i0 = iters;
+                const double om = act ? A.omega[pt] : 0.0;
+                int s2;
+                if constexpr (LU::lane_per_instance) {
+                    __shared__ int s_act[WAVE * MAX_WAVES];
+                    LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
+                    s2 = newton_ac_wg<MIXED, LU>(C, Z_, act, om, iters, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
+                                                 A.inst_bytes, g);
+                } else {
+                    s2 = newton_ac<MIXED>(C, Z_, act, om, iters);
+                }
+                its = (int)(iters - i0);
+                if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
+                if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
+                return s2;
+            };
+            // Any frequency of [p, e) failed or unsolved (the lanes of the segment share the scan).
+            auto failed_in = [&](long long p, long long e) {
+                bool f = false;
+                for (long long q = p + C.sl; q < e; q += C.L) f = f || rec[q] == 0 || (rec[q] >> 8) != S21_OK;
+                return seg_any(C, f);
+            };
+            // A frequency solved (at least one full Newton step) on both paths: both hold its unique solution.
+            auto meets = [&](int its, int old) {
+                return its >= 2 && old != 0 && (old >> 8) == S21_OK && (old & 0xFF) >= 2;
+            };
+            if (A.ac_pass == 0) {  // run u from x = 0 (analysis.rs:215-233)
+                bool alive = valid;
+                for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});
+                wave_sync();
+                const long long p0 = first_pt(u), p1 = first_pt(u + 1);
+                for (long long pt = p0;; pt++) {
+                    const bool here = alive && pt < p1;
+                    if (!any(here)) break;
+                    int its;
+                    const int s2 = point(here, pt, its);
+                    if (here) alive = s2 == S21_OK;
+                }
+                store_x(valid, ends + (size_t)u * 2 * T.n);
+            } else if (A.ac_pass == 1) {  // run r from the first pass's end of run r - 1, until the paths meet
+                const int r = u + 1;
+                bool live = valid;
+                load_x(live, ends + (size_t)(r - 1) * 2 * T.n);
+                int state = AC_UNMET;
+                const long long p0 = first_pt(r), p1 = first_pt(r + 1);
+                for (long long pt = p0;; pt++) {
+                    const bool here = live && pt < p1;
+                    if (!any(here)) break;
+                    const int old = here ? rec[pt] : 0;
+                    int its;
+                    const int s2 = point(here, pt, its);
+                    if (here && s2 != S21_OK) {
+                        state = AC_FAILED;
+                        live = false;
+                    } else if (here && meets(its, old)) {
+                        state = AC_MET;
+                        live = false;
                     }
-                    next = first_pt(q);
-                    if (bad) live = false;
-                    if (live && next < F) load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
                 }
-                if (next >= F) live = false;
-                k = next;
+                if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
+            } else {  // in order from run rs, jumping to the next run wherever the paths meet
+                const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
+                bool live = valid && rs > 0;
+                long long k = live ? first_pt(rs) : 1;
+                load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
+                while (any(live)) {
+                    const int old = live ? rec[k] : 0;
+                    int its;
+                    const int s2 = point(live, k, its);
+                    if (live) {
+                        long long next = k + 1;
+                        if (s2 != S21_OK) {
+                            live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
+                        } else if (meets(its, old)) {
+                            int rr = (int)((k * R) / F);
+                            while (first_pt(rr + 1) <= k) rr++;
+                            while (first_pt(rr) > k) rr--;
+                            // The rest of run rr is stored data of a path through frequency k's solution: the
+                            // sweep's (a failure there is the sweep's).
+                            bool bad = failed_in(k + 1, first_pt(rr + 1));
+                            int q = rr + 1;
+                            // Run rr ends as the first pass ended it unless its re-solve never met that pass;
+                            // then every following run whose re-solve met is the sweep's, and so is a run whose
+                            // re-solve ran to its end (its start was the sweep's) or failed (the sweep fails there).
+                            if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
+                                while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
+                                    bad = failed_in(first_pt(q), first_pt(q + 1));
+                                    q++;
+                                }
+                                if (!bad && q < R) {
+                                    if (G.ac_state[(size_t)inst * R + q] == AC_FAILED)
+                                        bad = true;
+                                    else
+                                        q++;
+                                }
+                            }
+                            next = first_pt(q);
+                            if (bad) live = false;
+                            if (live && next < F) load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
+                        }
+                        if (next >= F) live = false;
+                        k = next;
+                    }
+                }
             }
         }
         return;
     }
-    C.x = dbase;
-    C.c = C.x + T.n;
-    C.a = C.c + T.n;
-    C.rhs = C.a + T.nnz;
-    C.hub = C.rhs + T.n;
-    C.part = C.hub + T.n_hub;
-    C.lu32 = (LDS float*)(C.part + T.n_part);
+    if constexpr (LU::kind != 2) {
+        C.x = dbase;
+        C.c = C.x + T.n;
+        C.a = C.c + T.n;
+        C.rhs = C.a + T.nnz;
+        C.hub = C.rhs + T.n;
+        C.part = C.hub + T.n_hub;
+        C.lu32 = (LDS float*)(C.part + T.n_part);
 
-    for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) C.x[k] = G.x[(size_t)inst * T.n + k];
-    wave_sync();
-
-    if (A.mode == MODE_ASSEMBLE) {
-        assemble(C, valid, A.asm_tran != 0, A);
+        for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) C.x[k] = G.x[(size_t)inst * T.n + k];
         wave_sync();
-        for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
-        return;
-    }
 
-    long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const long long tstart = A.prof ? cycles() : 0;
-    long long iters = 0;
-    int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
-    if (A.resume && valid) {
-        iters = G.iters[inst];
-        pt0 = G.resume_at[2 * inst];
-        used0 = G.resume_at[2 * inst + 1];
-    }
-    int last_pt = 0;
-    long long last_start = 0;  // iteration count when the last solve began
-    LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
-    auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
-        if constexpr (LU::lane_per_instance) {
-            __shared__ int s_act[WAVE * MAX_WAVES];
-            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
-                                        A.waves * A.ipw, region0, A.inst_bytes, g);
-        } else {
-            return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
+        if (A.mode == MODE_ASSEMBLE) {
+            assemble(C, valid, A.asm_tran != 0, A);
+            wave_sync();
+            for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
+            return;
         }
-    };
-    last_start = iters - used0;
-    const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
-    int st = solve(valid && op_here, false, A.max_iter - (A.resume ? used0 : 0), !A.resume);
-    if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
-    if (A.mode == MODE_TRAN) {
-        bool alive = valid && st == S21_OK;
-        double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
-        for (int k = alive && op_here ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
-        for (long long pt = 1; pt < A.npts; pt++) {  // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
-            if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
-            const bool here = alive && pt >= pt0;   // a resumed instance joins at its own point
-            const bool cont = A.resume && pt == pt0;  // ... with the solve it continues
-            if (here) {
-                last_pt = (int)pt;
-                last_start = cont ? iters - used0 : iters;
+
+        long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const long long tstart = A.prof ? cycles() : 0;
+        long long iters = 0;
+        int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
+        if (A.resume && valid) {
+            iters = G.iters[inst];
+            pt0 = G.resume_at[2 * inst];
+            used0 = G.resume_at[2 * inst + 1];
+        }
+        int last_pt = 0;
+        long long last_start = 0;  // iteration count when the last solve began
+        LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
+        auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
+            if constexpr (LU::lane_per_instance) {
+                __shared__ int s_act[WAVE * MAX_WAVES];
+                return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
+                                            A.waves * A.ipw, region0, A.inst_bytes, g);
+            } else {
+                return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
             }
-            const int s2 = solve(here, true, cont ? A.max_iter - used0 : A.max_iter, !cont);
-            if (here) {
-                st = s2;
-                alive = s2 == S21_OK;
+        };
+        last_start = iters - used0;
+        const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
+        int st = solve(valid && op_here, false, A.max_iter - (A.resume ? used0 : 0), !A.resume);
+        if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
+        if (A.mode == MODE_TRAN) {
+            bool alive = valid && st == S21_OK;
+            double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
+            for (int k = alive && op_here ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
+            // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
+            for (long long pt = 1; pt < A.npts; pt++) {
+                if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
+                const bool here = alive && pt >= pt0;   // a resumed instance joins at its own point
+                const bool cont = A.resume && pt == pt0;  // ... with the solve it continues
+                if (here) {
+                    last_pt = (int)pt;
+                    last_start = cont ? iters - used0 : iters;
+                }
+                const int s2 = solve(here, true, cont ? A.max_iter - used0 : A.max_iter, !cont);
+                if (here) {
+                    st = s2;
+                    alive = s2 == S21_OK;
+                }
+                for (int k = here && alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
             }
-            for (int k = here && alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
         }
-    }
-    if (!valid) return;
-    for (int k = C.sl; k < T.n; k += C.L) G.x[(size_t)inst * T.n + k] = C.x[k];
-    if (C.sl == 0) {
-        G.status[inst] = st;
-        G.iters[inst] = iters;
-        if (G.resume_at) {
-            G.resume_at[2 * inst] = last_pt;
-            G.resume_at[2 * inst + 1] = (int)(iters - last_start);
-        }
-        if (A.prof && G.prof) {
-            pc[7] = cycles() - tstart;
-            for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
+        if (!valid) return;
+        for (int k = C.sl; k < T.n; k += C.L) G.x[(size_t)inst * T.n + k] = C.x[k];
+        if (C.sl == 0) {
+            G.status[inst] = st;
+            G.iters[inst] = iters;
+            if (G.resume_at) {
+                G.resume_at[2 * inst] = last_pt;
+                G.resume_at[2 * inst + 1] = (int)(iters - last_start);
+            }
+            if (A.prof && G.prof) {
+                pc[7] = cycles() - tstart;
+                for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
+            }
         }
     }
 }

@@ -231,6 +231,31 @@ def test_ac_runs_of_frequencies_match_one_sequential_sweep():
         assert worst < 1e-9, (runs, worst)
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_ac_generated_factorisation_matches_level_schedule(precision):
+    """The AC kernel with the complex factorisation written out (one lane per instance) against the
+    record-driven one: the same operations, so the same iteration counts, and values equal to rounding."""
+    n_inst = 24
+    nl, over = W.amp_ladder(n_inst)
+    g = Group(nl.to_c(), n_inst, over)
+    log("ac levels", precision)
+    a = Batch(g, precision=precision).set_jit(-1).ac(1000, 100_000_000_000, 500)
+    assert a.config()[2] is False
+    log("ac generated", precision)
+    b = Batch(g, precision=precision).set_jit(1).ac(1000, 100_000_000_000, 500)
+    assert b.config()[2] is True
+    sa, ia = a.status()
+    sb, ib = b.status()
+    assert np.all(sa == 0) and np.all(sb == 0)
+    worst = max(rel_err(b.wave(0, i).ravel(), a.wave(0, i).ravel()) for i in range(n_inst))
+    log("  iterations", int(ia.sum()), int(ib.sum()), "max rel", worst)
+    assert np.array_equal(ia, ib)
+    assert worst <= (1e-12 if precision == FP64 else 1e-8)
+    if precision == FP64:
+        fo, Xo, _ = O.ac(materialize(nl, over, 3).to_c(), 1000, 100_000_000_000, 500, max_pts=600)
+        assert rel_err(b.wave(0, 3).ravel(), Xo.ravel()) < 1e-9
+
+
 def test_ac_mixed_close_to_fp64():
     n_inst = 16
     nl, over = W.amp_ladder(n_inst)
