@@ -82,9 +82,20 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
 
 
-def extra_measurements():
+def extra_measurements(which=("tran", "ac", "ring5")):
     """The metric's transient rate and the other workloads, by HIP-event kernel time (each analysis call
     also rebuilds its host structures, which is not the kernel's throughput)."""
+    out = {}
+    if "tran" in which:
+        out.update(_tran_ring_batch())
+    if "ac" in which:
+        out.update(_ac_amp_ladder())
+    if "ring5" in which:
+        out.update(_tran_single_ring5())
+    return out
+
+
+def _tran_ring_batch():
     import numpy as np
 
     from spice21_amd import workloads as W
@@ -107,20 +118,41 @@ def extra_measurements():
     rec["mixed_vs_fp64_max_node_v"] = float(max(np.abs(a - c).max() for a, c in zip(waves["fp64"], waves["mixed"])))
     out["tran_ring_batch"] = {"workload": "1024 ring oscillators of 3..101 stages (C, VDD per instance), "
                                           "0-5 ns at 1 ps, 5000 points", **rec}
+    return out
+
+
+def _ac_amp_ladder():
+    import numpy as np
+
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
+
+    out = {}
     # batched AC sweep
     nl, over = W.amp_ladder(256)
     g = Group(nl.to_c(), 256, over)
     rec, sw = {}, {}
     for name, prec in (("fp64", FP64), ("mixed", MIXED)):
-        b = Batch(g, precision=prec).ac(1000, 100_000_000_000, 1000)
-        ms = b.timing()[0]
+        b = Batch(g, precision=prec).ac(1000, 100_000_000_000, 1000)  # untimed: builds the generated kernel
+        b.ac(1000, 100_000_000_000, 1000)
+        ms, launches, _ = b.timing()
         nf = len(b.axis())
-        rec[name] = {"instance_freqs_per_s": 256 * nf / (ms * 1e-3), "kernel_ms": ms,
-                     "failed_instances": int(np.sum(b.status()[0] != 0))}
+        st, it = b.status()
+        rec[name] = {"instance_freqs_per_s": 256 * nf / (ms * 1e-3), "kernel_ms": ms, "launches": launches,
+                     "runs_per_instance": b.ac_runs(), "generated_factorisation": b.config()[2],
+                     "newton_iters": int(it.sum()), "failed_instances": int(np.sum(st != 0))}
         sw[name] = [b.wave(0, i) for i in range(0, 256, 37)]
     rec["mixed_vs_fp64_max_abs"] = float(max(np.abs(a - c).max() for a, c in zip(sw["fp64"], sw["mixed"])))
     out["ac_amp_ladder"] = {"workload": "256 two-stage CS amplifiers with a 20-section RC ladder, 1 kHz-100 GHz, "
                                         "1002 frequencies", **rec}
+    return out
+
+
+def _tran_single_ring5():
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import Batch, Group
+
+    out = {}
     # one 5-stage ring oscillator (latency)
     b = Batch(Group(W.ring5().to_c())).tran(1e-12, 1e-8, [("1", 0.5)])
     ms = b.timing()[0]
@@ -156,6 +188,8 @@ def main():
     ap.add_argument("--ipw", type=int, default=0, help="instances per wavefront (implies --no-autotune)")
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the transient / AC / single-ring measurements")
+    ap.add_argument("--extra-only", choices=["tran", "ac", "ring5"], default=None,
+                    help="only run one of the other workloads and print its record (for profiling runs)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="'mixed' / 'fp64' time one path only (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -164,6 +198,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.extra_only:
+        print(json.dumps(extra_measurements((args.extra_only,))), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
