@@ -411,15 +411,22 @@ static int auto_ipw(int total_inst, int simds) {
 }
 
 // Derive per-instance parameters, build the topology and upload everything for one analysis.
-// Batches of at most this many instances are latency-bound (one wavefront each at most): every shared
-// slot becomes a hub so that each device class is evaluated in one round instead of one per colour.
-constexpr int LATENCY_INST = 64;
+// A batch of at most two instances per SIMD leaves each SIMD at most two wavefronts: latency-bound, so
+// every shared slot becomes a hub and each device class is evaluated in one round instead of one per
+// colour (more LDS cells and a longer gather, fewer dependent rounds).  Not for AC, whose device
+// stamps are cheap: there the longer complex gather costs more than the rounds it saves (measured on
+// the 256-instance amplifier sweep: 5.5 ms without, 6.8 ms with, in mixed precision).
+constexpr int LATENCY_INST_PER_SIMD = 2;
 
 static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr, const AcInput* ac = nullptr) {
     HostPrep hp;
     int64_t total = 0;
     for (const GroupHost& x : b->groups) total += x.n_inst;
-    host_prepare(b->opts, g, tr, hp, ac, total <= LATENCY_INST ? 1 : HUB_CONTRIB);
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool latency = total <= (int64_t)LATENCY_INST_PER_SIMD * 4 * std::max(cus, 1);
+    host_prepare(b->opts, g, tr, hp, ac, latency && !ac ? 1 : HUB_CONTRIB);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
 
