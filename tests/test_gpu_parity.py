@@ -256,6 +256,29 @@ def test_ac_generated_factorisation_matches_level_schedule(precision):
         assert rel_err(b.wave(0, 3).ravel(), Xo.ravel()) < 1e-9
 
 
+@pytest.mark.parametrize("jit", [-1, 1])
+def test_ac_batch_of_two_patterns_matches_oracle(jit):
+    """Two groups of different AC patterns in one batch (two generated complex factorisations chosen by
+    group index, or the record-driven ones), each sweep cut into runs and stitched: every instance
+    against the oracle, with the sequential sweep's iteration counts."""
+    parts = [(W.amp_ladder(5, sections=20, seed=1), 5), (W.amp_ladder(3, sections=7, seed=2), 3)]
+    groups = [Group(nl.to_c(), k, over) for (nl, over), k in parts]
+    log("ac two patterns, jit", jit)
+    b = Batch(groups).set_jit(jit).ac(1000, 100_000_000_000, 400)
+    assert b.config()[2] is (jit == 1) and b.ac_runs() > 1
+    seq = Batch(groups).set_jit(jit).set_ac_runs(1).ac(1000, 100_000_000_000, 400)
+    for gi, ((nl, over), k) in enumerate(parts):
+        st, it = b.status(gi)
+        st1, it1 = seq.status(gi)
+        assert np.all(st == 0) and np.all(st1 == 0)
+        assert np.array_equal(it, it1), (gi, it, it1)
+        for i in range(k):
+            fo, Xo, _ = O.ac(materialize(nl, over, i).to_c(), 1000, 100_000_000_000, 400, max_pts=500)
+            assert np.array_equal(b.axis(), fo)
+            err = rel_err(b.wave(gi, i).ravel(), Xo.ravel())
+            assert err < 1e-9, (gi, i, err)
+
+
 def test_ac_mixed_close_to_fp64():
     n_inst = 16
     nl, over = W.amp_ladder(n_inst)
