@@ -673,10 +673,12 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
             return n;
         };
         const long target = std::min(cus, workgroups(1));
+        long largest = 0;  // launch units of the largest group: no wavefront beyond them has work
+        for (const GroupHost& g : b->groups) largest = std::max(largest, (long)g.n_inst * ac_units(A));
         int best = 0;
         for (int k = 1; k <= MAX_WAVES; k++) {
             if (blob + (size_t)k * ipw * stride > max_block) break;
-            if (k > 1 && workgroups(k) < target) break;
+            if (k > 1 && (workgroups(k) < target || (long)(k - 1) * ipw >= largest)) break;
             best = k;
         }
         if (best > 0) {
