@@ -133,3 +133,35 @@ def diode_rc(cj0=1e-12, tt=0.0, forward=False):
         n.diode("d1", "", "a", dm, n.diode_inst())
     n.c(1e-12, "a", "")
     return n
+
+
+def random_ac_circuit(seed):
+    """A random netlist of the components AC supports (R, C, Mos1 with and without terminal resistances
+    and gate capacitances, one or two sources with AC magnitudes), every node tied to ground by 1e-6 S."""
+    import numpy as np
+
+    rng = np.random.default_rng(1000 + seed)
+    n = Netlist()
+    nodes = [f"n{k}" for k in range(int(rng.integers(3, 12)))]
+    for nd in nodes:
+        n.addv(nd)
+    nm = n.mos1_model(vt0=0.4, kp=100e-6, lambda_=0.02, tox=1e-8, cgso=2e-10, cgdo=2e-10)
+    pm = n.mos1_model(mos_type=1, vt0=-0.4, kp=50e-6)
+    rm = n.mos1_model(vt0=0.3, rd=50.0, rs=20.0, tox=2e-8)
+    ip = n.mos1_inst(w=1e-6, l=0.1e-6)
+    pick = lambda: nodes[int(rng.integers(len(nodes)))] if rng.random() > 0.15 else ""
+    n.v("vs", float(rng.uniform(0.5, 1.5)), nodes[0], "", acm=1.0)
+    if rng.random() < 0.5:
+        n.v("va", float(rng.uniform(0.0, 0.5)), nodes[1], "", acm=float(rng.uniform(0.1, 2.0)))
+    for k in range(int(rng.integers(4, 20))):
+        c = rng.random()
+        if c < 0.35:
+            n.r(float(rng.uniform(1e-4, 1e-2)), pick(), pick())
+        elif c < 0.6:
+            n.c(float(rng.uniform(1e-15, 1e-12)), pick(), pick())
+        else:
+            m = [nm, pm, rm][int(rng.integers(3))]
+            n.mos1(f"m{k}", pick(), pick(), pick(), pick(), m, ip)
+    for nd in nodes:  # keep every node connected
+        n.r(1e-6, nd, "")
+    return n

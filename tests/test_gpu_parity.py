@@ -13,7 +13,7 @@ from oracle import oracle_py as O
 from spice21_amd import workloads as W
 from spice21_amd.batch import FP64, MIXED, Batch, Group
 from spice21_amd.elab import elaborate
-from tests.circuits import GOLDEN_RUNS, materialize, py_ring3, ro3
+from tests.circuits import GOLDEN_RUNS, materialize, py_ring3, random_ac_circuit, ro3
 from tests.test_symbolic import random_circuit
 
 pytestmark = pytest.mark.gpu
@@ -277,6 +277,32 @@ def test_ac_batch_of_two_patterns_matches_oracle(jit):
             assert np.array_equal(b.axis(), fo)
             err = rel_err(b.wave(gi, i).ravel(), Xo.ravel())
             assert err < 1e-9, (gi, i, err)
+
+
+@pytest.mark.parametrize("jit", [-1, 1])
+def test_random_circuits_ac_match_oracle(jit):
+    """30 random netlists of R, C, Mos1 (some with terminal resistances, some with gate capacitances) and
+    AC sources, one group each in one batch, swept from 1 kHz to 10 GHz in runs of frequencies: each
+    sweep's outcome and values against the oracle (3 of the 30 fail to converge there)."""
+    nls = [random_ac_circuit(s) for s in range(30)]
+    log("random ac, jit", jit)
+    b = Batch([Group(nl.to_c()) for nl in nls]).set_jit(jit).ac(1000, 10_000_000_000, 200)
+    log("  generated", b.config()[2], "runs", b.ac_runs())
+    f = b.axis()
+    fails = 0
+    for gi, nl in enumerate(nls):
+        st, _ = b.status(gi)
+        try:
+            fo, Xo, _ = O.ac(nl.to_c(), 1000, 10_000_000_000, 200, max_pts=300)
+        except O.OracleError as e:
+            assert e.desc == "Convergence Failed" and st[0] == 2, (gi, e.desc, st[0])
+            fails += 1
+            continue
+        assert st[0] == 0, (gi, st[0])
+        assert np.array_equal(f, fo)
+        err = rel_err(b.wave(gi, 0).ravel(), Xo.ravel())
+        assert err < 1e-9, (gi, err)
+    assert fails == 3
 
 
 def test_ac_mixed_close_to_fp64():
