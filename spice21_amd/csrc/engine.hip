@@ -653,6 +653,7 @@ static size_t inst_stride(size_t raw, int ipw, bool lane_per_instance = false) {
     return ((raw + 255) / 256) * 256 + (lane_per_instance ? 8 : (L * 8) % 256);
 }
 constexpr size_t JIT_STATIC_LDS = sizeof(int) * WAVE * MAX_WAVES;  // newton_wg's per-slot flags
+constexpr size_t AC_STATIC_LDS = 128;  // the AC third pass's flags and outcome (64 bytes), with alignment
 
 // Launch shape: instances per wavefront (the batch's setting, or auto_ipw) and wavefronts per workgroup:
 // the most that still leave a workgroup for every CU (or as many workgroups as one wavefront per
@@ -662,7 +663,20 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
                         Args& A, size_t& stride) {
     int ipw = b->ipw;
     const bool jit = b->has_jit;
+    blob += AC_STATIC_LDS;
     if (jit) blob += JIT_STATIC_LDS;
+    if (A.mode == MODE_AC && A.ac_pass == 2) {  // a workgroup per instance, its wavefronts looking ahead
+        stride = inst_stride(raw, 1, jit);
+        if (blob + stride > max_block)
+            fail(S21_INVALID, "circuit too large for the batched kernel: " + std::to_string(blob + stride) +
+                                  " bytes of LDS per instance");
+        int k = MAX_WAVES;
+        while (k > 1 && blob + (size_t)k * stride > max_block) k--;
+        A.ipw = 1;
+        A.waves = k;
+        A.ac_look = k;
+        return;
+    }
     if (ipw <= 0) ipw = auto_ipw(total_inst, simds);
     const long cus = std::max(simds / 4, 1);
     for (;; ipw /= 2) {
@@ -784,6 +798,7 @@ static Args base_args(const s21_batch* b, int mode) {
     A.omega = nullptr;
     A.ac_runs = 1;
     A.ac_pass = 0;
+    A.ac_look = 1;
     return A;
 }
 
@@ -1300,15 +1315,17 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
         // into contiguous runs, solved in parallel from x = 0 (first pass); each run is then re-solved
         // from the previous run's end until both paths take a full step at the same frequency, after
         // which they agree (second pass); a run whose re-solve never meets its first pass is continued
-        // in order (third pass).  The result is the sequential sweep's, and so are the iteration counts.
+        // in order (third pass, a workgroup per instance whose wavefronts test the next frequencies for
+        // the reference's immediate acceptance together).  The result is the sequential sweep's, and so
+        // are the iteration counts.
         int runs = b->ac_runs;
-        if (runs <= 0) {  // about eight wavefronts per CU in the first pass
+        if (runs <= 0) {  // about sixteen wavefronts per CU in the first pass
             int dev = 0, cus = 0;
             HIPCHECK(hipGetDevice(&dev));
             HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             int64_t inst = 0;
             for (const GroupHost& g : b->groups) inst += g.n_inst;
-            runs = (int)std::min<int64_t>((32 * (int64_t)std::max(cus, 1) + inst - 1) / inst, std::max<int64_t>(F / 16, 1));
+            runs = (int)std::min<int64_t>((64 * (int64_t)std::max(cus, 1) + inst - 1) / inst, std::max<int64_t>(F / 16, 1));
         }
         runs = (int)std::max<int64_t>(std::min<int64_t>(runs, F), 1);
         for (GroupHost& g : b->groups) {

@@ -105,12 +105,15 @@ struct Args {
     int ac_runs;          // AC: each instance's frequencies split into this many contiguous runs (1 outside AC)
     int ac_pass;          // AC: 0 = every run from x = 0; 1 = each run re-solved from the previous run's
                           // end until it meets the first pass; 2 = in order from G.ac_state's run
+    int ac_look;          // AC, pass 2: wavefronts (a workgroup) per instance, each testing one of the next
+                          // frequencies for the reference's immediate acceptance, so that a stretch of
+                          // kept solutions costs one round
 };
 
 // AC launch units per instance: one per run, one per run boundary, or one.
 __host__ __device__ inline int ac_units(const Args& A) {
     if (A.mode != MODE_AC) return 1;
-    return A.ac_pass == 0 ? A.ac_runs : A.ac_pass == 1 ? A.ac_runs - 1 : 1;
+    return A.ac_pass == 0 ? A.ac_runs : A.ac_pass == 1 ? A.ac_runs - 1 : A.ac_look;
 }
 enum { AC_UNMET = 0, AC_MET = 1, AC_FAILED = 2 };
 
@@ -1386,30 +1389,75 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
                     }
                 }
                 if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
-            } else {  // in order from run rs, jumping to the next run wherever the paths meet
+            } else {
+                // In order from run rs, jumping to the next run wherever the paths meet.  The A.ac_look
+                // wavefronts of the workgroup serve one instance: from the current solution x, wavefront w
+                // runs iteration 0 of frequency k + w (assembly and residual: the reference keeps x when
+                // every |res| < 1e-9, since dx starts at zero); the first frequency that does not keep x
+                // is solved by wavefront 0, and the ones before it keep x.
+                __shared__ int s_flag[MAX_WAVES];
+                __shared__ long long s_out[4];
+                const int w = u;                       // this wavefront's offset
+                const int lead = slot - w;             // the instance's first slot in the workgroup
+                LDS double* x_lead = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) +
+                                                   (size_t)lead * A.inst_bytes);
                 const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
                 bool live = valid && rs > 0;
                 long long k = live ? first_pt(rs) : 1;
                 load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
-                while (any(live)) {
-                    const int old = live ? rec[k] : 0;
-                    int its;
-                    const int s2 = point(live, k, its);
+                // The workgroup is this instance's A.ac_look wavefronts, so live is the same in all of them.
+                while (__syncthreads_or(live)) {
+                    const long long kw = k + w;
+                    const bool test = live && kw < F;
+                    bool keep = false;
+                    if (wave_any(test)) {
+                        assemble_ac(C, Z_, test, test ? A.omega[kw] : 0.0);
+                        wave_sync();
+                        const bool bad = seg_any(C, residual_ac(C, Z_, test));
+                        wave_sync();
+                        keep = test && !bad;
+                    }
+                    if (C.sl == 0) s_flag[slot] = keep ? 1 : 0;
+                    __syncthreads();
+                    int ws = A.ac_look;  // first offset that does not keep x
+                    for (int j = 0; j < A.ac_look; j++)
+                        if (!s_flag[lead + j]) {
+                            ws = j;
+                            break;
+                        }
+                    if (live && w < ws) {  // kept: x stands, one iteration
+                        if (C.sl == 0) rec[kw] = (S21_OK << 8) | 1;
+                        store_x(true, wv + (size_t)kw * 2 * T.n);
+                    }
+                    const long long ks = k + ws;
+                    const bool solve = live && ws < A.ac_look && ks < F;
+                    const int old = solve && w == 0 ? rec[ks] : 0;  // before wavefront 0 records ks
+                    int its = 0;
+                    const int s2 = point(solve && w == 0, ks, its);
+                    if (solve && w == 0 && C.sl == 0) {
+                        s_out[0] = s2;
+                        s_out[1] = its;
+                        s_out[2] = old;
+                    }
+                    __syncthreads();
                     if (live) {
-                        long long next = k + 1;
-                        if (s2 != S21_OK) {
+                        long long next = ks + 1;
+                        if (!solve) {
+                            next = ks;  // every tested frequency kept x (or the sweep ended)
+                        } else if (s_out[0] != S21_OK) {
                             live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
-                        } else if (meets(its, old)) {
-                            int rr = (int)((k * R) / F);
-                            while (first_pt(rr + 1) <= k) rr++;
-                            while (first_pt(rr) > k) rr--;
-                            // The rest of run rr is stored data of a path through frequency k's solution: the
-                            // sweep's (a failure there is the sweep's).
-                            bool bad = failed_in(k + 1, first_pt(rr + 1));
+                        } else if (meets((int)s_out[1], (int)s_out[2])) {
+                            int rr = (int)((ks * R) / F);
+                            while (first_pt(rr + 1) <= ks) rr++;
+                            while (first_pt(rr) > ks) rr--;
+                            // The rest of run rr is stored data of a path through frequency ks's solution:
+                            // the sweep's (a failure there is the sweep's).
+                            bool bad = failed_in(ks + 1, first_pt(rr + 1));
                             int q = rr + 1;
-                            // Run rr ends as the first pass ended it unless its re-solve never met that pass;
-                            // then every following run whose re-solve met is the sweep's, and so is a run whose
-                            // re-solve ran to its end (its start was the sweep's) or failed (the sweep fails there).
+                            // Run rr ends as the first pass ended it unless its re-solve never met that
+                            // pass; then every following run whose re-solve met is the sweep's, and so is a
+                            // run whose re-solve ran to its end (its start was the sweep's) or failed (the
+                            // sweep fails there).
                             if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
                                 while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
                                     bad = failed_in(first_pt(q), first_pt(q + 1));
@@ -1424,11 +1472,18 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
                             }
                             next = first_pt(q);
                             if (bad) live = false;
-                            if (live && next < F) load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
                         }
                         if (next >= F) live = false;
+                        if (live && solve) {  // every wavefront continues from the sweep's solution at next - 1
+                            if (next - 1 == ks)
+                                for (int j = C.sl; j < 2 * T.n; j += C.L) Z_.x[j] = x_lead[j];
+                            else  // stored by an earlier launch
+                                load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
+                            wave_sync();
+                        }
                         k = next;
                     }
+                    __syncthreads();  // wavefront 0's x is read before its next solve overwrites it
                 }
             }
         }
