@@ -1335,7 +1335,7 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
             g.gg.ac_end = (double*)g.d_ac->p;
             g.gg.ac_rec = (int*)(g.gg.ac_end + ni * runs * n2);
             g.gg.ac_state = g.gg.ac_rec + ni * (size_t)F;
-            HIPCHECK(hipMemset(g.gg.ac_rec, 0, ni * (size_t)F * sizeof(int)));
+            HIPCHECK(hipMemsetAsync(g.gg.ac_rec, 0, ni * (size_t)F * sizeof(int), b->stream));  // before pass 0
         }
         std::vector<double> omega(F);
         for (int64_t p = 0; p < F; p++) omega[p] = 2.0 * M_PI * freqs[p];  // AcState::omega, analysis.rs:801
