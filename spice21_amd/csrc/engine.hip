@@ -411,11 +411,15 @@ static int auto_ipw(int total_inst, int simds) {
 }
 
 // Derive per-instance parameters, build the topology and upload everything for one analysis.
-// A batch of at most two instances per SIMD leaves each SIMD at most two wavefronts: latency-bound, so
-// every shared slot becomes a hub and each device class is evaluated in one round instead of one per
-// colour (more LDS cells and a longer gather, fewer dependent rounds).  Not for AC, whose device
-// stamps are cheap: there the longer complex gather costs more than the rounds it saves (measured on
-// the 256-instance amplifier sweep: 5.5 ms without, 6.8 ms with, in mixed precision).
+// A latency-bound batch makes every shared slot a hub, so that each device class is evaluated in one
+// round instead of one per colour (more LDS cells and a longer gather, fewer dependent rounds): a batch
+// of at most 64 instances (one wavefront each at most; the single 5-stage ring: 14.2 -> 9.3 us per
+// iteration), and in mixed precision one of at most two instances per SIMD.  Measured: the 1024-ring
+// transient batch ran 137 ms per ns with hubs against 151 without in mixed precision, but 304 against
+// 278 in FP64 (whose exact-division factorisation is most of an iteration).  Not for AC, whose complex
+// gather costs more than the rounds it saves (the 256-instance amplifier sweep: 5.5 ms without, 6.8 ms
+// with, mixed).
+constexpr int LATENCY_INST = 64;
 constexpr int LATENCY_INST_PER_SIMD = 2;
 
 static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr, const AcInput* ac = nullptr) {
@@ -425,7 +429,8 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     int dev = 0, cus = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const bool latency = total <= (int64_t)LATENCY_INST_PER_SIMD * 4 * std::max(cus, 1);
+    const bool latency = total <= LATENCY_INST || (b->precision == S21_MIXED &&
+                                                   total <= (int64_t)LATENCY_INST_PER_SIMD * 4 * std::max(cus, 1));
     host_prepare(b->opts, g, tr, hp, ac, latency && !ac ? 1 : HUB_CONTRIB);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
