@@ -1217,6 +1217,307 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
     return st == RUNNING ? S21_CONVERGENCE : st;
 }
 
+// The AC sweep of newton_body: slot's instance (C.inst) for every frequency of run u (first pass), its
+// re-solve from the previous run's end (second pass, run u + 1), or its in-order continuation with u the
+// wavefront's look-ahead offset (third pass).  region0: the workgroup's first instance region.
+template <bool MIXED, class LU>
+__device__ inline void ac_sweep(Ctx& C, const Args& A, int g, int u, int slot, bool valid, LDS char* region0) {
+    const GroupGPU& G = *C.G;
+    const TopoGPU& T = C.T;
+    const int inst = C.inst;
+    LDS double* dbase = (LDS double*)(region0 + (size_t)slot * A.inst_bytes);
+    AcCtx Z_;
+    Z_.x = dbase;
+    Z_.c = Z_.x + 2 * T.n;
+    Z_.a = Z_.c + 2 * T.n;
+    Z_.rhs = Z_.a + 2 * T.nnz;
+    Z_.hub = Z_.rhs + 2 * T.n;
+    Z_.part = Z_.hub + 2 * T.n_hub;
+    Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
+    Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
+    Z_.nm = mos1_count(T);
+    {  // parameters read every iteration: LDS instead of global-memory latency per colour
+        const int nl = T.n_cls[CLS_LIN];
+        GLB const double* par = glb(G.par);
+        for (int q = valid ? C.sl : M1_CORE_F * Z_.nm; q < M1_CORE_F * Z_.nm; q += C.L) {
+            const int f = q / Z_.nm, j = q - f * Z_.nm;
+            Z_.par[q] = par[G.m1_off[f] + (long long)inst * G.m1_istr[f] + j];
+        }
+        for (int j = valid ? C.sl : nl; j < nl; j += C.L)
+            Z_.par[M1_CORE_F * Z_.nm + j] = par[G.lin_off[2] + (long long)inst * G.lin_istr[2] + j];
+        wave_sync();
+    }
+    C.x = Z_.x;
+    const int R = A.ac_runs;
+    const long long F = A.npts;
+    auto first_pt = [&](int r) { return F * r / R; };
+    GLB int* rec = glb(G.ac_rec) + (size_t)inst * F;
+    GLB double* wv = glb(G.wave) + (size_t)inst * F * 2 * T.n;
+    GLB double* ends = glb(G.ac_end) + (size_t)inst * R * 2 * T.n;
+    auto load_x = [&](bool act, GLB const double* src) {
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{src[2 * k], src[2 * k + 1]});
+        wave_sync();
+    };
+    auto store_x = [&](bool act, GLB double* dst) {
+        for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
+            const zd v = ldz(Z_.x, k);
+            dst[2 * k] = v.re;
+            dst[2 * k + 1] = v.im;
+        }
+    };
+    // Loops run in step across the workgroup when one lane per instance factors (its barriers).
+    auto any = [&](bool p) {
+        if constexpr (LU::lane_per_instance)
+            return __syncthreads_or(p) != 0;
+        else
+            return wave_any(p);
+    };
+    long long iters = 0;
+    // Newton at frequency pt from the current x; records the point and returns its status.
+    auto point = [&](bool act, long long pt, int& its) {
+        const long long i0 = iters;
+        const double om = act ? A.omega[pt] : 0.0;
+        int s2;
+        if constexpr (LU::lane_per_instance) {
+            __shared__ int s_act[WAVE * MAX_WAVES];
+            s2 = newton_ac_wg<MIXED, LU>(C, Z_, act, om, iters, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
+                                         A.inst_bytes, g);
+        } else {
+            s2 = newton_ac<MIXED>(C, Z_, act, om, iters);
+        }
+        its = (int)(iters - i0);
+        if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
+        if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
+        return s2;
+    };
+    // Any frequency of [p, e) failed or unsolved (the lanes of the segment share the scan).
+    auto failed_in = [&](long long p, long long e) {
+        bool f = false;
+        for (long long q = p + C.sl; q < e; q += C.L) f = f || rec[q] == 0 || (rec[q] >> 8) != S21_OK;
+        return seg_any(C, f);
+    };
+    // A frequency solved (at least one full Newton step) on both paths: both hold its unique solution.
+    auto meets = [&](int its, int old) {
+        return its >= 2 && old != 0 && (old >> 8) == S21_OK && (old & 0xFF) >= 2;
+    };
+    if (A.ac_pass == 0) {  // run u from x = 0 (analysis.rs:215-233)
+        bool alive = valid;
+        for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});
+        wave_sync();
+        const long long p0 = first_pt(u), p1 = first_pt(u + 1);
+        for (long long pt = p0;; pt++) {
+            const bool here = alive && pt < p1;
+            if (!any(here)) break;
+            int its;
+            const int s2 = point(here, pt, its);
+            if (here) alive = s2 == S21_OK;
+        }
+        store_x(valid, ends + (size_t)u * 2 * T.n);
+    } else if (A.ac_pass == 1) {  // run r from the first pass's end of run r - 1, until the paths meet
+        const int r = u + 1;
+        bool live = valid;
+        load_x(live, ends + (size_t)(r - 1) * 2 * T.n);
+        int state = AC_UNMET;
+        const long long p0 = first_pt(r), p1 = first_pt(r + 1);
+        for (long long pt = p0;; pt++) {
+            const bool here = live && pt < p1;
+            if (!any(here)) break;
+            const int old = here ? rec[pt] : 0;
+            int its;
+            const int s2 = point(here, pt, its);
+            if (here && s2 != S21_OK) {
+                state = AC_FAILED;
+                live = false;
+            } else if (here && meets(its, old)) {
+                state = AC_MET;
+                live = false;
+            }
+        }
+        if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
+    } else {
+        // In order from run rs, jumping to the next run wherever the paths meet.  The A.ac_look
+        // wavefronts of the workgroup serve one instance: from the current solution x, wavefront w
+        // runs iteration 0 of frequency k + w (assembly and residual: the reference keeps x when
+        // every |res| < 1e-9, since dx starts at zero); the first frequency that does not keep x
+        // is solved by wavefront 0, and the ones before it keep x.
+        __shared__ int s_flag[MAX_WAVES];
+        __shared__ long long s_out[4];
+        const int w = u;                       // this wavefront's offset
+        const int lead = slot - w;             // the instance's first slot in the workgroup
+        LDS double* x_lead = (LDS double*)(region0 + (size_t)lead * A.inst_bytes);
+        const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
+        bool live = valid && rs > 0;
+        long long k = live ? first_pt(rs) : 1;
+        load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
+        // The workgroup is this instance's A.ac_look wavefronts, so live is the same in all of them.
+        while (__syncthreads_or(live)) {
+            const long long kw = k + w;
+            const bool test = live && kw < F;
+            bool keep = false;
+            if (wave_any(test)) {
+                assemble_ac(C, Z_, test, test ? A.omega[kw] : 0.0);
+                wave_sync();
+                const bool bad = seg_any(C, residual_ac(C, Z_, test));
+                wave_sync();
+                keep = test && !bad;
+            }
+            if (C.sl == 0) s_flag[slot] = keep ? 1 : 0;
+            __syncthreads();
+            int ws = A.ac_look;  // first offset that does not keep x
+            for (int j = 0; j < A.ac_look; j++)
+                if (!s_flag[lead + j]) {
+                    ws = j;
+                    break;
+                }
+            if (live && w < ws) {  // kept: x stands, one iteration
+                if (C.sl == 0) rec[kw] = (S21_OK << 8) | 1;
+                store_x(true, wv + (size_t)kw * 2 * T.n);
+            }
+            const long long ks = k + ws;
+            const bool solve = live && ws < A.ac_look && ks < F;
+            const int old = solve && w == 0 ? rec[ks] : 0;  // before wavefront 0 records ks
+            int its = 0;
+            const int s2 = point(solve && w == 0, ks, its);
+            if (solve && w == 0 && C.sl == 0) {
+                s_out[0] = s2;
+                s_out[1] = its;
+                s_out[2] = old;
+            }
+            __syncthreads();
+            if (live) {
+                long long next = ks + 1;
+                if (!solve) {
+                    next = ks;  // every tested frequency kept x (or the sweep ended)
+                } else if (s_out[0] != S21_OK) {
+                    live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
+                } else if (meets((int)s_out[1], (int)s_out[2])) {
+                    int rr = (int)((ks * R) / F);
+                    while (first_pt(rr + 1) <= ks) rr++;
+                    while (first_pt(rr) > ks) rr--;
+                    // The rest of run rr is stored data of a path through frequency ks's solution:
+                    // the sweep's (a failure there is the sweep's).
+                    bool bad = failed_in(ks + 1, first_pt(rr + 1));
+                    int q = rr + 1;
+                    // Run rr ends as the first pass ended it unless its re-solve never met that
+                    // pass; then every following run whose re-solve met is the sweep's, and so is a
+                    // run whose re-solve ran to its end (its start was the sweep's) or failed (the
+                    // sweep fails there).
+                    if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
+                        while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
+                            bad = failed_in(first_pt(q), first_pt(q + 1));
+                            q++;
+                        }
+                        if (!bad && q < R) {
+                            if (G.ac_state[(size_t)inst * R + q] == AC_FAILED)
+                                bad = true;
+                            else
+                                q++;
+                        }
+                    }
+                    next = first_pt(q);
+                    if (bad) live = false;
+                }
+                if (next >= F) live = false;
+                if (live && solve) {  // every wavefront continues from the sweep's solution at next - 1
+                    if (next - 1 == ks)
+                        for (int j = C.sl; j < 2 * T.n; j += C.L) Z_.x[j] = x_lead[j];
+                    else  // stored by an earlier launch
+                        load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
+                    wave_sync();
+                }
+                k = next;
+            }
+            __syncthreads();  // wavefront 0's x is read before its next solve overwrites it
+        }
+    }
+}
+
+// The operating point and transient of newton_body for slot's instance (C.inst).
+template <bool MIXED, class LU>
+__device__ inline void op_tran(Ctx& C, const Args& A, int g, int slot, bool valid, LDS char* region0) {
+    const GroupGPU& G = *C.G;
+    const TopoGPU& T = C.T;
+    const int inst = C.inst;
+    LDS double* dbase = (LDS double*)(region0 + (size_t)slot * A.inst_bytes);
+    C.x = dbase;
+    C.c = C.x + T.n;
+    C.a = C.c + T.n;
+    C.rhs = C.a + T.nnz;
+    C.hub = C.rhs + T.n;
+    C.part = C.hub + T.n_hub;
+    C.lu32 = (LDS float*)(C.part + T.n_part);
+
+    for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) C.x[k] = G.x[(size_t)inst * T.n + k];
+    wave_sync();
+
+    if (A.mode == MODE_ASSEMBLE) {
+        assemble(C, valid, A.asm_tran != 0, A);
+        wave_sync();
+        for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
+        return;
+    }
+
+    long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const long long tstart = A.prof ? cycles() : 0;
+    long long iters = 0;
+    int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
+    if (A.resume && valid) {
+        iters = G.iters[inst];
+        pt0 = G.resume_at[2 * inst];
+        used0 = G.resume_at[2 * inst + 1];
+    }
+    int last_pt = 0;
+    long long last_start = 0;  // iteration count when the last solve began
+    auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
+        if constexpr (LU::lane_per_instance) {
+            __shared__ int s_act[WAVE * MAX_WAVES];
+            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
+                                        A.waves * A.ipw, region0, A.inst_bytes, g);
+        } else {
+            return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
+        }
+    };
+    last_start = iters - used0;
+    const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
+    int st = solve(valid && op_here, false, A.max_iter - (A.resume ? used0 : 0), !A.resume);
+    if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
+    if (A.mode == MODE_TRAN) {
+        bool alive = valid && st == S21_OK;
+        double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
+        for (int k = alive && op_here ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
+        // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
+        for (long long pt = 1; pt < A.npts; pt++) {
+            if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
+            const bool here = alive && pt >= pt0;   // a resumed instance joins at its own point
+            const bool cont = A.resume && pt == pt0;  // ... with the solve it continues
+            if (here) {
+                last_pt = (int)pt;
+                last_start = cont ? iters - used0 : iters;
+            }
+            const int s2 = solve(here, true, cont ? A.max_iter - used0 : A.max_iter, !cont);
+            if (here) {
+                st = s2;
+                alive = s2 == S21_OK;
+            }
+            for (int k = here && alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
+        }
+    }
+    if (!valid) return;
+    for (int k = C.sl; k < T.n; k += C.L) G.x[(size_t)inst * T.n + k] = C.x[k];
+    if (C.sl == 0) {
+        G.status[inst] = st;
+        G.iters[inst] = iters;
+        if (G.resume_at) {
+            G.resume_at[2 * inst] = last_pt;
+            G.resume_at[2 * inst + 1] = (int)(iters - last_start);
+        }
+        if (A.prof && G.prof) {
+            pc[7] = cycles() - tstart;
+            for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
+        }
+    }
+}
+
 // Workgroup = A.waves wavefronts x A.ipw instances, sharing one LDS copy of the group's records.
 template <bool MIXED, class LU>
 __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_groups, const Args& A) {
@@ -1276,298 +1577,13 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     C.I.col = b16 + T.o_col;
     C.I.dev = b16 + T.o_dev;
     C.I.code = b16 + T.o_code;
-    LDS double* dbase = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) + (size_t)slot * A.inst_bytes);
-    if (A.mode == MODE_AC) {  // complex: x | c | A | RHS | hub | partials (| complex-float LU) | parameters
-        if constexpr (LU::kind != 1) {
-            AcCtx Z_;
-            Z_.x = dbase;
-            Z_.c = Z_.x + 2 * T.n;
-            Z_.a = Z_.c + 2 * T.n;
-            Z_.rhs = Z_.a + 2 * T.nnz;
-            Z_.hub = Z_.rhs + 2 * T.n;
-            Z_.part = Z_.hub + 2 * T.n_hub;
-            Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
-            Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
-            Z_.nm = mos1_count(T);
-            {  // parameters read every iteration: LDS instead of global-memory latency per colour
-                const int nl = T.n_cls[CLS_LIN];
-                GLB const double* par = glb(G.par);
-                for (int q = valid ? C.sl : M1_CORE_F * Z_.nm; q < M1_CORE_F * Z_.nm; q += C.L) {
-                    const int f = q / Z_.nm, j = q - f * Z_.nm;
-                    Z_.par[q] = par[G.m1_off[f] + (long long)inst * G.m1_istr[f] + j];
-                }
-                for (int j = valid ? C.sl : nl; j < nl; j += C.L)
-                    Z_.par[M1_CORE_F * Z_.nm + j] = par[G.lin_off[2] + (long long)inst * G.lin_istr[2] + j];
-                wave_sync();
-            }
-            C.x = Z_.x;
-            const int R = A.ac_runs;
-            const long long F = A.npts;
-            const int u = valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0;
-            auto first_pt = [&](int r) { return F * r / R; };
-            GLB int* rec = glb(G.ac_rec) + (size_t)inst * F;
-            GLB double* wv = glb(G.wave) + (size_t)inst * F * 2 * T.n;
-            GLB double* ends = glb(G.ac_end) + (size_t)inst * R * 2 * T.n;
-            auto load_x = [&](bool act, GLB const double* src) {
-                for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{src[2 * k], src[2 * k + 1]});
-                wave_sync();
-            };
-            auto store_x = [&](bool act, GLB double* dst) {
-                for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
-                    const zd v = ldz(Z_.x, k);
-                    dst[2 * k] = v.re;
-                    dst[2 * k + 1] = v.im;
-                }
-            };
-            // Loops run in step across the workgroup when one lane per instance factors (its barriers).
-            auto any = [&](bool p) {
-                if constexpr (LU::lane_per_instance)
-                    return __syncthreads_or(p) != 0;
-                else
-                    return wave_any(p);
-            };
-            long long iters = 0;
-            // Newton at frequency pt from the current x; records the point and returns its status.
-            auto point = [&](bool act, long long pt, int& its) {
-                const long long i0 = iters;
-                const double om = act ? A.omega[pt] : 0.0;
-                int s2;
-                if constexpr (LU::lane_per_instance) {
-                    __shared__ int s_act[WAVE * MAX_WAVES];
-                    LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
-                    s2 = newton_ac_wg<MIXED, LU>(C, Z_, act, om, iters, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
-                                                 A.inst_bytes, g);
-                } else {
-                    s2 = newton_ac<MIXED>(C, Z_, act, om, iters);
-                }
-                its = (int)(iters - i0);
-                if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
-                if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
-                return s2;
-            };
-            // Any frequency of [p, e) failed or unsolved (the lanes of the segment share the scan).
-            auto failed_in = [&](long long p, long long e) {
-                bool f = false;
-                for (long long q = p + C.sl; q < e; q += C.L) f = f || rec[q] == 0 || (rec[q] >> 8) != S21_OK;
-                return seg_any(C, f);
-            };
-            // A frequency solved (at least one full Newton step) on both paths: both hold its unique solution.
-            auto meets = [&](int its, int old) {
-                return its >= 2 && old != 0 && (old >> 8) == S21_OK && (old & 0xFF) >= 2;
-            };
-            if (A.ac_pass == 0) {  // run u from x = 0 (analysis.rs:215-233)
-                bool alive = valid;
-                for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.x, k, zd{0.0, 0.0});
-                wave_sync();
-                const long long p0 = first_pt(u), p1 = first_pt(u + 1);
-                for (long long pt = p0;; pt++) {
-                    const bool here = alive && pt < p1;
-                    if (!any(here)) break;
-                    int its;
-                    const int s2 = point(here, pt, its);
-                    if (here) alive = s2 == S21_OK;
-                }
-                store_x(valid, ends + (size_t)u * 2 * T.n);
-            } else if (A.ac_pass == 1) {  // run r from the first pass's end of run r - 1, until the paths meet
-                const int r = u + 1;
-                bool live = valid;
-                load_x(live, ends + (size_t)(r - 1) * 2 * T.n);
-                int state = AC_UNMET;
-                const long long p0 = first_pt(r), p1 = first_pt(r + 1);
-                for (long long pt = p0;; pt++) {
-                    const bool here = live && pt < p1;
-                    if (!any(here)) break;
-                    const int old = here ? rec[pt] : 0;
-                    int its;
-                    const int s2 = point(here, pt, its);
-                    if (here && s2 != S21_OK) {
-                        state = AC_FAILED;
-                        live = false;
-                    } else if (here && meets(its, old)) {
-                        state = AC_MET;
-                        live = false;
-                    }
-                }
-                if (valid && C.sl == 0) G.ac_state[(size_t)inst * R + r] = state;
-            } else {
-                // In order from run rs, jumping to the next run wherever the paths meet.  The A.ac_look
-                // wavefronts of the workgroup serve one instance: from the current solution x, wavefront w
-                // runs iteration 0 of frequency k + w (assembly and residual: the reference keeps x when
-                // every |res| < 1e-9, since dx starts at zero); the first frequency that does not keep x
-                // is solved by wavefront 0, and the ones before it keep x.
-                __shared__ int s_flag[MAX_WAVES];
-                __shared__ long long s_out[4];
-                const int w = u;                       // this wavefront's offset
-                const int lead = slot - w;             // the instance's first slot in the workgroup
-                LDS double* x_lead = (LDS double*)((LDS char*)((LDS uint4*)smem_u4 + T.blob_vec) +
-                                                   (size_t)lead * A.inst_bytes);
-                const int rs = valid ? G.ac_state[(size_t)inst * R] : 0;
-                bool live = valid && rs > 0;
-                long long k = live ? first_pt(rs) : 1;
-                load_x(live, wv + (size_t)(k - 1) * 2 * T.n);
-                // The workgroup is this instance's A.ac_look wavefronts, so live is the same in all of them.
-                while (__syncthreads_or(live)) {
-                    const long long kw = k + w;
-                    const bool test = live && kw < F;
-                    bool keep = false;
-                    if (wave_any(test)) {
-                        assemble_ac(C, Z_, test, test ? A.omega[kw] : 0.0);
-                        wave_sync();
-                        const bool bad = seg_any(C, residual_ac(C, Z_, test));
-                        wave_sync();
-                        keep = test && !bad;
-                    }
-                    if (C.sl == 0) s_flag[slot] = keep ? 1 : 0;
-                    __syncthreads();
-                    int ws = A.ac_look;  // first offset that does not keep x
-                    for (int j = 0; j < A.ac_look; j++)
-                        if (!s_flag[lead + j]) {
-                            ws = j;
-                            break;
-                        }
-                    if (live && w < ws) {  // kept: x stands, one iteration
-                        if (C.sl == 0) rec[kw] = (S21_OK << 8) | 1;
-                        store_x(true, wv + (size_t)kw * 2 * T.n);
-                    }
-                    const long long ks = k + ws;
-                    const bool solve = live && ws < A.ac_look && ks < F;
-                    const int old = solve && w == 0 ? rec[ks] : 0;  // before wavefront 0 records ks
-                    int its = 0;
-                    const int s2 = point(solve && w == 0, ks, its);
-                    if (solve && w == 0 && C.sl == 0) {
-                        s_out[0] = s2;
-                        s_out[1] = its;
-                        s_out[2] = old;
-                    }
-                    __syncthreads();
-                    if (live) {
-                        long long next = ks + 1;
-                        if (!solve) {
-                            next = ks;  // every tested frequency kept x (or the sweep ended)
-                        } else if (s_out[0] != S21_OK) {
-                            live = false;  // the sweep stops at a failed frequency (analysis.rs:801)
-                        } else if (meets((int)s_out[1], (int)s_out[2])) {
-                            int rr = (int)((ks * R) / F);
-                            while (first_pt(rr + 1) <= ks) rr++;
-                            while (first_pt(rr) > ks) rr--;
-                            // The rest of run rr is stored data of a path through frequency ks's solution:
-                            // the sweep's (a failure there is the sweep's).
-                            bool bad = failed_in(ks + 1, first_pt(rr + 1));
-                            int q = rr + 1;
-                            // Run rr ends as the first pass ended it unless its re-solve never met that
-                            // pass; then every following run whose re-solve met is the sweep's, and so is a
-                            // run whose re-solve ran to its end (its start was the sweep's) or failed (the
-                            // sweep fails there).
-                            if (rr == 0 || G.ac_state[(size_t)inst * R + rr] != AC_UNMET) {
-                                while (!bad && q < R && G.ac_state[(size_t)inst * R + q] == AC_MET) {
-                                    bad = failed_in(first_pt(q), first_pt(q + 1));
-                                    q++;
-                                }
-                                if (!bad && q < R) {
-                                    if (G.ac_state[(size_t)inst * R + q] == AC_FAILED)
-                                        bad = true;
-                                    else
-                                        q++;
-                                }
-                            }
-                            next = first_pt(q);
-                            if (bad) live = false;
-                        }
-                        if (next >= F) live = false;
-                        if (live && solve) {  // every wavefront continues from the sweep's solution at next - 1
-                            if (next - 1 == ks)
-                                for (int j = C.sl; j < 2 * T.n; j += C.L) Z_.x[j] = x_lead[j];
-                            else  // stored by an earlier launch
-                                load_x(true, wv + (size_t)(next - 1) * 2 * T.n);
-                            wave_sync();
-                        }
-                        k = next;
-                    }
-                    __syncthreads();  // wavefront 0's x is read before its next solve overwrites it
-                }
-            }
-        }
-        return;
-    }
-    if constexpr (LU::kind != 2) {
-        C.x = dbase;
-        C.c = C.x + T.n;
-        C.a = C.c + T.n;
-        C.rhs = C.a + T.nnz;
-        C.hub = C.rhs + T.n;
-        C.part = C.hub + T.n_hub;
-        C.lu32 = (LDS float*)(C.part + T.n_part);
-
-        for (int k = valid ? C.sl : T.n; k < T.n; k += C.L) C.x[k] = G.x[(size_t)inst * T.n + k];
-        wave_sync();
-
-        if (A.mode == MODE_ASSEMBLE) {
-            assemble(C, valid, A.asm_tran != 0, A);
-            wave_sync();
-            for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
-            return;
-        }
-
-        long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const long long tstart = A.prof ? cycles() : 0;
-        long long iters = 0;
-        int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
-        if (A.resume && valid) {
-            iters = G.iters[inst];
-            pt0 = G.resume_at[2 * inst];
-            used0 = G.resume_at[2 * inst + 1];
-        }
-        int last_pt = 0;
-        long long last_start = 0;  // iteration count when the last solve began
-        LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
-        auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
-            if constexpr (LU::lane_per_instance) {
-                __shared__ int s_act[WAVE * MAX_WAVES];
-                return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
-                                            A.waves * A.ipw, region0, A.inst_bytes, g);
-            } else {
-                return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
-            }
-        };
-        last_start = iters - used0;
-        const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
-        int st = solve(valid && op_here, false, A.max_iter - (A.resume ? used0 : 0), !A.resume);
-        if (A.capture && G.acop) capture_acop(C, valid && st == S21_OK, A);
-        if (A.mode == MODE_TRAN) {
-            bool alive = valid && st == S21_OK;
-            double* wv = G.wave + (size_t)inst * (size_t)A.npts * T.n;
-            for (int k = alive && op_here ? C.sl : T.n; k < T.n; k += C.L) wv[k] = C.x[k];
-            // t = tstep, 2 tstep, ... while t < tstop (analysis.rs:553-570)
-            for (long long pt = 1; pt < A.npts; pt++) {
-                if (LU::lane_per_instance ? !__syncthreads_or(alive) : !wave_any(alive)) break;
-                const bool here = alive && pt >= pt0;   // a resumed instance joins at its own point
-                const bool cont = A.resume && pt == pt0;  // ... with the solve it continues
-                if (here) {
-                    last_pt = (int)pt;
-                    last_start = cont ? iters - used0 : iters;
-                }
-                const int s2 = solve(here, true, cont ? A.max_iter - used0 : A.max_iter, !cont);
-                if (here) {
-                    st = s2;
-                    alive = s2 == S21_OK;
-                }
-                for (int k = here && alive ? C.sl : T.n; k < T.n; k += C.L) wv[(size_t)pt * T.n + k] = C.x[k];
-            }
-        }
-        if (!valid) return;
-        for (int k = C.sl; k < T.n; k += C.L) G.x[(size_t)inst * T.n + k] = C.x[k];
-        if (C.sl == 0) {
-            G.status[inst] = st;
-            G.iters[inst] = iters;
-            if (G.resume_at) {
-                G.resume_at[2 * inst] = last_pt;
-                G.resume_at[2 * inst + 1] = (int)(iters - last_start);
-            }
-            if (A.prof && G.prof) {
-                pc[7] = cycles() - tstart;
-                for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
-            }
-        }
+    LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
+    if (A.mode == MODE_AC) {
+        if constexpr (LU::kind != 1)
+            ac_sweep<MIXED, LU>(C, A, g, valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0, slot,
+                                valid, region0);
+    } else {
+        if constexpr (LU::kind != 2) op_tran<MIXED, LU>(C, A, g, slot, valid, region0);
     }
 }
 
