@@ -115,6 +115,31 @@ def test_chain_mc_batch_matches_oracle_per_instance():
     assert np.max(np.abs(Xm[:, nv] - X[:, nv])) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_results_bit_identical_across_runs_and_launch_shapes(precision):
+    """Stamps are summed in a fixed order (slot codes, hub cells) and each instance is factored by its own
+    lanes, so neither a repeated call nor another launch shape (instances per wavefront, wavefronts per
+    workgroup) changes a single bit of the solution or an iteration count; nor does a repeated AC sweep."""
+    nl, over = W.chain_mc(n_inst=512)
+    g = Group(nl.to_c(), 512, over)
+    ref = None
+    for ipw in (2, 1, 4, 2):
+        log("determinism", precision, "ipw", ipw)
+        b = Batch(g, precision=precision).set_ipw(ipw).op()
+        x, (st, it) = b.x(), b.status()
+        assert np.all(st == 0)
+        if ref is None:
+            ref = (x, it)
+        else:
+            assert np.array_equal(x, ref[0]) and np.array_equal(it, ref[1]), ipw
+    nl, over = W.amp_ladder(16)
+    ga = Group(nl.to_c(), 16, over)
+    a = Batch(ga, precision=precision).ac(1000, 100_000_000_000, 300)
+    c = Batch(ga, precision=precision).ac(1000, 100_000_000_000, 300)
+    for i in range(16):
+        assert np.array_equal(a.wave(0, i), c.wave(0, i)), i
+
+
 def test_mixed_tran_close_to_fp64():
     cn = ro3("cmos").to_c()
     a = Batch(Group(cn)).tran(1e-11, 1e-8, [("1", 0.0)]).wave()
