@@ -187,6 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-inst", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the CPU baseline (median reported)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the instances-per-wavefront autotuning before the timed steps")
@@ -290,13 +291,17 @@ def main():
         from oracle import oracle_py as O  # the CPU restatement, timed as the baseline and used as the checker
 
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        t0 = time.perf_counter()
-        xo, ito, sto = O.dcop_batch(group, threads=threads)
-        cpu_s = time.perf_counter() - t0
+        walls = []
+        for _ in range(args.cpu_reps):  # the median of a few repetitions: a rate steadier than one pass
+            t0 = time.perf_counter()
+            xo, ito, sto = O.dcop_batch(group, threads=threads)
+            walls.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(walls))
         line["cpu_baseline"] = {
             "value": float(ito.sum()) / cpu_s, "unit": "instance-iters/s", "cores": threads, "kind": "port",
-            "sample": f"all {args.n_inst} instances, one dcop each ({int(ito.sum())} Newton iterations, "
-                      f"{cpu_s:.2f} s wall on {threads} threads)"}
+            "sample": f"all {args.n_inst} instances, one dcop each ({int(ito.sum())} Newton iterations), "
+                      f"{len(walls)} times on {threads} threads: median {cpu_s:.2f} s wall "
+                      f"(about {sum(walls) * threads:.0f} s of CPU work in all)"}
         if "fp64" in res:
             xf = res["fp64"]["x"]
             rel = np.abs(xf - xo) / np.maximum(np.abs(xo), 1e-9)
