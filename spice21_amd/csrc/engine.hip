@@ -570,9 +570,11 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     const size_t vw = ac ? 2 : 1;  // values per entry: complex for AC
     g.inst_bytes = (size_t)(3 * t.n + t.nnz + t.n_hub + t.n_part) * sizeof(double) * vw +
                    (b->precision == S21_MIXED ? (size_t)(t.nnz + t.n) * sizeof(float) * vw : 0);
-    if (ac)  // the instance's AC parameters, cached in LDS by the kernel
+    if (ac) {  // the instance's AC parameters, cached in LDS by the kernel; in FP64 the copy of A it factors
         g.inst_bytes += (size_t)(M1_CORE_F * (t.cls_start[CLS_MOS1T + 1] - t.cls_start[CLS_MOS1]) +
                                  (t.cls_start[CLS_LIN + 1] - t.cls_start[CLS_LIN])) * sizeof(double);
+        if (b->precision != S21_MIXED) g.inst_bytes += (size_t)t.nnz * 2 * sizeof(double);
+    }
     g.gg.acop = nullptr;
 }
 

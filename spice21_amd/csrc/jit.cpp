@@ -254,6 +254,19 @@ void gen_cx(std::ostringstream& o, const Topology& t, const std::string& fn, con
     o << "    bool z = false;\n";
     for (int i = 0; i + 1 < t.n; i++) o << "    z = z || zzero(ldz(lu, " << t.diag[i] << "));\n";
     o << "    return z;\n}\n";
+    // The forward sweep alone, on the factors lu_gen left (later Newton iterations at one frequency).
+    o << "template <typename R>\n" << fn << " void fw_gen" << sfx
+      << "(LDS const R* __restrict__ lu, LDS R* __restrict__ c) {\n"
+         "    using V = Z<R>;\n"
+         "    V y, ck;\n";
+    for (int i = 0; i < t.n; i++) {
+        if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;
+        o << "    y = ldz(c, " << i << ");";
+        for (int q = t.rp[i]; q < t.diag[i]; q++)
+            o << " ck = ldz(c, " << t.ci[q] << "); if (!zzero(ck)) y = y - ck * ldz(lu, " << q << ");";
+        o << " stz(c, " << i << ", y);\n";
+    }
+    o << "}\n";
     o << "template <typename R>\n" << fn << " void bw_gen" << sfx
       << "(LDS const R* __restrict__ lu, LDS R* __restrict__ c) {\n"
          "    using V = Z<R>;\n"
@@ -310,8 +323,12 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
          "    template <typename V>\n"
          "    static __device__ void backward_lane(int g, LDS const V* lu, LDS V* c) {\n"
       << dispatch("", "bw_gen")
-      << "    }\n"
-         "};\n}  // namespace s21\n";
+      << "    }\n";
+    if (complex)
+        o << "    template <typename V>\n"
+             "    static __device__ void forward_lane(int g, LDS const V* lu, LDS V* c) {\n"
+          << dispatch("", "fw_gen") << "    }\n";
+    o << "};\n}  // namespace s21\n";
     o << "extern \"C\" __global__ __launch_bounds__(s21::WAVE * s21::MAX_WAVES, " << waves_per_simd
       << ") void s21_newton_jit("
          "const s21::GroupGPU* __restrict__ groups, int n_groups, s21::Args A) {\n"

@@ -767,7 +767,8 @@ __device__ inline zd zwide(V a) { return zd{(double)a.re, (double)a.im}; }
 // Complex arrays live in LDS as interleaved (re, im) pairs.
 struct AcCtx {
     LDS double *x, *c, *a, *rhs, *hub, *part;
-    LDS float* lu32;
+    LDS float* lu32;   // mixed precision: complex64 copy of A, factored, then the residual's copy
+    LDS double* lu64;  // FP64: copy of A, factored (A itself stays for the residual)
     LDS double* par;  // the instance's AC parameters: M1_CORE_F fields of each Mos1, then each linear value
     int nm;           // Mos1 devices
 };
@@ -978,6 +979,29 @@ __device__ inline bool factor_forward_ac(const Ctx& C, const AcCtx& Z_, bool act
     return zero_pivot;
 }
 
+// The forward sweep of factor_forward_ac alone, on factors it has left in lu: the same records and
+// operations, so a matrix factored once serves every Newton iteration at one frequency (the AC matrix
+// does not depend on x).
+template <typename R>
+__device__ inline void forward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS const R* lu) {
+    using V = Z<R>;
+    const Idx& I = C.I;
+    for (int l = 0; l < C.T.n_clvl; l++) {
+        const int e = I.clvl[l + 1];
+        for (int ch = act ? I.clvl[l] + C.sl : e; ch < e; ch += C.L) {
+            const R4 h = r4(I.fch[ch]);
+            V y = zcvt<V>(ldz(Z_.c, h.x));
+            for (int j = h.y; j < h.z; j++) {
+                const R4 L = r4(I.fl[j]);
+                const V ck = zcvt<V>(ldz(Z_.c, L.z));
+                if (!zzero(ck)) y = y - ck * ldz(lu, L.x);
+            }
+            stz(Z_.c, h.x, zwide(y));
+        }
+        wave_sync();
+    }
+}
+
 template <typename R>
 __device__ inline void backward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS const R* lu) {
     using V = Z<R>;
@@ -998,7 +1022,10 @@ __device__ inline void backward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS 
 }
 
 // Complex Newton at one frequency (Solver<Complex>::solve, analysis.rs:253-304): at most 20 iterations,
-// converged when every |dx| < 1e-3 and every |res| < 1e-9, |dx| clamped to 1.
+// converged when every |dx| < 1e-3 and every |res| < 1e-9, |dx| clamped to 1.  The reference assembles
+// and factors at every iteration; the AC matrix and right-hand side do not depend on x, so every
+// iteration would assemble the same values and factor them into the same LU: the first iteration does
+// both (the factors into a copy of A, which the residual keeps reading) and later iterations reuse them.
 template <bool MIXED>
 __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, long long& iters) {
     const TopoGPU& T = C.T;
@@ -1007,7 +1034,7 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
     for (int it = 0; it < 20; it++) {
         if (!wave_any(act)) break;
         if (act) iters++;
-        assemble_ac(C, Z_, act, omega);
+        if (it == 0) assemble_ac(C, Z_, act, omega);
         wave_sync();
         const bool bad = seg_any(C, residual_ac(C, Z_, act));
         wave_sync();
@@ -1021,13 +1048,21 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
             act = false;
         }
         if (!wave_any(act)) break;
-        bool zero;
-        if (MIXED) {
-            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
-            wave_sync();
-            zero = seg_any(C, factor_forward_ac<float>(C, Z_, act, Z_.lu32));
+        bool zero = false;
+        if (it == 0) {
+            if (MIXED) {
+                for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
+                wave_sync();
+                zero = seg_any(C, factor_forward_ac<float>(C, Z_, act, Z_.lu32));
+            } else {
+                for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu64, p, ldz(Z_.a, p));
+                wave_sync();
+                zero = seg_any(C, factor_forward_ac<double>(C, Z_, act, Z_.lu64));
+            }
+        } else if (MIXED) {
+            forward_ac<float>(C, Z_, act, Z_.lu32);
         } else {
-            zero = seg_any(C, factor_forward_ac<double>(C, Z_, act, Z_.a));
+            forward_ac<double>(C, Z_, act, Z_.lu64);
         }
         if (act && zero) {
             st = S21_PIVOT;
@@ -1036,7 +1071,7 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
         if (MIXED)
             backward_ac<float>(C, Z_, act, Z_.lu32);
         else
-            backward_ac<double>(C, Z_, act, Z_.a);
+            backward_ac<double>(C, Z_, act, Z_.lu64);
         double mx = 0.0;
         for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
             const double d = znorm(ldz(Z_.c, C.I.ce2i[k]));
@@ -1068,7 +1103,7 @@ __device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omeg
     for (int it = 0; it < 20; it++) {
         if (!__syncthreads_or(act)) break;
         if (act) iters++;
-        assemble_ac(C, Z_, act, omega);
+        if (it == 0) assemble_ac(C, Z_, act, omega);  // the same matrix at every iteration (newton_ac)
         wave_sync();
         const bool bad = seg_any(C, residual_ac(C, Z_, act));
         wave_sync();
@@ -1082,8 +1117,11 @@ __device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omeg
             act = false;
         }
         if (MIXED) {
-            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
+            if (it == 0)
+                for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu32, p, zcvt<zf>(ldz(Z_.a, p)));
             for (int k = act ? C.sl : T.n; k < T.n; k += C.L) stz(Z_.lu32 + 2 * T.nnz, k, zcvt<zf>(ldz(Z_.c, k)));
+        } else if (it == 0) {
+            for (int p = act ? C.sl : T.nnz; p < T.nnz; p += C.L) stz(Z_.lu64, p, ldz(Z_.a, p));
         }
         if (C.sl == 0) s_act[slot] = act ? 1 : 0;
         __syncthreads();
@@ -1091,14 +1129,21 @@ __device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omeg
             if (!s_act[sl]) continue;
             LDS double* c = (LDS double*)(region0 + (size_t)sl * stride) + 2 * T.n;
             LDS double* a = c + 2 * T.n;
-            bool z;
+            LDS double* lu_end = a + 2 * (T.nnz + T.n + T.n_hub + T.n_part);  // the LU copy follows them
+            bool z = false;
             if (MIXED) {
-                LDS float* lu = (LDS float*)(a + 2 * (T.nnz + T.n + T.n_hub + T.n_part));
-                z = LU::template factor_lane<float>(gidx, lu, lu + 2 * T.nnz);
+                LDS float* lu = (LDS float*)lu_end;
+                if (it == 0)
+                    z = LU::template factor_lane<float>(gidx, lu, lu + 2 * T.nnz);
+                else
+                    LU::template forward_lane<float>(gidx, lu, lu + 2 * T.nnz);
                 if (!z) LU::template backward_lane<float>(gidx, lu, lu + 2 * T.nnz);
             } else {
-                z = LU::template factor_lane<double>(gidx, a, c);
-                if (!z) LU::template backward_lane<double>(gidx, a, c);
+                if (it == 0)
+                    z = LU::template factor_lane<double>(gidx, lu_end, c);
+                else
+                    LU::template forward_lane<double>(gidx, lu_end, c);
+                if (!z) LU::template backward_lane<double>(gidx, lu_end, c);
             }
             s_act[sl] = z ? 2 : 1;
         }
@@ -1234,7 +1279,8 @@ __device__ inline void ac_sweep(Ctx& C, const Args& A, int g, int u, int slot, b
     Z_.hub = Z_.rhs + 2 * T.n;
     Z_.part = Z_.hub + 2 * T.n_hub;
     Z_.lu32 = (LDS float*)(Z_.part + 2 * T.n_part);
-    Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.part + 2 * T.n_part;
+    Z_.lu64 = Z_.part + 2 * T.n_part;  // FP64: where the complex64 copies would be
+    Z_.par = MIXED ? (LDS double*)(Z_.lu32 + 2 * (T.nnz + T.n)) : Z_.lu64 + 2 * T.nnz;
     Z_.nm = mos1_count(T);
     {  // parameters read every iteration: LDS instead of global-memory latency per colour
         const int nl = T.n_cls[CLS_LIN];
