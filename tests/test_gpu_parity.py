@@ -378,6 +378,31 @@ def test_uneven_ring_batch_transient_matches_oracle():
             assert err <= 1e-9, (gi, i, err)
 
 
+def test_mixed_ring_batch_with_hub_cells_matches_fp64_and_oracle():
+    """256 uneven rings in mixed precision: at most two instances per SIMD, so every shared slot is a hub
+    (one evaluation round per device class); the FP64 run of the same batch keeps the colour rounds.
+    Every instance's waveform within the mixed-precision bar (1e-6 V) of the FP64 one, and a few of them
+    against the oracle."""
+    gs = W.ring_groups(256, seed=0xC3)
+    groups = [Group(n.to_c(), k, over) for n, k, over, _ in gs]
+    log("hub-cell ring batch", len(groups), "groups")
+    m = Batch(groups, precision=MIXED).tran(1e-12, 2e-10, [("1", 0.0)])
+    f = Batch(groups).tran(1e-12, 2e-10, [("1", 0.0)])
+    assert m.structure(0)["n_contrib"] > f.structure(0)["n_contrib"]  # the hub cells
+    worst = 0.0
+    for gi, (n, k, over, _) in enumerate(gs):
+        assert np.all(m.status(gi)[0] == 0) and np.all(f.status(gi)[0] == 0), gi
+        for i in range(k):
+            worst = max(worst, float(np.max(np.abs(m.wave(gi, i) - f.wave(gi, i)))))
+    log("  mixed vs fp64 max |dv|", worst)
+    assert worst <= 1e-6
+    for gi in (0, len(gs) // 2, len(gs) - 1):
+        n, k, over, _ = gs[gi]
+        t, X, _, _ = O.tran(materialize(n, over, k - 1).to_c(), 1e-12, 2e-10, [("1", 0.0)])
+        assert float(np.max(np.abs(m.wave(gi, k - 1) - X))) <= 1e-6
+        assert float(np.max(np.abs(f.wave(gi, k - 1) - X))) <= 1e-9
+
+
 S21_CONVERGENCE = 2
 
 
