@@ -330,6 +330,19 @@ def test_random_circuits_ac_match_oracle(jit):
     assert fails == 3
 
 
+@pytest.mark.parametrize("f0,f1,npts", [(1000, 1000, 10), (1000, 2000, 1), (1000, 100_000, 17), (7, 7_000_000, 40)])
+def test_ac_short_sweeps_match_oracle(f0, f1, npts):
+    """Sweeps of one, two and a few dozen frequencies (one run, or runs of a couple of frequencies)."""
+    nl, over = W.amp_ladder(4)
+    b = Batch(Group(nl.to_c(), 4, over)).ac(f0, f1, npts)
+    f = b.axis()
+    assert np.all(b.status()[0] == 0)
+    for i in range(4):
+        fo, Xo, _ = O.ac(materialize(nl, over, i).to_c(), f0, f1, npts, max_pts=npts + 10)
+        assert np.array_equal(f, fo)
+        assert rel_err(b.wave(0, i).ravel(), Xo.ravel()) < 1e-9, (i, b.ac_runs())
+
+
 def test_ac_mixed_close_to_fp64():
     n_inst = 16
     nl, over = W.amp_ladder(n_inst)
