@@ -1,8 +1,8 @@
 """MI355X engine against the oracle (and the reference's golden waveforms).  Needs the GPU.
 
 Tolerances: device evaluation uses the GPU's exp/sqrt (within an ulp or so of glibc's), so assembled
-values agree to 1e-12 relative; FP64 Newton solutions agree with the oracle to 1e-9 relative (the
-north-star bar for the plain FP64 path); mixed precision agrees with FP64 to 1e-6 V."""
+values agree to 1e-12 relative; FP64 Newton solutions agree with the oracle to 1e-9 relative (the bar
+for the plain FP64 path); mixed precision agrees with FP64 to 1e-6 V."""
 import json
 import os
 
