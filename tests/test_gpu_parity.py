@@ -200,7 +200,7 @@ def test_generated_factorisation_transient_matches_golden(ipw):
 
 
 def test_ac_batch_matches_oracle_per_instance():
-    """C5-style sweep: two-stage CS amplifier with a per-instance RC ladder, 1 kHz to 100 GHz."""
+    """24 two-stage CS amplifiers with per-instance 20-section RC ladders, swept from 1 kHz to 100 GHz."""
     n_inst = 24
     nl, over = W.amp_ladder(n_inst)
     cn = nl.to_c()
@@ -374,7 +374,8 @@ def test_ac_byte_abi_matches_oracle():
 
 
 def test_uneven_ring_batch_transient_matches_oracle():
-    """C3-style: several groups of different size in one launch, per-instance C and VDD, 300 steps."""
+    """Uneven ring oscillators: several groups of different size in one launch, per-instance C and VDD,
+    300 steps."""
     gs = W.ring_groups(48, seed=0xC3)[:4]
     groups = [Group(n.to_c(), k, over) for n, k, over, _ in gs]
     log("ring batch groups", [(g.cnet.n_vars, g.n_inst) for g in groups])
