@@ -633,7 +633,8 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
         }
     }
     if (changed) prepare_jit(b);
-    HIPCHECK(hipStreamSynchronize(b->stream));
+    // No wait here: the resets are ordered before the launch on the batch's stream, and nothing before
+    // the launch touches the state or output buffers from another stream.
 }
 
 template <bool MIXED>
