@@ -105,6 +105,7 @@ struct s21_batch {
     bool has_jit = false;
     std::vector<double> axis;
     std::unique_ptr<s21::DevMem> d_wave, d_groups;
+    std::vector<char> groups_uploaded;  // the descriptors d_groups holds
     int last_mode = -1;
     bool fetched = true;  // host copies of x / status / iterations are current
     ~s21_batch() {
@@ -744,9 +745,16 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
         blk += (gg.n_inst * ac_units(A) + per_block - 1) / per_block;
     }
     if (!b->d_groups) b->d_groups.reset(new DevMem());
-    if (b->d_groups->bytes < hg.size() * sizeof(GroupGPU)) b->d_groups->alloc(hg.size() * sizeof(GroupGPU));
-    HIPCHECK(hipMemcpyAsync(b->d_groups->p, hg.data(), hg.size() * sizeof(GroupGPU), hipMemcpyHostToDevice,
-                            b->stream));
+    const size_t gbytes = hg.size() * sizeof(GroupGPU);
+    if (b->d_groups->bytes < gbytes) {
+        b->d_groups->alloc(gbytes);
+        b->groups_uploaded.clear();
+    }
+    // The descriptors change only with a preparation or an analysis's buffers: a repeated call skips them.
+    if (b->groups_uploaded.size() != gbytes || std::memcmp(b->groups_uploaded.data(), hg.data(), gbytes) != 0) {
+        HIPCHECK(hipMemcpyAsync(b->d_groups->p, hg.data(), gbytes, hipMemcpyHostToDevice, b->stream));
+        b->groups_uploaded.assign((const char*)hg.data(), (const char*)hg.data() + gbytes);
+    }
     const bool jit = b->has_jit && A.mode != MODE_ASSEMBLE;
     HIPCHECK(hipEventRecord(b->ev0, b->stream));
     if (jit) {
@@ -1028,6 +1036,7 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
             if (!sb.d_groups) sb.d_groups.reset(new DevMem());
             if (sb.d_groups->bytes < sizeof(GroupGPU)) sb.d_groups->alloc(sizeof(GroupGPU));
             HIPCHECK(hipMemcpy(sb.d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice));
+            sb.groups_uploaded.clear();
             launch_any(&sb, (const GroupGPU*)sb.d_groups->p, 1, 1, h.blob_bytes + stride, A);
             HIPCHECK(hipStreamSynchronize(sb.stream));
             const Topology& t = h.topo;
@@ -1584,6 +1593,7 @@ int s21_batch_assemble(s21_batch* b, int32_t g, int32_t inst, const double* x, i
         if (!b->d_groups) b->d_groups.reset(new DevMem());
         if (b->d_groups->bytes < sizeof(GroupGPU)) b->d_groups->alloc(sizeof(GroupGPU));
         HIPCHECK(hipMemcpy(b->d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice));
+        b->groups_uploaded.clear();
         launch_any(b, (const GroupGPU*)b->d_groups->p, 1, 1, gh.blob_bytes + stride, A);
         HIPCHECK(hipStreamSynchronize(b->stream));
         std::vector<double> dbg(t.nnz + t.n);
