@@ -274,3 +274,31 @@ def test_mos1_reversed_body_effect_quirk():
     vt = 1.3806226e-23 / 1.6021918e-19 * 300.15
     gbs = 1e-20 / vt + 1e-12
     assert J[1, 1] == pytest.approx(gm + gds + gbs + gmbs, rel=1e-9)
+
+
+def test_ac_common_source_gain_closed_form():
+    """A level-1 NMOS common-source stage without capacitances (no tox, no junction capacitance): its AC
+    response is the small-signal gain at every frequency, -gm / (gds + 1/RD + gbd), with gm and gds of the
+    square law at the oracle's own operating point and gbd the drain junction's exp term plus gmin.  Pins
+    the Mos1 AC stamps (gm from gate to drain, gds and gbd on the drain) and their signs."""
+    for lam, rd in [(0.05, 5e3), (0.0, 2e3), (0.1, 8e3)]:
+        n = Netlist()
+        nm = n.mos1_model(vt0=0.4, kp=100e-6, lambda_=lam)
+        ip = n.mos1_inst(w=1e-6, l=0.1e-6)
+        n.v("vin", 0.6, "g", "", acm=1.0)
+        n.v("vdd", 1.0, "vdd", "")
+        n.r(1.0 / rd, "vdd", "d")
+        n.mos1("m", "d", "g", "", "", nm, ip)
+        s, _ = op(n)
+        vds = s["d"]
+        beta, vov = 100e-6 * 10.0, 0.2
+        assert vds >= vov  # saturation
+        gm = beta * vov * (1.0 + lam * vds)
+        gds = lam * beta / 2.0 * vov * vov
+        vt = 1.3806226e-23 / 1.6021918e-19 * 300.15
+        gbd = 1e-8 * 1e-12 / vt * math.exp(-vds / vt) + 1e-12
+        gain = -gm / (gds + 1.0 / rd + gbd)
+        f, X, _ = O.ac(n.to_c(), 1, 1_000_000_000, 9)
+        out = X[:, n.names.index("d")]
+        assert np.max(np.abs(out - gain)) <= 1e-9 * abs(gain), (lam, rd, out[0], gain)
+        assert np.all(X[:, n.names.index("g")] == 1.0 + 0.0j)
