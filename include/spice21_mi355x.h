@@ -246,10 +246,12 @@ S21_API int s21_batch_set_profiling(s21_batch* b, int32_t on);
 S21_API int s21_batch_profile(const s21_batch* b, int32_t group, int64_t* out /* [n_inst][8] */);
 /* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the
  * voltages x[n_vars], in OP mode: returns matrix values per created element, in the order
- * Matrix::make created them (row_out/col_out give their external positions), and the RHS. */
+ * Matrix::make created them (row_out/col_out give their external positions), and the RHS
+ * (rhs_out[n_vars]).  row_out, col_out and val_out hold `cap` entries; a matrix with more elements
+ * returns S21_INVALID and writes nothing. */
 S21_API int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const double* x,
                        int32_t* row_out, int32_t* col_out, double* val_out, double* rhs_out,
-                       int64_t* n_elem);
+                       int64_t cap, int64_t* n_elem);
 
 /* ---- byte ABI mirroring CallableProto::call_bytes (proto.rs:32-145) ------------------------ */
 /* Op -> OpResult, Tran -> TranResult, Ac -> AcResult, all protobuf-encoded.  *out is allocated

@@ -630,6 +630,15 @@ struct AnalysisInfo {
     double omega = 0.0;
 };
 // TranState::integrate, BE branch (analysis.rs:420-428)
+// Planted mistakes (0 = none): tests set one to show that a closed-form check fails on it.
+//   1  Mos1 transient: the gate-source integration kept and the gate-drain one stored apart (the
+//      reference overwrites tr.gs with the gate-drain integration, mos.rs:787-802)
+//   2  Mos1 AC: the (G, dr) admittance stamped once (the reference stamps it twice, mos.rs:951,954)
+//   3  Diode: depletion charge without the transit-time factor (diode.rs:307-309 multiplies by tt)
+//   4  Meyer gate-bulk capacitance below threshold over phi instead of 2 phi (mos.rs:727-735)
+//   5  Meyer saturation gate-source capacitance cox/2 instead of cox/3 (mos.rs:740-743)
+static int g_plant = 0;
+
 static inline ChargeInteg integq(const TranState& st, double dq, double dq_dv, double vguess) {
     double dt = st.dt;
     double g = dq_dv / dt;
@@ -1069,15 +1078,15 @@ struct Mos1 : Component {
             cgs1 = 0.0;
             cgd1 = 0.0;
         } else if (vov <= -intp.phi_t / 2.0) {
-            cgb1 = -vov * cox / (2.0 * intp.phi_t);
+            cgb1 = -vov * cox / ((g_plant == 4 ? 1.0 : 2.0) * intp.phi_t);
             cgs1 = 0.0;
             cgd1 = 0.0;
         } else if (vov <= 0.0) {
-            cgb1 = -vov * cox / (2.0 * intp.phi_t);
+            cgb1 = -vov * cox / ((g_plant == 4 ? 1.0 : 2.0) * intp.phi_t);
             cgs1 = vov * cox / (1.5 * intp.phi_t) + cox / 3.0;
             cgd1 = 0.0;
         } else if (vdsat <= vds) {
-            cgs1 = cox / 3.0;
+            cgs1 = cox / (g_plant == 5 ? 2.0 : 3.0);
             cgd1 = 0.0;
             cgb1 = 0.0;
         } else {
@@ -1105,7 +1114,10 @@ struct Mos1 : Component {
             }
             {
                 double dqgd = same ? (vgd - op.vgd) * cgd : (vgd - op.vgs) * cgd;
-                tr_gs = integq(st, dqgd, cgd, vgd);  // overwrites tr.gs, as mos.rs:801 does
+                if (g_plant == 1)
+                    tr_gd = integq(st, dqgd, cgd, vgd);
+                else
+                    tr_gs = integq(st, dqgd, cgd, vgd);  // overwrites tr.gs, as mos.rs:801 does
             }
             {
                 double dqgb = (vgb - op.vgb) * cgb;
@@ -1198,7 +1210,7 @@ struct Mos1 : Component {
             {matps[MG][dr], {0.0, -gcgd}},
             {matps[MG][sr], {0.0, -gcgs}},
             {matps[MB][MG], {0.0, -gcgb}},
-            {matps[MG][dr], {0.0, -gcgd}},  // stamped twice, as mos.rs:951,954
+            {g_plant == 2 ? -1 : matps[MG][dr], {0.0, -gcgd}},  // stamped twice, as mos.rs:951,954
             {matps[MB][dr], {-gbd, 0.0}},
             {matps[MB][sr], {-gbs, 0.0}},
             {matps[dr][MB], {-gbd + gmbs, 0.0}},
@@ -1372,7 +1384,7 @@ struct Diode : Component {
         if (vd < intp.dep_threshold) {
             double a = 1.0 - vd / model.vj;
             double s = -model.m * std::log(a);
-            qd = model.tt * model.vj * intp.cz * (1.0 - a * s) / (1.0 - model.m);
+            qd = (g_plant == 3 ? 1.0 : model.tt) * model.vj * intp.cz * (1.0 - a * s) / (1.0 - model.m);
             cd = model.tt * gd + intp.cz * s;
         } else {
             qd = model.tt * id + intp.cz * intp.f1 +
@@ -1592,6 +1604,8 @@ static int code_of(const std::string& d) {
 }
 
 extern "C" {
+
+void oracle_plant(int k) { g_plant = k; }
 
 /* DC operating point (analysis.rs:383-388).  x_out[n_vars]; iters_out: Newton iterations. */
 int oracle_dcop(const s21_netlist* net, const s21_options* o, double* x_out, int64_t* iters_out, int64_t* facts_out,

@@ -56,6 +56,8 @@ def lib():
         L.oracle_mat_swap_rows.argtypes = [C.c_void_p, i64, i64]
         L.oracle_mat_row_mapping.argtypes = [C.c_void_p, i64, P(i64), P(i64)]
         L.oracle_mat_check.argtypes = [C.c_void_p]
+        L.oracle_plant.argtypes = [C.c_int]
+        L.oracle_plant.restype = None
         _lib = L
     return _lib
 
@@ -264,3 +266,19 @@ class Matrix:
 
     def check(self):
         return lib().oracle_mat_check(self.h)
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake ``k`` (see g_plant in oracle.cpp), so that a
+    test can show that its check fails on that mistake."""
+
+    def __init__(self, k):
+        self.k = int(k)
+
+    def __enter__(self):
+        lib().oracle_plant(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_plant(0)
+        return False

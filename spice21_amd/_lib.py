@@ -73,7 +73,7 @@ def lib():
     _sig(L, "s21_batch_last_ac_runs", i32, vp)
     _sig(L, "s21_batch_profile", C.c_int, vp, i32, P(i64))
     _sig(L, "s21_batch_structure", C.c_int, vp, i32, P(i64), P(i64), P(i64), P(i64))
-    _sig(L, "s21_batch_assemble", C.c_int, vp, i32, i32, P(d), P(i32), P(i32), P(d), P(d), P(i64))
+    _sig(L, "s21_batch_assemble", C.c_int, vp, i32, i32, P(d), P(i32), P(i32), P(d), P(d), i64, P(i64))
     _lib = L
     return L
 

@@ -239,13 +239,18 @@ class Batch:
         return dict(nnz_a=v[0].value, nnz_lu=v[1].value, levels=v[2].value, n_contrib=v[3].value)
 
     def assemble(self, x, g=0, inst=0, max_elems=1 << 20):
+        """Matrix elements (rows, cols, values in creation order) and RHS of instance ``inst`` at the
+        operating-point voltages ``x`` (one value per unknown of the group)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
+        n_vars = self.n_vars(g)
+        if x.shape != (n_vars,):
+            raise ValueError(f"x has shape {x.shape}, the group has {n_vars} unknowns")
         rows = np.zeros(max_elems, dtype=np.int32)
         cols = np.zeros(max_elems, dtype=np.int32)
         vals = np.zeros(max_elems)
-        rhs = np.zeros(max(len(x), 1))
+        rhs = np.zeros(max(n_vars, 1))
         ne = C.c_int64()
         check(lib().s21_batch_assemble(self.h, g, inst, _dp(x), _i32p(rows), _i32p(cols), _dp(vals), _dp(rhs),
-                                       C.byref(ne)))
+                                       int(max_elems), C.byref(ne)))
         n = ne.value
-        return rows[:n], cols[:n], vals[:n], rhs[: len(x)]
+        return rows[:n], cols[:n], vals[:n], rhs[:n_vars]

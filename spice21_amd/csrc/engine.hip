@@ -634,8 +634,8 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
         }
     }
     if (changed) prepare_jit(b);
-    // No wait here: the resets are ordered before the launch on the batch's stream, and nothing before
-    // the launch touches the state or output buffers from another stream.
+    // No wait here: the resets are ordered on the batch's stream, so every later write into the state or
+    // output buffers (the launch, s21_batch_assemble's upload of x) must be issued on that stream too.
 }
 
 template <bool MIXED>
@@ -1015,6 +1015,7 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
             write_state(h, s.state);
         };
         prepare_all(&sb, tr);
+        HIPCHECK(hipStreamSynchronize(sb.stream));  // the state resets land before the uploads below
         // Instances that already ran in FP64 need a new order: the interrupted iteration's matrix, as the
         // kernel assembles it from the iterate and the state, searched after the earlier ones.
         bool rebuilt = false;
@@ -1051,6 +1052,7 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
             rebuilt = true;
         }
         if (rebuilt) prepare_all(&sb, tr);
+        HIPCHECK(hipStreamSynchronize(sb.stream));
         for (size_t k = 0; k < todo.size(); k++) load_inputs(k);
         std::unique_ptr<DevMem> wave;
         if (tr) {
@@ -1572,16 +1574,19 @@ int s21_batch_structure(const s21_batch* b, int32_t g, int64_t* nnz_a, int64_t* 
     });
 }
 int s21_batch_assemble(s21_batch* b, int32_t g, int32_t inst, const double* x, int32_t* row_out, int32_t* col_out,
-                       double* val_out, double* rhs_out, int64_t* n_elem) {
+                       double* val_out, double* rhs_out, int64_t cap, int64_t* n_elem) {
     return guard([&] {
         if (!b || g < 0 || g >= (int)b->groups.size() || !x) fail(S21_INVALID, "bad arguments");
         GroupHost& gh = b->groups[g];
         if (inst < 0 || inst >= gh.n_inst) fail(S21_INVALID, "instance out of range");
         prepare_all(b, nullptr);
         const Topology& t = gh.topo;
-        HIPCHECK(hipMemcpy(gh.gg.x + (size_t)inst * t.n, x, t.n * sizeof(double), hipMemcpyHostToDevice));
-        // run only this group
-        std::vector<GroupHost> saved;
+        const int64_t ne = (int64_t)t.erow.size();
+        if (ne > cap) fail(S21_INVALID, "element buffers hold " + std::to_string(cap) + " entries, the matrix has " +
+                                            std::to_string(ne));
+        // Every write below is on the batch's stream, after the state resets prepare_all queued there.
+        HIPCHECK(hipMemcpyAsync(gh.gg.x + (size_t)inst * t.n, x, t.n * sizeof(double), hipMemcpyHostToDevice,
+                                b->stream));
         Args A = base_args(b, MODE_ASSEMBLE);
         A.dbg_inst = inst;
         A.waves = 1;
@@ -1592,13 +1597,12 @@ int s21_batch_assemble(s21_batch* b, int32_t g, int32_t inst, const double* x, i
         gg.blk0 = 0;
         if (!b->d_groups) b->d_groups.reset(new DevMem());
         if (b->d_groups->bytes < sizeof(GroupGPU)) b->d_groups->alloc(sizeof(GroupGPU));
-        HIPCHECK(hipMemcpy(b->d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpyAsync(b->d_groups->p, &gg, sizeof(GroupGPU), hipMemcpyHostToDevice, b->stream));
         b->groups_uploaded.clear();
         launch_any(b, (const GroupGPU*)b->d_groups->p, 1, 1, gh.blob_bytes + stride, A);
-        HIPCHECK(hipStreamSynchronize(b->stream));
         std::vector<double> dbg(t.nnz + t.n);
-        HIPCHECK(hipMemcpy(dbg.data(), gh.gg.dbg, dbg.size() * sizeof(double), hipMemcpyDeviceToHost));
-        const int64_t ne = (int64_t)t.erow.size();
+        HIPCHECK(hipMemcpyAsync(dbg.data(), gh.gg.dbg, dbg.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));
         for (int64_t e = 0; e < ne; e++) {
             if (row_out) row_out[e] = t.erow[e];
             if (col_out) col_out[e] = t.ecol[e];
