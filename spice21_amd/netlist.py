@@ -16,6 +16,7 @@ class Netlist:
     def __init__(self, signals=()):
         self.names = []
         self.kinds = []
+        self._index = {}  # name -> first variable of that name (the reference's linear search finds the first)
         self.comps = []  # tuples (type, nodes[6], model, inst, value, acm, name)
         self.mos1_models = []
         self.mos1_insts = []
@@ -30,6 +31,7 @@ class Netlist:
     def add(self, name, kind):
         self.names.append(name)
         self.kinds.append(kind)
+        self._index.setdefault(name, len(self.names) - 1)
         return len(self.names) - 1
 
     def addv(self, name):
@@ -39,10 +41,7 @@ class Netlist:
         return self.add(name, A.S21_VAR_I)
 
     def find(self, name):
-        try:
-            return self.names.index(name)
-        except ValueError:
-            return None
+        return self._index.get(name)
 
     def node(self, name):
         """Variables::find_or_create for a top-level node; '' / None is ground."""
