@@ -173,6 +173,21 @@ S21_API void s21_batch_free(s21_batch* b);
 
 /* DC operating point of every instance (analysis.rs:383-388 and Solver::solve :169-210). */
 S21_API int s21_batch_op(s21_batch* b);
+/* Operating point of every instance as s21_batch_op computes it, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).
+ *   - The batch must already hold its operating-point structure: one earlier s21_batch_op call on it
+ *     builds and uploads that (and allocates); otherwise S21_INVALID.
+ *   - x [sum over groups of n_inst * n_vars], status and iters [sum over groups of n_inst] are caller-
+ *     allocated DEVICE buffers on the batch's device, groups in order and instances in order within a
+ *     group.  x's contents are ignored: the solve starts from x = 0 as the reference's does.
+ *   - The Newton state lives in a fixed 64 MiB device array of the library (one per GPU, kept until
+ *     the process ends, cleared by each call); a batch whose state needs more returns S21_INVALID.
+ *   - An instance whose frozen pivot order meets a zero pivot reports S21_PIVOT (s21_batch_op resumes
+ *     such instances on a new order in a launch of its own, which allocates).
+ *   - The batch's own results (s21_batch_get_*) are left unchanged; s21_batch_timing reports this call.
+ *   - Every call on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+S21_API int s21_batch_op_noalloc(s21_batch* b, double* x, int32_t* status, int64_t* iters);
 /* Fixed-step Backward-Euler transient with initial-condition forcing (analysis.rs:489-574). */
 S21_API int s21_batch_tran(s21_batch* b, const s21_tran_options* t);
 /* Operating point, then complex small-signal sweep (analysis.rs:761-832). */

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -88,6 +89,7 @@ struct s21_batch {
     std::vector<s21::GroupHost> groups;
     s21_options opts;
     int precision = S21_FP64;
+    int device = 0;  // the HIP device current at creation, which holds every buffer of the batch
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, kernel_avg_ms = 0.0;
@@ -715,16 +717,23 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
     }
 }
 
-// Runs the kernel over every group in one launch.
-static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
+// Launch of the kernel over every group: its arguments, LDS bytes, workgroups and group descriptors.
+struct LaunchPlan {
+    Args A;
+    size_t smem = 0;
+    int blk = 0;
     std::vector<GroupGPU> hg;
+};
+
+static LaunchPlan plan_launch(const s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
+    LaunchPlan P;
     size_t wave_off = 0, blob = 0, raw = 0;
     int total = 0;
-    for (GroupHost& g : b->groups) {
+    for (const GroupHost& g : b->groups) {
         GroupGPU gg = g.gg;
         gg.wave = d_wave ? d_wave + wave_off : nullptr;
         wave_off += (size_t)g.n_inst * (size_t)npts * g.topo.n;
-        hg.push_back(gg);
+        P.hg.push_back(gg);
         blob = std::max(blob, g.blob_bytes);
         raw = std::max(raw, g.inst_bytes);
         total += g.n_inst * ac_units(A0);
@@ -733,38 +742,33 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    Args A = A0;
+    P.A = A0;
     size_t stride = 0;
-    pick_config(b, blob, raw, total, (size_t)std::max(max_block, 65536), 4 * std::max(cus, 1), A, stride);
-    A.inst_bytes = (int)stride;
-    const size_t smem = blob + (size_t)A.waves * A.ipw * stride;
-    const int per_block = A.waves * A.ipw;
-    int blk = 0;
-    for (GroupGPU& gg : hg) {
-        gg.blk0 = blk;
-        blk += (gg.n_inst * ac_units(A) + per_block - 1) / per_block;
+    pick_config(b, blob, raw, total, (size_t)std::max(max_block, 65536), 4 * std::max(cus, 1), P.A, stride);
+    P.A.inst_bytes = (int)stride;
+    P.smem = blob + (size_t)P.A.waves * P.A.ipw * stride;
+    const int per_block = P.A.waves * P.A.ipw;
+    for (GroupGPU& gg : P.hg) {
+        gg.blk0 = P.blk;
+        P.blk += (gg.n_inst * ac_units(P.A) + per_block - 1) / per_block;
     }
-    if (!b->d_groups) b->d_groups.reset(new DevMem());
-    const size_t gbytes = hg.size() * sizeof(GroupGPU);
-    if (b->d_groups->bytes < gbytes) {
-        b->d_groups->alloc(gbytes);
-        b->groups_uploaded.clear();
-    }
-    // The descriptors change only with a preparation or an analysis's buffers: a repeated call skips them.
-    if (b->groups_uploaded.size() != gbytes || std::memcmp(b->groups_uploaded.data(), hg.data(), gbytes) != 0) {
-        HIPCHECK(hipMemcpyAsync(b->d_groups->p, hg.data(), gbytes, hipMemcpyHostToDevice, b->stream));
-        b->groups_uploaded.assign((const char*)hg.data(), (const char*)hg.data() + gbytes);
-    }
+    return P;
+}
+
+// Launches a plan whose descriptors are already at `dgroups` (device memory), timed by the batch's events.
+static void launch_plan(s21_batch* b, const LaunchPlan& P, const GroupGPU* dgroups) {
+    const Args& A = P.A;
     const bool jit = b->has_jit && A.mode != MODE_ASSEMBLE;
     HIPCHECK(hipEventRecord(b->ev0, b->stream));
     if (jit) {
-        const GroupGPU* dg = (const GroupGPU*)b->d_groups->p;
-        int ng = (int)hg.size();
-        void* args[] = {(void*)&dg, (void*)&ng, (void*)&A};
-        HIPCHECK(hipModuleLaunchKernel(b->jitk.fn, blk, 1, 1, WAVE * A.waves, 1, 1, (unsigned)smem, b->stream,
+        const GroupGPU* dg = dgroups;
+        int ng = (int)P.hg.size();
+        Args a = A;
+        void* args[] = {(void*)&dg, (void*)&ng, (void*)&a};
+        HIPCHECK(hipModuleLaunchKernel(b->jitk.fn, P.blk, 1, 1, WAVE * A.waves, 1, 1, (unsigned)P.smem, b->stream,
                                        args, nullptr));
     } else {
-        launch_any(b, (const GroupGPU*)b->d_groups->p, (int)hg.size(), blk, smem, A);
+        launch_any(b, dgroups, (int)P.hg.size(), P.blk, P.smem, A);
     }
     HIPCHECK(hipEventRecord(b->ev1, b->stream));
     HIPCHECK(hipEventSynchronize(b->ev1));
@@ -776,6 +780,23 @@ static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
     b->launches = 1;
     b->last_ms = ms;
     b->kernel_avg_ms = ms;
+}
+
+// Runs the kernel over every group in one launch.
+static void run(s21_batch* b, const Args& A0, double* d_wave, int64_t npts) {
+    const LaunchPlan P = plan_launch(b, A0, d_wave, npts);
+    if (!b->d_groups) b->d_groups.reset(new DevMem());
+    const size_t gbytes = P.hg.size() * sizeof(GroupGPU);
+    if (b->d_groups->bytes < gbytes) {
+        b->d_groups->alloc(gbytes);
+        b->groups_uploaded.clear();
+    }
+    // The descriptors change only with a preparation or an analysis's buffers: a repeated call skips them.
+    if (b->groups_uploaded.size() != gbytes || std::memcmp(b->groups_uploaded.data(), P.hg.data(), gbytes) != 0) {
+        HIPCHECK(hipMemcpyAsync(b->d_groups->p, P.hg.data(), gbytes, hipMemcpyHostToDevice, b->stream));
+        b->groups_uploaded.assign((const char*)P.hg.data(), (const char*)P.hg.data() + gbytes);
+    }
+    launch_plan(b, P, (const GroupGPU*)b->d_groups->p);
 }
 
 // Results stay on the device after an analysis; the first query copies them to the host.
@@ -1103,6 +1124,45 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
     }
 }
 
+// ---- allocation-free operating point ----
+// s21_batch_op_noalloc keeps every instance's Newton state (committed and guess operating points of the
+// devices, charges) in this fixed array instead of buffers of the batch; the group descriptors sit at its
+// start.  Each device holds its own copy from module load until the process ends.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+
+// The array's device address on the current device, resolved once per device.
+static char* scratch_on_current_device() {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) fail(S21_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(S21_DEVICE, "scratch array has no device address");
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Clears n 32-bit words with a kernel (a fill kernel rather than a memset node, so that the call can be
+// captured into a HIP graph).
+__global__ void fill_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+static void zero_async(void* p, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 4;
+    if (!n) return;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(fill_zero_u32, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, n);
+    HIPCHECK(hipGetLastError());
+}
+
 template <typename F>
 static int guard(F&& f) {
     try {
@@ -1214,6 +1274,7 @@ int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_option
         b->precision = precision == S21_MIXED ? S21_MIXED : S21_FP64;
         b->groups.resize(n_groups);
         for (int gi = 0; gi < n_groups; gi++) group_from(b->groups[gi], groups[gi]);
+        HIPCHECK(hipGetDevice(&b->device));
         HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&b->ev0));
         HIPCHECK(hipEventCreate(&b->ev1));
@@ -1433,6 +1494,67 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
         for (size_t gi = 0; gi < b->groups.size(); gi++)
             for (int i = 0; i < b->groups[gi].n_inst; i++)
                 if (dc_status[gi][i] != S21_OK) b->groups[gi].status[i] = dc_status[gi][i];
+    });
+}
+
+int s21_batch_op_noalloc(s21_batch* b, double* x, int32_t* status, int64_t* iters) {
+    return guard([&] {
+        if (!b || !x || !status || !iters) fail(S21_INVALID, "null argument");
+        if ((uintptr_t)x % alignof(double) || (uintptr_t)status % alignof(int32_t) ||
+            (uintptr_t)iters % alignof(int64_t))
+            fail(S21_INVALID, "s21_batch_op_noalloc: misaligned output buffer");
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        if (dev != b->device) fail(S21_INVALID, "s21_batch_op_noalloc: the current device is not the batch's");
+        for (const GroupHost& g : b->groups)
+            if (g.prepared != "op")
+                fail(S21_INVALID, "s21_batch_op_noalloc: the batch holds no operating-point structure; "
+                                  "one s21_batch_op call on it builds and uploads it");
+        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+        std::vector<size_t> st_off;
+        size_t off = up(b->groups.size() * sizeof(GroupGPU));
+        for (const GroupHost& g : b->groups) {
+            st_off.push_back(off);
+            off += up(g.d_state->bytes);
+        }
+        if (off > NOALLOC_SCRATCH_BYTES)
+            fail(S21_INVALID, "s21_batch_op_noalloc: the batch's Newton state needs " + std::to_string(off) +
+                                  " bytes, the scratch array holds " + std::to_string(NOALLOC_SCRATCH_BYTES));
+        char* base = scratch_on_current_device();
+        Args A = base_args(b, MODE_OP);
+        A.prof = 0;
+        LaunchPlan P = plan_launch(b, A, nullptr, 1);
+        size_t xo = 0, io = 0;
+        for (size_t k = 0; k < b->groups.size(); k++) {
+            const GroupHost& g = b->groups[k];
+            GroupGPU& gg = P.hg[k];
+            char* sp = base + st_off[k];
+            const char* s0 = (const char*)g.gg.m1_state;  // the batch's own state buffer, same layout
+            gg.m1_state = (double*)sp;
+            gg.d_state = (double*)(sp + ((const char*)g.gg.d_state - s0));
+            gg.c_q = (double*)(sp + ((const char*)g.gg.c_q - s0));
+            gg.x = x + xo;
+            gg.status = status + io;
+            gg.iters = (long long*)(iters + io);
+            gg.dbg = nullptr;
+            gg.prof = nullptr;
+            gg.resume_at = nullptr;
+            gg.acop = nullptr;
+            gg.wave = nullptr;
+            gg.ac_rec = nullptr;
+            gg.ac_end = nullptr;
+            gg.ac_state = nullptr;
+            xo += (size_t)g.n_inst * g.topo.n;
+            io += (size_t)g.n_inst;
+        }
+        // Default operating points (zeros) and the reference's start x = 0: the array keeps the last
+        // call's contents, and the caller's buffers hold whatever they held.
+        zero_async(base + st_off[0], off - st_off[0], b->stream);
+        zero_async(x, xo * sizeof(double), b->stream);
+        zero_async(status, io * sizeof(int32_t), b->stream);
+        zero_async(iters, io * sizeof(int64_t), b->stream);
+        HIPCHECK(hipMemcpyAsync(base, P.hg.data(), P.hg.size() * sizeof(GroupGPU), hipMemcpyHostToDevice, b->stream));
+        launch_plan(b, P, (const GroupGPU*)base);
     });
 }
 
