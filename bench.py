@@ -54,17 +54,50 @@ def algorithmic_bytes(cnet, structure, factor_bytes):
     return dev + 8 * structure["nnz_a"] + 2 * factor_bytes * structure["nnz_lu"] + 4 * 8 * n
 
 
-def measured_traffic():
-    """HBM bytes per launch of the mixed kernel on this workload, from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size); None when absent."""
+COUNTERS_FILE = os.path.join(HERE, "profiles", "s21_newton_jit_chain_mc4096_counters.txt")
+
+
+def summary_fields(path):
+    """`key: value` lines of a committed rocprofv3 summary (profiles/), as a dict of strings."""
+    out = {}
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and not line.startswith(" ") and line.split(":", 1)[0].replace("_", "").isalnum():
+                    k, v = line.split(":", 1)
+                    out[k.strip()] = v.strip()
     except OSError:
         pass
-    return None
+    return out
+
+
+def measured_traffic():
+    """HBM bytes per launch of the mixed kernel on this workload, copied from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size; counters cannot be read inside
+    this run); (None, None) when absent."""
+    f = summary_fields(TRAFFIC_FILE)
+    if "traffic_bytes_per_launch" not in f:
+        return None, None
+    return float(f["traffic_bytes_per_launch"]), f"copied from profiles/{os.path.basename(TRAFFIC_FILE)}"
+
+
+def binding_unit(kernel_avg_ms):
+    """What bounds the kernel, from the committed hardware-counter summary of the same workload (rocprofv3
+    --pmc on s21_newton_jit): FP64 VALU and LDS utilisation beside the HBM figure."""
+    f = summary_fields(COUNTERS_FILE)
+    if not f:
+        return None
+    out = {"source": f"copied from profiles/{os.path.basename(COUNTERS_FILE)}"}
+    for k in ("bound_by", "fp64_tflops", "fp64_frac_of_peak", "valu_busy_frac", "lds_busy_frac",
+              "lds_bank_conflict_frac", "hbm_gbs_measured", "hbm_frac_of_peak", "scratch_bytes_per_lane"):
+        if k in f:
+            v = f[k]
+            try:
+                v = float(v)
+            except ValueError:
+                pass
+            out[k] = v
+    return out
 
 
 def copy_calibration(mib):
@@ -149,20 +182,29 @@ def _ac_amp_ladder():
 
 
 def _tran_single_ring5():
+    import numpy as np
+
     from spice21_amd import workloads as W
-    from spice21_amd.batch import Batch, Group
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
 
     out = {}
     # one 5-stage ring oscillator (latency): the default path (record-driven factorisation) and the one
     # with the factorisation generated for its pattern (a run-time compilation a single run rarely repays)
+    nl = W.ring5()
     rec = {"workload": "5-stage CMOS ring, 0-10 ns at 1 ps"}
-    for jit, key in ((0, ""), (1, "_generated")):
-        b = Batch(Group(W.ring5().to_c())).set_jit(jit).tran(1e-12, 1e-8, [("1", 0.5)])
+    waves = {}
+    for prec, jit, key in ((FP64, 0, ""), (FP64, 1, "_generated"), (MIXED, 0, "_mixed")):
+        b = Batch(Group(nl.to_c()), precision=prec).set_jit(jit).tran(1e-12, 1e-8, [("1", 0.5)])
         b.tran(1e-12, 1e-8, [("1", 0.5)])
         ms = b.timing()[0]
         it = int(b.status()[1][0])
-        rec.update({"points": len(b.axis()), "newton_iters": it, "kernel_ms" + key: ms,
-                    "us_per_newton_iter" + key: ms * 1e3 / it})
+        rec.update({"points": len(b.axis()), "newton_iters" + key: it, "kernel_ms" + key: ms,
+                    "tran_pts_per_s" + key: len(b.axis()) / (ms * 1e-3), "us_per_newton_iter" + key: ms * 1e3 / it})
+        waves[key] = b.wave()
+    v = [k for k, kind in enumerate(nl.kinds) if kind == 0]
+    i = [k for k, kind in enumerate(nl.kinds) if kind == 1]
+    rec["mixed_vs_fp64_max_node_v"] = float(np.abs(waves["_mixed"][:, v] - waves[""][:, v]).max())
+    rec["mixed_vs_fp64_max_branch_i"] = float(np.abs(waves["_mixed"][:, i] - waves[""][:, i]).max())
     out["tran_single_ring5"] = rec
     return out
 
@@ -251,7 +293,7 @@ def main():
     kernel_avg_ms = float(np.mean(h["kernel_ms"]))
     b_iter = algorithmic_bytes(cn, h["structure"], 4 if head == "mixed" else 8)
     achieved = b_iter * inst_iters / (kernel_avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic() if head == "mixed" else None
+    traffic, traffic_src = measured_traffic() if head == "mixed" else (None, None)
 
     line = {
         "metric": "batched Newton instance-iters/s & tran pts/s, mixed vs FP64 at \u22641e-6 V error",
@@ -270,9 +312,13 @@ def main():
                    "unknowns": cn.n_vars, "mos1_per_instance": 128, "nnz_a": h["structure"]["nnz_a"],
                    "nnz_lu": h["structure"]["nnz_lu"], "newton_iters_per_step": inst_iters,
                    "failed_instances": int(np.sum(h["status"] != 0)), "launch": h["launch"]},
+        # achieved: the algorithmic bytes of the Newton iterations (matrix, LU and vectors counted as if
+        # each went through HBM once) over the kernel time.  They stay in LDS here: measured HBM traffic is
+        # below 1 % of them, and the unit that binds the kernel is named in "binding".
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_inst_iter": b_iter, "kernel_avg_ms": kernel_avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "bytes_per_inst_iter": b_iter, "kernel_avg_ms": kernel_avg_ms,
+                     "binding": binding_unit(kernel_avg_ms) if head == "mixed" else None},
     }
     if "fp64" in res:
         f = res["fp64"]
@@ -290,18 +336,45 @@ def main():
     if not args.no_cpu:
         from oracle import oracle_py as O  # the CPU restatement, timed as the baseline and used as the checker
 
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        nproc = os.cpu_count() or 1
+        try:
+            allowed = len(os.sched_getaffinity(0))
+        except AttributeError:
+            allowed = nproc
+        # every core this process may use, up to the host-thread share of one GPU (16 on the GPU boxes)
+        threads = max(1, min(args.cpu_threads, allowed))
         walls = []
         for _ in range(args.cpu_reps):  # the median of a few repetitions: a rate steadier than one pass
             t0 = time.perf_counter()
             xo, ito, sto = O.dcop_batch(group, threads=threads)
             walls.append(time.perf_counter() - t0)
         cpu_s = float(np.median(walls))
+        n1 = min(args.n_inst, 512)  # one core: a sample of the same instances
+        walls1 = []
+        for _ in range(args.cpu_reps):
+            t0 = time.perf_counter()
+            _, it1, _ = O.dcop_batch(group, n_inst=n1, threads=1)
+            walls1.append(time.perf_counter() - t0)
+        cpu1_s = float(np.median(walls1))
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
         line["cpu_baseline"] = {
             "value": float(ito.sum()) / cpu_s, "unit": "instance-iters/s", "cores": threads, "kind": "port",
             "sample": f"all {args.n_inst} instances, one dcop each ({int(ito.sum())} Newton iterations), "
                       f"{len(walls)} times on {threads} threads: median {cpu_s:.2f} s wall "
-                      f"(about {sum(walls) * threads:.0f} s of CPU work in all)"}
+                      f"(about {sum(walls) * threads + sum(walls1):.0f} s of CPU work in all, one-core sample "
+                      f"included)",
+            "what": "the oracle: a C++ restatement of the reference algorithm (Markowitz re-pivoting and "
+                    "linked-list LU at every Newton iteration, fixed-step BE), not the Rust reference (no "
+                    "Rust toolchain on these hosts)",
+            "nproc": nproc, "cores_allowed": allowed, "cpu_model": model,
+            "one_core": {"value": float(it1.sum()) / cpu1_s, "unit": "instance-iters/s",
+                         "sample": f"the first {n1} instances on 1 thread, median of {len(walls1)}: "
+                                   f"{cpu1_s:.2f} s"}}
         if "fp64" in res:
             xf = res["fp64"]["x"]
             rel = np.abs(xf - xo) / np.maximum(np.abs(xo), 1e-9)
