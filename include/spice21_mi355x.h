@@ -268,6 +268,45 @@ S21_API int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const 
                        int32_t* row_out, int32_t* col_out, double* val_out, double* rhs_out,
                        int64_t cap, int64_t* n_elem);
 
+/* ---- single large instance ------------------------------------------------------------------
+ * One netlist of tens of thousands of unknowns (a power grid with its loads), whose matrix does not fit a
+ * batched instance's LDS region.  The same Newton control, device models, stamps, IC forcing and
+ * fixed-step Backward-Euler transient as the batched engine (analysis.rs:169-210, 489-574); the matrix
+ * lives in HBM and is factored by the multifrontal method over a fill-reducing order fixed once per
+ * analysis (static pivot rows from a maximum transversal, approximate minimum degree) in place of the
+ * reference's per-iteration Markowitz search (sparse21/mod.rs:647-991, which grows about as n^2.5).
+ * Components: R, C, V, I, Mos1, Diode. */
+typedef struct s21_large s21_large;
+S21_API int s21_large_create(const s21_netlist* net, const s21_options* opts, int32_t precision, s21_large** out);
+S21_API void s21_large_free(s21_large* l);
+/* Operating point (analysis.rs:383-388); transient with initial conditions (analysis.rs:489-574). */
+S21_API int s21_large_op(s21_large* l);
+S21_API int s21_large_tran(s21_large* l, const s21_tran_options* t);
+/* Unknowns and their names after the last analysis (a transient appends the IC variables). */
+S21_API int32_t s21_large_n_vars(const s21_large* l);
+S21_API const char* s21_large_var_name(const s21_large* l, int32_t k);
+/* Final solution [n_vars]; status (S21_*) and Newton iterations of the last analysis. */
+S21_API int s21_large_get_x(const s21_large* l, double* out);
+S21_API int s21_large_get_status(const s21_large* l, int32_t* status, int64_t* iters);
+/* Transient points stored, their times, and the waveforms of n_sel variables: out[p * n_sel + j]. */
+S21_API int64_t s21_large_n_points(const s21_large* l);
+S21_API int s21_large_get_axis(const s21_large* l, double* out);
+S21_API int s21_large_get_wave(const s21_large* l, int32_t n_sel, const int32_t* vars, double* out);
+/* Structure of the last analysis: nnz of A, nnz of L+U, supernodes (fronts), elimination-tree levels,
+ * largest front and the factorisation's flops. */
+S21_API int s21_large_structure(const s21_large* l, int64_t* nnz_a, int64_t* nnz_lu, int64_t* fronts,
+                                int64_t* levels, int64_t* max_front, double* flops_lu);
+/* Wall time of the last analysis (milliseconds) and factorisations it ran. */
+S21_API int s21_large_timing(const s21_large* l, double* total_ms, int64_t* factorisations);
+/* Factors the last assembled matrix `reps` times, then sweeps the last residual `reps` times, timed by HIP
+ * events on the instance's stream: average milliseconds of each. */
+S21_API int s21_large_time_factor(s21_large* l, int32_t reps, double* factor_ms, double* solve_ms);
+/* Host only (no GPU): the large path's symbolic phase and the CPU reference of its numeric phase on an
+ * n x n matrix of ne entries (duplicates summed): solves A x = b; reports nnz of L+U and flops. */
+S21_API int s21_large_host_check(int32_t n, int64_t ne, const int32_t* rows, const int32_t* cols,
+                                 const double* vals, const double* b, double relax, double* x,
+                                 int64_t* nnz_lu, double* flops_lu);
+
 /* ---- byte ABI mirroring CallableProto::call_bytes (proto.rs:32-145) ------------------------ */
 /* Op -> OpResult, Tran -> TranResult, Ac -> AcResult, all protobuf-encoded.  *out is allocated
  * by the library and released with s21_free_bytes. */

@@ -75,6 +75,22 @@ def lib():
     _sig(L, "s21_batch_profile", C.c_int, vp, i32, P(i64))
     _sig(L, "s21_batch_structure", C.c_int, vp, i32, P(i64), P(i64), P(i64), P(i64))
     _sig(L, "s21_batch_assemble", C.c_int, vp, i32, i32, P(d), P(i32), P(i32), P(d), P(d), i64, P(i64))
+    # single large instance
+    _sig(L, "s21_large_create", C.c_int, P(A.s21_netlist), P(A.s21_options), i32, P(vp))
+    _sig(L, "s21_large_free", None, vp)
+    _sig(L, "s21_large_op", C.c_int, vp)
+    _sig(L, "s21_large_tran", C.c_int, vp, P(A.s21_tran_options))
+    _sig(L, "s21_large_n_vars", i32, vp)
+    _sig(L, "s21_large_var_name", C.c_char_p, vp, i32)
+    _sig(L, "s21_large_get_x", C.c_int, vp, P(d))
+    _sig(L, "s21_large_get_status", C.c_int, vp, P(i32), P(i64))
+    _sig(L, "s21_large_n_points", i64, vp)
+    _sig(L, "s21_large_get_axis", C.c_int, vp, P(d))
+    _sig(L, "s21_large_get_wave", C.c_int, vp, i32, P(i32), P(d))
+    _sig(L, "s21_large_structure", C.c_int, vp, P(i64), P(i64), P(i64), P(i64), P(i64), P(d))
+    _sig(L, "s21_large_timing", C.c_int, vp, P(d), P(i64))
+    _sig(L, "s21_large_time_factor", C.c_int, vp, i32, P(d), P(d))
+    _sig(L, "s21_large_host_check", C.c_int, i32, i64, P(i32), P(i32), P(d), P(d), d, P(d), P(i64), P(d))
     _lib = L
     return L
 

@@ -131,64 +131,10 @@ static void check_device() {
         fail(S21_DEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
 }
 
-// Flattened components of the group for one analysis, IC forcing appended for a transient
-// (Tran::ic, analysis.rs:510-525).
+// Flattened components of the group for one analysis, IC forcing appended for a transient.
 static void build_comps(GroupHost& g, const s21_tran_options* tr, Topology& t) {
-    t.n = g.n_vars;
-    t.names = g.names;
-    t.kinds = g.kinds;
-    t.comps.clear();
-    t.tran = tr != nullptr;
-    for (const s21_comp& c : g.comps) {
-        Comp k{};
-        k.type = c.type;
-        for (int j = 0; j < 6; j++) k.node[j] = c.node[j];
-        k.model = c.model;
-        k.inst = c.inst;
-        k.value = c.value;
-        k.value_tran = c.value;
-        k.acm = c.acm;
-        if (c.type == S21_COMP_MOS1) {
-            if (c.model < 0 || c.model >= (int)g.mos1_models.size() || c.inst < 0 || c.inst >= (int)g.mos1_insts.size())
-                fail(S21_ERR, "Parameters not defined");
-        } else if (c.type == S21_COMP_DIODE) {
-            if (c.model < 0 || c.model >= (int)g.diode_models.size() || c.inst < 0 ||
-                c.inst >= (int)g.diode_insts.size())
-                fail(S21_ERR, "Parameters not defined");
-        }
-        for (int j = 0; j < 6; j++)
-            if (k.node[j] >= g.n_vars || k.node[j] < S21_GND) fail(S21_INVALID, "component node out of range");
-        t.comps.push_back(k);
-    }
-    if (tr) {
-        for (int k = 0; k < tr->n_ic; k++) {
-            const int node = tr->ic_var[k];
-            if (node < 0 || node >= g.n_vars) fail(S21_INVALID, "initial condition node out of range");
-            std::string nm = tr->ic_name ? tr->ic_name[k] : g.names[node];
-            const int fnode = t.n++, ivar = t.n++;
-            t.names.push_back("." + nm + ".vic");
-            t.kinds.push_back(S21_VAR_V);
-            t.names.push_back("." + nm + ".iic");
-            t.kinds.push_back(S21_VAR_I);
-            Comp r{};
-            r.type = S21_COMP_R;
-            r.node[0] = fnode;
-            r.node[1] = node;
-            r.node[2] = r.node[3] = r.node[4] = r.node[5] = S21_GND;
-            r.value = 1.0;
-            r.value_tran = 1e-9;  // analysis.rs:547-549
-            t.comps.push_back(r);
-            Comp v{};
-            v.type = S21_COMP_V;
-            v.node[0] = fnode;
-            v.node[1] = S21_GND;
-            v.node[2] = ivar;
-            v.node[3] = v.node[4] = v.node[5] = S21_GND;
-            v.value = tr->ic_val[k];
-            v.value_tran = 0.0;  // analysis.rs:544-546
-            t.comps.push_back(v);
-        }
-    }
+    build_comps(g.n_vars, g.names, g.kinds, g.comps, (int)g.mos1_models.size(), (int)g.mos1_insts.size(),
+                (int)g.diode_models.size(), (int)g.diode_insts.size(), tr, t);
 }
 
 // Host half of the preparation: the frozen structure, then per-instance derived parameters in the

@@ -100,6 +100,11 @@ struct Topology {
     int64_t flops_lu = 0;
 };
 
+// Components of a flattened netlist for one analysis; a transient appends its initial-condition forcing
+// (a 1 S resistor and a source per IC, then 1e-9 S and 0 V after the operating point).
+void build_comps(int n_vars, const std::vector<std::string>& names, const std::vector<int>& kinds,
+                 const std::vector<s21_comp>& comps, int n_m1_models, int n_m1_insts, int n_d_models, int n_d_insts,
+                 const s21_tran_options* tr, Topology& t);
 // Builds a topology from a flattened netlist (already IC-augmented when tran); `first_values` are the
 // matrix values of the reference's first Newton iteration, in element order, used for pivoting.
 void topo_elements(Topology& t);

@@ -87,6 +87,67 @@ void topo_elements(Topology& t) {
     }
 }
 
+// Flattened components of one analysis, IC forcing appended for a transient (Tran::ic,
+// analysis.rs:510-525).
+void build_comps(int n_vars, const std::vector<std::string>& names, const std::vector<int>& kinds,
+                 const std::vector<s21_comp>& comps, int n_m1_models, int n_m1_insts, int n_d_models, int n_d_insts,
+                 const s21_tran_options* tr, Topology& t) {
+    t.n = n_vars;
+    t.names = names;
+    t.kinds = kinds;
+    t.comps.clear();
+    t.tran = tr != nullptr;
+    for (const s21_comp& c : comps) {
+        Comp k{};
+        k.type = c.type;
+        for (int j = 0; j < 6; j++) k.node[j] = c.node[j];
+        k.model = c.model;
+        k.inst = c.inst;
+        k.value = c.value;
+        k.value_tran = c.value;
+        k.acm = c.acm;
+        if (c.type == S21_COMP_MOS1) {
+            if (c.model < 0 || c.model >= n_m1_models || c.inst < 0 || c.inst >= n_m1_insts)
+                fail(S21_ERR, "Parameters not defined");
+        } else if (c.type == S21_COMP_DIODE) {
+            if (c.model < 0 || c.model >= n_d_models || c.inst < 0 || c.inst >= n_d_insts)
+                fail(S21_ERR, "Parameters not defined");
+        }
+        for (int j = 0; j < 6; j++)
+            if (k.node[j] >= n_vars || k.node[j] < S21_GND) fail(S21_INVALID, "component node out of range");
+        t.comps.push_back(k);
+    }
+    if (tr) {
+        for (int k = 0; k < tr->n_ic; k++) {
+            const int node = tr->ic_var[k];
+            if (node < 0 || node >= n_vars) fail(S21_INVALID, "initial condition node out of range");
+            std::string nm = tr->ic_name ? tr->ic_name[k] : names[node];
+            const int fnode = t.n++, ivar = t.n++;
+            t.names.push_back("." + nm + ".vic");
+            t.kinds.push_back(S21_VAR_V);
+            t.names.push_back("." + nm + ".iic");
+            t.kinds.push_back(S21_VAR_I);
+            Comp r{};
+            r.type = S21_COMP_R;
+            r.node[0] = fnode;
+            r.node[1] = node;
+            r.node[2] = r.node[3] = r.node[4] = r.node[5] = S21_GND;
+            r.value = 1.0;
+            r.value_tran = 1e-9;  // analysis.rs:547-549
+            t.comps.push_back(r);
+            Comp v{};
+            v.type = S21_COMP_V;
+            v.node[0] = fnode;
+            v.node[1] = S21_GND;
+            v.node[2] = ivar;
+            v.node[3] = v.node[4] = v.node[5] = S21_GND;
+            v.value = tr->ic_val[k];
+            v.value_tran = 0.0;  // analysis.rs:544-546
+            t.comps.push_back(v);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Host assembly in the reference's order (analysis.rs:153-168): components in order, stamps in order.
 // ------------------------------------------------------------------------------------------------

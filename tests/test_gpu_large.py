@@ -1,0 +1,111 @@
+"""The single-large-instance path on the GPU: small power grids and the reference's ring oscillators
+against the oracle (FP64 to 1e-9), mixed precision against FP64, and the 200 x 200 power grid (45,040
+unknowns) checked through its KCL residual in the oracle's own assembly."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from spice21_amd import _abi as A
+from spice21_amd import workloads as W
+from spice21_amd.large import FP64, MIXED, Large
+from tests.circuits import GOLDEN_RUNS, ro3
+from tests.test_gpu_parity import wave_within
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def small_grid(size=12):
+    return W.power_grid(size=size, n_inv=size * size // 8, n_diode=size * size // 20, pad_every=4)
+
+
+@pytest.mark.parametrize("size", [8, 16])
+def test_small_power_grid_dcop_matches_oracle(size):
+    nl = small_grid(size)
+    cn = nl.to_c()
+    log("large dcop", size, "unknowns", cn.n_vars)
+    g = Large(cn).op()
+    st, it = g.status()
+    xo, ito, _ = O.dcop(cn)
+    log("  status", st, "iterations", it, "oracle", ito, "structure", g.structure())
+    assert st == A.S21_OK
+    x = g.x()
+    rel = np.abs(x - xo) / np.maximum(np.abs(xo), 1e-9)
+    log("  max relative |x - oracle|", rel.max())
+    assert rel.max() < 1e-9
+    log("large dcop mixed", size)
+    gm = Large(cn, precision=MIXED).op()
+    assert gm.status()[0] == A.S21_OK
+    v = np.asarray(nl.kinds) == A.S21_VAR_V
+    err = np.abs(gm.x() - x)[v].max()
+    log("  mixed vs fp64 max node V", err)
+    assert err <= 1e-6
+
+
+def test_small_power_grid_transient_matches_oracle():
+    nl = small_grid(10)
+    cn = nl.to_c()
+    ics = [("g0_0", 0.0)]
+    log("large tran: oracle")
+    t_o, X_o, names_o, it_o = O.tran(cn, 1e-12, 30e-12, ics)
+    log("large tran: engine")
+    g = Large(cn).tran(1e-12, 30e-12, ics)
+    st, it = g.status()
+    log("  status", st, "iterations", it, "oracle", it_o)
+    assert st == A.S21_OK
+    assert np.array_equal(g.axis(), t_o) and g.var_names() == list(names_o)
+    kinds = list(nl.kinds) + [0, 1]
+    assert wave_within(g.wave(), X_o, names_o, kinds, "grid tran")
+
+
+@pytest.mark.parametrize("fixture", sorted(GOLDEN_RUNS))
+def test_ring_goldens_through_the_large_path(fixture):
+    """The reference's golden ring-oscillator transients run through the large path too."""
+    kind, tstep, tstop = GOLDEN_RUNS[fixture]
+    gold = json.load(open(os.path.join(GOLDEN, fixture)))
+    nl = ro3(kind)
+    if any(c[0] == A.S21_COMP_MOS0 for c in nl.comps):
+        pytest.skip("the large path takes R, C, V, I, Mos1 and Diode components (no Mos0)")
+    log("large golden", fixture)
+    g = Large(nl.to_c()).tran(tstep, tstop, [("1", 0.0)])
+    assert g.status()[0] == A.S21_OK
+    assert np.array_equal(g.axis(), np.array(gold["time"]))
+    names = g.var_names()
+    R = np.stack([np.array(gold[nm]) for nm in names], axis=1)
+    assert wave_within(g.wave(), R, names, list(nl.kinds) + [0, 1], "large golden")
+
+
+def test_full_power_grid_dcop_kcl_and_mixed():
+    """The 200 x 200 grid with 5,000 inverters and 2,000 diodes (45,040 unknowns): FP64 and mixed
+    operating points converge; the FP64 solution's KCL residual in the oracle's assembly is within the
+    reference's current tolerance; mixed agrees with FP64 to 1e-6 V."""
+    nl = W.power_grid()
+    cn = nl.to_c()
+    log("full grid fp64 dcop")
+    g = Large(cn).op()
+    st, it = g.status()
+    ms, nf = g.timing()
+    log("  status", st, "iterations", it, "ms", ms, "factorisations", nf, "structure", g.structure())
+    assert st == A.S21_OK
+    x = g.x()
+    r, c, v, rhs = O.assemble_op(cn, x, max_elems=1 << 22)
+    res = rhs.copy()
+    np.subtract.at(res, np.asarray(r), np.asarray(v) * x[np.asarray(c)])
+    log("  oracle-assembled KCL residual", np.abs(res).max())
+    assert np.abs(res).max() <= 1e-10
+    log("full grid mixed dcop")
+    gm = Large(cn, precision=MIXED).op()
+    stm, itm = gm.status()
+    log("  status", stm, "iterations", itm, "ms", gm.timing()[0])
+    assert stm == A.S21_OK
+    vmask = np.asarray(nl.kinds) == A.S21_VAR_V
+    err = np.abs(gm.x() - x)[vmask].max()
+    log("  mixed vs fp64 max node V", err)
+    assert err <= 1e-6
