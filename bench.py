@@ -115,7 +115,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
 
 
-def extra_measurements(which=("tran", "ac", "ring5")):
+def extra_measurements(which=("tran", "ac", "ring5", "grid")):
     """The metric's transient rate and the other workloads, by HIP-event kernel time (each analysis call
     also rebuilds its host structures, which is not the kernel's throughput)."""
     out = {}
@@ -125,7 +125,55 @@ def extra_measurements(which=("tran", "ac", "ring5")):
         out.update(_ac_amp_ladder())
     if "ring5" in which:
         out.update(_tran_single_ring5())
+    if "grid" in which:
+        out.update(_power_grid_single())
+    if "grid-op" in which:
+        out.update(_power_grid_single(tran=False))
     return out
+
+
+def _power_grid_single(tran=True):
+    """The single large instance: 200 x 200 power-grid mesh with 5,000 inverters and 2,000 diodes (45,040
+    unknowns), dcop then 0-1 ns at 1 ps from a 0.5 V kick on mesh node (100, 100), FP64 and mixed.  Rates
+    are wall time of each analysis (the Newton loop is driven from the host, one read-back per iteration)."""
+    import numpy as np
+
+    from spice21_amd import workloads as W
+    from spice21_amd.large import FP64, MIXED, Large
+
+    nl = W.power_grid()
+    cn = nl.to_c()
+    vmask = np.asarray(list(nl.kinds) + [0, 1]) == 0
+    rec = {"workload": "200x200 power grid, 5000 inverters, 2000 diodes (45040 unknowns): dcop + 0-1 ns at 1 ps, "
+                       "0.5 V kick on mesh node (100, 100)"}
+    xs, ws = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        g = Large(cn, precision=prec).op()
+        st, it = g.status()
+        ms, nf = g.timing()
+        fms, sms = g.time_factor(3)
+        S = g.structure()
+        rec[name] = {"dcop": {"status": st, "newton_iters": it, "ms": ms, "instance_iters_per_s": it / (ms * 1e-3),
+                              "factorisations": nf},
+                     "factor_ms": fms, "sweeps_ms": sms, "factor_tflops": S["flops_lu"] / (fms * 1e-3) / 1e12}
+        xs[name] = g.x()
+        rec["structure"] = S
+        if not tran:
+            continue
+        g.tran(1e-12, 1e-9, [("g100_100", 0.5)])
+        st, it = g.status()
+        ms, nf = g.timing()
+        npts = len(g.axis())
+        rec[name]["tran"] = {"status": st, "points": npts, "newton_iters": it, "ms": ms,
+                             "tran_pts_per_s": npts / (ms * 1e-3), "factorisations": nf}
+        ws[name] = g.wave()
+    v = np.asarray(nl.kinds) == 0
+    rec["mixed_vs_fp64_dcop_max_node_v"] = float(np.abs(xs["mixed"] - xs["fp64"])[v].max())
+    if not tran:
+        return {"power_grid_single": rec}
+    n = min(len(ws["mixed"]), len(ws["fp64"]))
+    rec["mixed_vs_fp64_tran_max_node_v"] = float(np.abs(ws["mixed"][:n] - ws["fp64"][:n])[:, vmask].max())
+    return {"power_grid_single": rec}
 
 
 def _tran_ring_batch():
@@ -236,7 +284,7 @@ def main():
     ap.add_argument("--ipw", type=int, default=0, help="instances per wavefront (implies --no-autotune)")
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the transient / AC / single-ring measurements")
-    ap.add_argument("--extra-only", choices=["tran", "ac", "ring5"], default=None,
+    ap.add_argument("--extra-only", choices=["tran", "ac", "ring5", "grid", "grid-op"], default=None,
                     help="only run one of the other workloads and print its record (for profiling runs)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="'mixed' / 'fp64' time one path only (for profiling runs)")

@@ -296,6 +296,13 @@ S21_API int s21_large_get_wave(const s21_large* l, int32_t n_sel, const int32_t*
  * largest front and the factorisation's flops. */
 S21_API int s21_large_structure(const s21_large* l, int64_t* nnz_a, int64_t* nnz_lu, int64_t* fronts,
                                 int64_t* levels, int64_t* max_front, double* flops_lu);
+/* Factor reuse (no counterpart in the reference, which factors at every iteration): 0 factors every
+ * Newton iteration; 1 lets an iteration keep the factors of an earlier Jacobian (of an earlier iteration or
+ * time point) while the FP64 residual still falls at least fourfold per iteration (at most 32 iterations
+ * in a row), so that Newton converges
+ * to the same solution under the same test as iterative refinement.  Default: 1 in S21_MIXED, 0 in
+ * S21_FP64. */
+S21_API int s21_large_set_reuse(s21_large* l, int32_t on);
 /* Wall time of the last analysis (milliseconds) and factorisations it ran. */
 S21_API int s21_large_timing(const s21_large* l, double* total_ms, int64_t* factorisations);
 /* Factors the last assembled matrix `reps` times, then sweeps the last residual `reps` times, timed by HIP

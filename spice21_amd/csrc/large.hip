@@ -41,9 +41,17 @@ namespace lg {
 
 constexpr int TPB = 256;
 constexpr int NB = 64;                   // panel width of the blocked factorisation of large fronts
-constexpr int CB = 64;                   // column block of the large-front assembly
+constexpr int CB = 16;                   // column block of the large-front assembly
 constexpr size_t SMALL_LDS = 128 * 1024; // LDS bytes a small front may take
 constexpr int M1F = 11, M1C = 4, M1S = 7, DPF = 14, DSF = 4;
+
+// Orders the LDS accesses of one wavefront's lanes (a wavefront executes its LDS operations in order; the
+// fences keep the compiler from moving accesses across this point).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename T>
 constexpr int small_max() {
@@ -173,21 +181,31 @@ __global__ void __launch_bounds__(TPB) k_gather(int n_slots, const int* __restri
 }
 
 // res = rhs - A x by rows of A (elements of each row in creation order); flags[0] |= 1 when some
-// |res_i| > iabstol (false for NaN, as the reference's comparison).
+// |res_i| > iabstol (false for NaN, as the reference's comparison); *resmax = max |res_i| (NaN skipped),
+// as the bits of a non-negative double.
 __global__ void __launch_bounds__(TPB) k_residual(int n, const int* __restrict__ rptr, const int* __restrict__ relem,
                                                   const int* __restrict__ ecol, const double* __restrict__ val,
                                                   const double* __restrict__ rhs, const double* __restrict__ x,
-                                                  double* __restrict__ res, double iabstol, int* flags) {
+                                                  double* __restrict__ res, double iabstol, int* flags,
+                                                  unsigned long long* resmax) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double acc = 0.0;
-    for (int q = rptr[r]; q < rptr[r + 1]; q++) {
-        const int e = relem[q];
-        acc = acc + val[e] * x[ecol[e]];
+    double m = 0.0;
+    if (r < n) {
+        double acc = 0.0;
+        for (int q = rptr[r]; q < rptr[r + 1]; q++) {
+            const int e = relem[q];
+            acc = acc + val[e] * x[ecol[e]];
+        }
+        const double v = rhs[r] - acc;
+        res[r] = v;
+        if (fabs(v) > iabstol) atomicOr(flags, 1);
+        if (fabs(v) == fabs(v)) m = fabs(v);
     }
-    const double v = rhs[r] - acc;
-    res[r] = v;
-    if (fabs(v) > iabstol) atomicOr(flags, 1);
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_xor(m, o, 64);
+        m = u > m ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(resmax, (unsigned long long)__double_as_longlong(m));
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -284,6 +302,7 @@ __global__ void __launch_bounds__(TPB) k_assemble(Fronts S, const int2* __restri
         while (j1 < mc && mp[j1] < c1) j1++;
         const T* U = pool + S.fo[ch];
         const int nj = j1 - lo;
+        if (nj == 0) continue;  // the same for every thread of the block
         for (int q = tid; q < nj * mc; q += TPB) {
             const int j = lo + q / mc, i = q % mc;
             A[(size_t)mp[j] * F + mp[i]] += U[(size_t)(kc + j) * Fc + kc + i];
@@ -292,45 +311,60 @@ __global__ void __launch_bounds__(TPB) k_assemble(Fronts S, const int2* __restri
     }
 }
 
-// Large fronts, panel [c, c + w) of front s = list[blockIdx.x]: LU of the w x w diagonal block in LDS.
+// Value of v in lane l (l the same for every lane), through the scalar unit.
+__device__ inline double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Large fronts, panel [c, c + w) of front s = list[blockIdx.x]: LU of the w x w diagonal block by one
+// wavefront, lane i holding row i in registers; step j reads the pivot row from lane j.
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_diag(Fronts S, const int* __restrict__ list, int panel, T* __restrict__ pool,
-                                              int* flags) {
-    __shared__ T D[NB * NB];
+__global__ void __launch_bounds__(64) k_diag(Fronts S, const int* __restrict__ list, int panel, T* __restrict__ pool,
+                                             int* flags) {
     const int s = list[blockIdx.x];
     const int k = fk(S, s), F = fF(S, s);
     const int c = panel * NB;
     if (c >= k) return;
     const int w = min(NB, k - c);
     T* A = pool + S.fo[s];
-    const int tid = threadIdx.x;
-    for (int q = tid; q < w * w; q += TPB) {
-        const int j = q / w, i = q % w;
-        D[j * NB + i] = A[(size_t)(c + j) * F + c + i];
-    }
-    __syncthreads();
-    for (int j = 0; j < w; j++) {
-        const T piv = D[j * NB + j];
-        if (piv == (T)0 && tid == 0) atomicOr(flags, 2);
-        for (int i = j + 1 + tid; i < w; i += TPB) D[j * NB + i] = D[j * NB + i] / piv;
-        __syncthreads();
-        const int m = w - j - 1;
-        for (int q = tid; q < m * m; q += TPB) {
-            const int cc = j + 1 + q / m, i = j + 1 + q % m;
-            D[cc * NB + i] = D[cc * NB + i] - D[j * NB + i] * D[cc * NB + j];
+    const int lane = threadIdx.x;
+    const bool own = lane < w;
+    T r[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) r[q] = (q < w && own) ? A[(size_t)(c + q) * F + c + lane] : (T)0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j < w) {
+            const T piv = lane_value(r[j], j);
+            if (piv == (T)0 && lane == 0) atomicOr(flags, 2);
+            const bool below = lane > j && own;
+            if (below) r[j] = r[j] / piv;
+            const T l = r[j];
+#pragma unroll
+            for (int q = j + 1; q < NB; q++) {
+                const T u = lane_value(r[q], j);
+                if (below) r[q] = r[q] - l * u;
+            }
         }
-        __syncthreads();
     }
-    for (int q = tid; q < w * w; q += TPB) {
-        const int j = q / w, i = q % w;
-        A[(size_t)(c + j) * F + c + i] = D[j * NB + i];
+    if (own) {
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if (q < w) A[(size_t)(c + q) * F + c + lane] = r[q];
     }
 }
 
 // Large fronts, panel triangular solves: blocks [0, nrow) take TPB rows of L21 each (x U11 = a), the
 // rest TPB columns of U12 each (L11 y = a, unit lower).  grid.y indexes the front list.
+constexpr int TT = 64;  // threads (rows or columns) per block of the panel triangular solves
+
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_trsm(Fronts S, const int* __restrict__ list, int panel, T* __restrict__ pool) {
+__global__ void __launch_bounds__(TT) k_trsm(Fronts S, const int* __restrict__ list, int panel, T* __restrict__ pool) {
     __shared__ T D[NB * NB];
     const int s = list[blockIdx.y];
     const int k = fk(S, s), F = fF(S, s);
@@ -338,18 +372,18 @@ __global__ void __launch_bounds__(TPB) k_trsm(Fronts S, const int* __restrict__ 
     if (c >= k) return;
     const int w = min(NB, k - c);
     const int rest = F - c - w;
-    const int nblk = (rest + TPB - 1) / TPB;
+    const int nblk = (rest + TT - 1) / TT;
     if ((int)blockIdx.x >= 2 * nblk) return;
     T* A = pool + S.fo[s];
     const int tid = threadIdx.x;
-    for (int q = tid; q < w * w; q += TPB) {
+    for (int q = tid; q < w * w; q += TT) {
         const int j = q / w, i = q % w;
         D[j * NB + i] = A[(size_t)(c + j) * F + c + i];
     }
     __syncthreads();
     T v[NB];  // one row (or column) per thread, kept in registers: every loop over it is unrolled
     if ((int)blockIdx.x < nblk) {  // L21 row i: x_j = (a_j - sum_{q<j} x_q U_qj) / U_jj
-        const int i = c + w + blockIdx.x * TPB + tid;
+        const int i = c + w + blockIdx.x * TT + tid;
         if (i >= F) return;
 #pragma unroll
         for (int j = 0; j < NB; j++) v[j] = j < w ? A[(size_t)(c + j) * F + i] : (T)0;
@@ -366,7 +400,7 @@ __global__ void __launch_bounds__(TPB) k_trsm(Fronts S, const int* __restrict__ 
         for (int j = 0; j < NB; j++)
             if (j < w) A[(size_t)(c + j) * F + i] = v[j];
     } else {  // U12 column cc: y_j = a_j - sum_{q<j} L_jq y_q
-        const int cc = c + w + (blockIdx.x - nblk) * TPB + tid;
+        const int cc = c + w + (blockIdx.x - nblk) * TT + tid;
         if (cc >= F) return;
         T* col = A + (size_t)cc * F + c;
 #pragma unroll
@@ -508,6 +542,134 @@ __global__ void __launch_bounds__(TPB) k_backward(Fronts S, const int* __restric
     for (int j = tid; j < k; j += TPB) y[f + j] = w[j];
 }
 
+// Sweeps of large fronts, the fronts of a level per launch (grid.y), panel by panel.
+// Forward: panel [c, c + w) solves L11 x = w[c, c + w) (unit lower, one wavefront), block 0 stores x in y,
+// every block updates its rows below: w_i -= L(i, c..c+w) x.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fwd_panel(Fronts S, const int* __restrict__ list, int panel,
+                                                   const T* __restrict__ pool, T* __restrict__ y, T* __restrict__ wbuf) {
+    __shared__ T Ls[NB][NB + 1];
+    __shared__ T xp[NB];
+    const int s = list[blockIdx.y];
+    const int k = fk(S, s), F = fF(S, s), f = S.first[s];
+    const int c = panel * NB;
+    if (c >= k) return;
+    const int w = min(NB, k - c);
+    const int r0 = c + w;
+    if ((int)blockIdx.x > 0 && (int)blockIdx.x * TPB >= F - r0) return;
+    const T* A = pool + S.fo[s];
+    T* wv = wbuf + S.w_off[s];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < w * w; q += TPB) {
+        const int j = q / w, i = q % w;
+        Ls[j][i] = A[(size_t)(c + j) * F + c + i];
+    }
+    if (tid < w) xp[tid] = wv[c + tid];
+    __syncthreads();
+    if (tid < 64) {
+        for (int j = 0; j < w; j++) {
+            wave_sync();
+            const T xj = xp[j];
+            if (tid > j && tid < w) xp[tid] = xp[tid] - Ls[j][tid] * xj;
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid < w) y[f + c + tid] = xp[tid];
+    const int i = r0 + blockIdx.x * TPB + tid;
+    if (i >= F) return;
+    T acc = wv[i];
+    for (int q = 0; q < w; q++) acc = acc - A[(size_t)(c + q) * F + i] * xp[q];
+    wv[i] = acc;
+}
+
+// Forward of large fronts: the sweep vector from the own right-hand side and the children's updates.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fwd_init(Fronts S, const int* __restrict__ list, const T* __restrict__ y,
+                                                  T* __restrict__ wbuf) {
+    const int s = list[blockIdx.x];
+    const int k = fk(S, s), F = fF(S, s), f = S.first[s];
+    T* w = wbuf + S.w_off[s];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < F; i += TPB) w[i] = i < k ? y[f + i] : (T)0;
+    __syncthreads();
+    for (int cc = S.ch_ptr[s]; cc < S.ch_ptr[s + 1]; cc++) {
+        const int ch = S.ch[cc];
+        const int kc = fk(S, ch), mc = fF(S, ch) - kc;
+        const int* mp = S.x_map + S.x_ptr[ch];
+        const T* wc = wbuf + S.w_off[ch] + kc;
+        for (int q = tid; q < mc; q += TPB) w[mp[q]] += wc[q];
+        __syncthreads();
+    }
+}
+
+// Backward of large fronts: w_j = y_j - U12(j, :) x_R for the front's pivots.  A block of 1024 threads
+// takes 64 pivots (grid.x): lane = pivot, its 16 wavefronts stride over the columns of R, and the 16
+// partial sums are added in wavefront order.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_bwd_init(Fronts S, const int* __restrict__ list, const T* __restrict__ pool,
+                                                   const T* __restrict__ y, T* __restrict__ wbuf) {
+    __shared__ T part[16][64];
+    const int s = list[blockIdx.y];
+    const int k = fk(S, s), F = fF(S, s), f = S.first[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    if ((int)blockIdx.x * 64 >= k) return;
+    const T* A = pool + S.fo[s];
+    const int* R = S.r_idx + S.r_ptr[s];
+    T acc = (T)0;
+    if (j < k)
+        for (int c = k + wv; c < F; c += 16) acc = acc + A[(size_t)c * F + j] * y[R[c - k]];
+    part[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0 && j < k) {
+        T sum = part[0][lane];
+        for (int q = 1; q < 16; q++) sum = sum + part[q][lane];
+        wbuf[S.w_off[s] + j] = y[f + j] - sum;
+    }
+}
+
+// Backward panel [c, c + w), panels from the last: U11 x = w[c, c + w) (one wavefront), block 0 stores x
+// in y, every block updates its rows above: w_i -= U(i, c..c+w) x.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_bwd_panel(Fronts S, const int* __restrict__ list, int panel,
+                                                   const T* __restrict__ pool, T* __restrict__ y, T* __restrict__ wbuf) {
+    __shared__ T Us[NB][NB + 1];
+    __shared__ T xp[NB];
+    const int s = list[blockIdx.y];
+    const int k = fk(S, s), F = fF(S, s), f = S.first[s];
+    const int c = panel * NB;
+    if (c >= k) return;
+    const int w = min(NB, k - c);
+    if ((int)blockIdx.x > 0 && (int)blockIdx.x * TPB >= c) return;
+    const T* A = pool + S.fo[s];
+    T* wv = wbuf + S.w_off[s];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < w * w; q += TPB) {
+        const int j = q / w, i = q % w;
+        Us[j][i] = A[(size_t)(c + j) * F + c + i];
+    }
+    if (tid < w) xp[tid] = wv[c + tid];
+    __syncthreads();
+    if (tid < 64) {
+        for (int j = w - 1; j >= 0; j--) {
+            wave_sync();
+            if (tid == j) xp[j] = xp[j] / Us[j][j];
+            wave_sync();
+            const T xj = xp[j];
+            if (tid < j) xp[tid] = xp[tid] - Us[j][tid] * xj;
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid < w) y[f + c + tid] = xp[tid];
+    const int i = blockIdx.x * TPB + tid;
+    if (i >= c) return;
+    T acc = wv[i];
+    for (int q = 0; q < w; q++) acc = acc - A[(size_t)(c + q) * F + i] * xp[q];
+    wv[i] = acc;
+}
+
 // internal right-hand side from the FP64 residual (rows in pivot order), and dx back in external order
 template <typename T>
 __global__ void k_permute_in(int n, const int* __restrict__ prow, const double* __restrict__ res, T* __restrict__ y) {
@@ -637,6 +799,9 @@ struct s21_large {
     int n_m1 = 0, n_d = 0, n_lin = 0, n_cells = 0, n_elem = 0;
     std::vector<s21::lg::Level> levels;
     std::vector<int> big_host;  // the big-front launch list (host copy)
+    int reuse = -1;             // factor reuse: 1 on, 0 off (factor every iteration), -1 = on in mixed precision
+    bool have_factors = false;  // the pool holds factors of this structure
+    int reused = 0;             // iterations since the last factorisation
     s21::lg::Buf m1_node, m1_par, m1_caps, m1_state, m1_cb, d_node, d_par, d_state, d_cb, l_node, l_type, l_val, c_q,
         l_cb, cv, g_ptr, g_idx, slots, r_ptr, r_elem, e_col, x, res, dx, flags, mxbits;
     s21::lg::Buf f_fo, f_first, f_rptr, f_ridx, f_chptr, f_ch, f_xptr, f_xmap, f_aptr, f_aelem, f_aoff, f_woff;
@@ -958,7 +1123,7 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     L->x.alloc((size_t)std::max(n, 1) * sizeof(double));
     L->res.alloc((size_t)std::max(n, 1) * sizeof(double));
     L->dx.alloc((size_t)std::max(n, 1) * sizeof(double));
-    L->flags.alloc(4 * sizeof(int));
+    L->flags.alloc(8 * sizeof(int));  // 3 flags, padding, the residual's max |.| (8 bytes)
     L->mxbits.alloc(sizeof(unsigned long long));
     upload(L->f_fo, S.front_off);
     upload(L->f_first, S.sn_first);
@@ -979,6 +1144,7 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     upload(L->l_all, lall);
     upload(L->perm, S.perm);
     upload(L->prow, S.prow);
+    L->have_factors = false;
     const size_t tsz = L->precision == S21_MIXED ? sizeof(float) : sizeof(double);
     L->pool.alloc((size_t)std::max<int64_t>(S.pool, 1) * tsz);
     L->yv.alloc((size_t)std::max(n, 1) * tsz);
@@ -1018,13 +1184,13 @@ static void factor(s21_large* L) {
                 const int c = p * NB;
                 if (c >= k) continue;
                 const int w = std::min(NB, k - c), rest = Fs - c - w;
-                tb = std::max(tb, 2 * ((rest + TPB - 1) / TPB));
+                tb = std::max(tb, 2 * ((rest + TT - 1) / TT));
                 const int tl = (rest + 63) / 64;
                 tiles = std::max(tiles, tl * tl);
             }
-            hipLaunchKernelGGL(k_diag<T>, dim3(lv.n_big), dim3(TPB), 0, L->stream, F, lbig + lv.big0, p, pool, flags + 1);
+            hipLaunchKernelGGL(k_diag<T>, dim3(lv.n_big), dim3(64), 0, L->stream, F, lbig + lv.big0, p, pool, flags + 1);
             if (tb > 0)
-                hipLaunchKernelGGL(k_trsm<T>, dim3(tb, lv.n_big), dim3(TPB), 0, L->stream, F, lbig + lv.big0, p, pool);
+                hipLaunchKernelGGL(k_trsm<T>, dim3(tb, lv.n_big), dim3(TT), 0, L->stream, F, lbig + lv.big0, p, pool);
             if (tiles > 0)
                 hipLaunchKernelGGL(k_gemm<T>, dim3(tiles, lv.n_big), dim3(TPB), 0, L->stream, F, lbig + lv.big0, p, pool);
         }
@@ -1040,13 +1206,47 @@ static void solve(s21_large* L) {
     T* y = L->yv.as<T>();
     T* w = L->wv.as<T>();
     const T* pool = L->pool.as<T>();
-    const int* lall = L->l_all.as<int>();
+    const int* lsmall = L->l_small.as<int>();
+    const int* lbig = L->l_big.as<int>();
+    const LargeSym& S = L->S;
     hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->prow.as<int>(),
                        L->res.as<double>(), y);
-    for (const Level& lv : L->levels)
-        hipLaunchKernelGGL(k_forward<T>, dim3(lv.n_all), dim3(TPB), 0, L->stream, F, lall + lv.all0, pool, y, w);
-    for (auto it = L->levels.rbegin(); it != L->levels.rend(); ++it)
-        hipLaunchKernelGGL(k_backward<T>, dim3(it->n_all), dim3(TPB), 0, L->stream, F, lall + it->all0, pool, y, w);
+    // grid sizes of a big-front panel launch: blocks of rows below (forward) or above (backward) the panel
+    auto rows_below = [&](const Level& lv, int p) {
+        int nb = 1;
+        for (int q = 0; q < lv.n_big; q++) {
+            const int s = L->big_host[lv.big0 + q];
+            const int k = S.sn_first[s + 1] - S.sn_first[s], Fs = k + (int)(S.r_ptr[s + 1] - S.r_ptr[s]);
+            if (p * NB < k) nb = std::max(nb, (Fs - p * NB - std::min(NB, k - p * NB) + TPB - 1) / TPB);
+        }
+        return nb;
+    };
+    for (const Level& lv : L->levels) {
+        if (lv.n_small)
+            hipLaunchKernelGGL(k_forward<T>, dim3(lv.n_small), dim3(TPB), 0, L->stream, F, lsmall + lv.small0, pool, y, w);
+        if (!lv.n_big) continue;
+        hipLaunchKernelGGL(k_fwd_init<T>, dim3(lv.n_big), dim3(TPB), 0, L->stream, F, lbig + lv.big0, y, w);
+        for (int p = 0; p < lv.panels; p++)
+            hipLaunchKernelGGL(k_fwd_panel<T>, dim3(rows_below(lv, p), lv.n_big), dim3(TPB), 0, L->stream, F,
+                               lbig + lv.big0, p, pool, y, w);
+    }
+    for (auto it = L->levels.rbegin(); it != L->levels.rend(); ++it) {
+        const Level& lv = *it;
+        if (lv.n_small)
+            hipLaunchKernelGGL(k_backward<T>, dim3(lv.n_small), dim3(TPB), 0, L->stream, F, lsmall + lv.small0, pool, y,
+                               w);
+        if (!lv.n_big) continue;
+        int kmax = 0;
+        for (int q = 0; q < lv.n_big; q++) {
+            const int s = L->big_host[lv.big0 + q];
+            kmax = std::max(kmax, S.sn_first[s + 1] - S.sn_first[s]);
+        }
+        hipLaunchKernelGGL(k_bwd_init<T>, dim3((kmax + 63) / 64, lv.n_big), dim3(1024), 0, L->stream, F,
+                           lbig + lv.big0, pool, y, w);
+        for (int p = lv.panels - 1; p >= 0; p--)
+            hipLaunchKernelGGL(k_bwd_panel<T>, dim3(std::max(1, (p * NB + TPB - 1) / TPB), lv.n_big), dim3(TPB), 0,
+                               L->stream, F, lbig + lv.big0, p, pool, y, w);
+    }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->perm.as<int>(), y,
                        L->dx.as<double>());
     LCHECK(hipGetLastError());
@@ -1060,22 +1260,29 @@ static void assemble_residual(s21_large* L, bool tran, double dt) {
     hipLaunchKernelGGL(k_gather, dim3(blocks(nslot)), dim3(TPB), 0, L->stream, nslot, L->g_ptr.as<int>(),
                        L->g_idx.as<int>(), L->cv.as<double>(), L->slots.as<double>());
     LCHECK(hipMemsetAsync(L->flags.p, 0, sizeof(int), L->stream));
+    LCHECK(hipMemsetAsync(L->flags.as<int>() + 4, 0, sizeof(unsigned long long), L->stream));
     hipLaunchKernelGGL(k_residual, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->r_ptr.as<int>(),
                        L->r_elem.as<int>(), L->e_col.as<int>(), L->slots.as<double>(),
                        L->slots.as<double>() + L->n_elem, L->x.as<double>(), L->res.as<double>(), L->opts.iabstol,
-                       L->flags.as<int>());
+                       L->flags.as<int>(), (unsigned long long*)(L->flags.as<int>() + 4));
     LCHECK(hipGetLastError());
 }
 
-// One Newton solve (Solver::solve, analysis.rs:169-210) from the current x.
+// One Newton solve (Solver::solve, analysis.rs:169-210) from the current x.  Every iteration assembles
+// and tests in FP64.  The FP64 path factors every iteration, as the reference does; with factor reuse on
+// (the mixed path's default) an iteration keeps the FP32 factors of an earlier Jacobian -- of an earlier
+// iteration or time point -- while the FP64 residual still falls at least fourfold per iteration (Newton
+// as iterative refinement, converging to the same solution under the same test), and factors afresh
+// otherwise, or after 32 reuses.
 static int newton(s21_large* L, bool tran, double dt) {
     const int n = L->t.n;
     bool prev_ok = true;  // dx starts at zero
-    LCHECK(hipMemsetAsync(L->flags.p, 0, 4 * sizeof(int), L->stream));
+    LCHECK(hipMemsetAsync(L->flags.p, 0, 8 * sizeof(int), L->stream));
+    double res_prev = 0.0;
     for (int it = 0; it < 100; it++) {  // analysis.rs:173
         L->iters++;
         assemble_residual(L, tran, dt);
-        int hf[4];
+        int hf[8];
         LCHECK(hipMemcpyAsync(hf, L->flags.p, sizeof(hf), hipMemcpyDeviceToHost, L->stream));
         LCHECK(hipStreamSynchronize(L->stream));
         if (hf[1]) return S21_PIVOT;  // the last factorisation met a zero pivot
@@ -1087,14 +1294,31 @@ static int newton(s21_large* L, bool tran, double dt) {
             LCHECK(hipGetLastError());
             return S21_OK;
         }
-        LCHECK(hipMemsetAsync(L->flags.as<int>() + 1, 0, 3 * sizeof(int), L->stream));
-        if (L->precision == S21_MIXED) {
-            factor<float>(L);
-            solve<float>(L);
-        } else {
-            factor<double>(L);
-            solve<double>(L);
+        double res_now;
+        {
+            unsigned long long bits;
+            std::memcpy(&bits, hf + 4, sizeof(bits));
+            res_now = __builtin_bit_cast(double, bits);
         }
+        // the first iteration of a solve has no residual of its own to compare with: it keeps the factors
+        const bool refactor =
+            !L->reuse || !L->have_factors || L->reused >= 32 || (it > 0 && !(res_now <= 0.25 * res_prev));
+        res_prev = res_now;
+        LCHECK(hipMemsetAsync(L->flags.as<int>() + 1, 0, 3 * sizeof(int), L->stream));
+        if (refactor) {
+            if (L->precision == S21_MIXED)
+                factor<float>(L);
+            else
+                factor<double>(L);
+            L->have_factors = true;
+            L->reused = 0;
+        } else {
+            L->reused++;
+        }
+        if (L->precision == S21_MIXED)
+            solve<float>(L);
+        else
+            solve<double>(L);
         LCHECK(hipMemsetAsync(L->mxbits.p, 0, sizeof(unsigned long long), L->stream));
         hipLaunchKernelGGL(k_maxabs, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->dx.as<double>(),
                            L->mxbits.as<unsigned long long>());
@@ -1111,6 +1335,7 @@ static void reset_state(s21_large* L) {
         LCHECK(hipMemsetAsync(b->p, 0, b->bytes, L->stream));
     L->iters = 0;
     L->factorisations = 0;
+    L->have_factors = false;  // each analysis starts from a fresh factorisation
 }
 
 }  // namespace lg
@@ -1184,6 +1409,7 @@ int s21_large_create(const s21_netlist* net, const s21_options* opts, int32_t pr
         else
             s21_default_options(&L->opts);
         L->precision = precision == S21_MIXED ? S21_MIXED : S21_FP64;
+        L->reuse = L->precision == S21_MIXED ? 1 : 0;
         LCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
         LCHECK(hipEventCreate(&L->ev0));
         LCHECK(hipEventCreate(&L->ev1));
@@ -1315,6 +1541,13 @@ int s21_large_timing(const s21_large* L, double* total_ms, int64_t* factorisatio
         if (!L) fail(S21_INVALID, "null instance");
         if (total_ms) *total_ms = L->ms_total;
         if (factorisations) *factorisations = L->factorisations;
+    });
+}
+
+int s21_large_set_reuse(s21_large* L, int32_t on) {
+    return lguard([&] {
+        if (!L) fail(S21_INVALID, "null instance");
+        L->reuse = on ? 1 : 0;
     });
 }
 

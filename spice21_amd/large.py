@@ -39,6 +39,11 @@ class Large:
         except Exception:
             pass
 
+    def set_reuse(self, on):
+        """Factor reuse across Newton iterations (default on in mixed precision, off in FP64)."""
+        check(lib().s21_large_set_reuse(self.h, 1 if on else 0))
+        return self
+
     def op(self):
         check(lib().s21_large_op(self.h))
         return self
