@@ -282,6 +282,20 @@ S21_API void s21_large_free(s21_large* l);
 /* Operating point (analysis.rs:383-388); transient with initial conditions (analysis.rs:489-574). */
 S21_API int s21_large_op(s21_large* l);
 S21_API int s21_large_tran(s21_large* l, const s21_tran_options* t);
+/* Trapezoidal transient (no counterpart in the reference, whose trapezoidal branch panics,
+ * analysis.rs:429-435; its companion formula is the one that branch holds: g = 2 C / dt,
+ * i = 2 dq / dt - i_prev, each charge with its own previous current).  lte = 0: fixed steps of tstep;
+ * lte = 1: steps chosen by the local truncation error of the node voltages (third divided difference over
+ * the last four accepted points, in FP64 on the device) against trtol * (reltol * |v| + abstol), at most
+ * tstep, a rejected step retried from the last accepted point; results are linearly interpolated onto the
+ * reference's output grid (t = 0, tstep, 2 tstep, ...).  "Timestep too small" below dt_min (0: 1e-9 tstep). */
+typedef struct {
+    int32_t lte;
+    double reltol, abstol, trtol, dt_min;
+} s21_trap_options;
+S21_API int s21_large_tran_trap(s21_large* l, const s21_tran_options* t, const s21_trap_options* o);
+/* Accepted and rejected steps of the last trapezoidal transient. */
+S21_API int s21_large_trap_stats(const s21_large* l, int64_t* accepted, int64_t* rejected);
 /* Unknowns and their names after the last analysis (a transient appends the IC variables). */
 S21_API int32_t s21_large_n_vars(const s21_large* l);
 S21_API const char* s21_large_var_name(const s21_large* l, int32_t k);

@@ -133,3 +133,8 @@ class s21_group(C.Structure):
         ("n_over", C.c_int32),
         ("over", C.POINTER(s21_override)),
     ]
+
+
+class s21_trap_options(C.Structure):
+    _fields_ = [("lte", C.c_int32), ("reltol", C.c_double), ("abstol", C.c_double), ("trtol", C.c_double),
+                ("dt_min", C.c_double)]

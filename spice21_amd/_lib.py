@@ -91,6 +91,8 @@ def lib():
     _sig(L, "s21_large_timing", C.c_int, vp, P(d), P(i64))
     _sig(L, "s21_large_time_factor", C.c_int, vp, i32, P(d), P(d))
     _sig(L, "s21_large_set_reuse", C.c_int, vp, i32)
+    _sig(L, "s21_large_tran_trap", C.c_int, vp, P(A.s21_tran_options), P(A.s21_trap_options))
+    _sig(L, "s21_large_trap_stats", C.c_int, vp, P(i64), P(i64))
     _sig(L, "s21_large_host_check", C.c_int, i32, i64, P(i32), P(i32), P(d), P(d), d, P(d), P(i64), P(d))
     _lib = L
     return L

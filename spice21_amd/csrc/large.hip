@@ -80,6 +80,14 @@ struct EvalArgs {
     const double* c_q;    // [n_lin] committed capacitor charge
     const int* l_cb;
     double* cv;           // contribution cells
+    // trapezoidal integration: trap = 1 integrates by the trapezoidal rule, 2 by Backward Euler while
+    // recording each charge's current (the first step, from an operating point whose capacitor currents
+    // are zero), 0 by Backward Euler alone.  Per charge, the current at the last accepted point
+    // (committed) and the one of the current guess.
+    int trap;
+    double* m1_icur;      // [4][n_m1]: committed gate-drain (tr.gs) and gate-bulk currents, then guesses
+    double* d_icur;       // [2][n_d]: committed junction-charge current, then guess
+    double* c_icur;       // [2][n_lin]: committed capacitor current, then guess
 };
 
 __device__ inline double xval(const double* x, int v) { return v < 0 ? 0.0 : x[v]; }
@@ -103,10 +111,20 @@ __global__ void __launch_bounds__(TPB) k_eval(EvalArgs E) {
         Mos1State op{st[0 * nm + j], st[1 * nm + j], st[2 * nm + j], st[3 * nm + j],
                      st[4 * nm + j], st[5 * nm + j], st[6 * nm + j]};
         Mos1State gs{};
-        if (E.tran)
-            mos1_load<true>(P, &K, &op, vd, vg, vs, vb, E.gmin, E.dt, put, &gs);
-        else
+        if (E.tran) {
+            if (E.trap) {
+                const double ip[2] = {E.m1_icur[(size_t)0 * nm + j], E.m1_icur[(size_t)1 * nm + j]};
+                double io[2];
+                mos1_load<true>(P, &K, &op, vd, vg, vs, vb, E.gmin, E.dt, put, &gs, nullptr, E.trap == 1 ? ip : nullptr,
+                                io);
+                E.m1_icur[(size_t)2 * nm + j] = io[0];
+                E.m1_icur[(size_t)3 * nm + j] = io[1];
+            } else {
+                mos1_load<true>(P, &K, &op, vd, vg, vs, vb, E.gmin, E.dt, put, &gs);
+            }
+        } else {
             mos1_load<false>(P, &K, &op, vd, vg, vs, vb, E.gmin, E.dt, put, &gs);
+        }
         st[7 * nm + j] = gs.vgs;
         st[8 * nm + j] = gs.vgd;
         st[9 * nm + j] = gs.vgb;
@@ -127,7 +145,10 @@ __global__ void __launch_bounds__(TPB) k_eval(EvalArgs E) {
         auto put = [&](int e, double v) { cv[e] = v; };
         DiodeGuess gs;
         const double vr = xval(E.x, nd[DR_]), vn = xval(E.x, nd[DN_]);
-        if (E.tran)
+        if (E.tran && E.trap)
+            diode_load<true>(P, vr, vn, E.gmin, E.dt, vd0, st[2 * nd_ + j], put, &gs,
+                             E.trap == 1 ? E.d_icur + j : nullptr, E.d_icur + nd_ + j);
+        else if (E.tran)
             diode_load<true>(P, vr, vn, E.gmin, E.dt, vd0, st[2 * nd_ + j], put, &gs);
         else
             diode_load<false>(P, vr, vn, E.gmin, E.dt, vd0, st[2 * nd_ + j], put, &gs);
@@ -153,7 +174,8 @@ __global__ void __launch_bounds__(TPB) k_eval(EvalArgs E) {
                 for (int e = 0; e < C_NE; e++) cv[e] = 0.0;
             } else {
                 cap_load_tran(val, xval(E.x, nd[0]) - xval(E.x, nd[1]), E.c_q[j], E.dt,
-                              [&](int e, double v) { cv[e] = v; });
+                              [&](int e, double v) { cv[e] = v; }, E.trap == 1 ? E.c_icur + j : nullptr,
+                              E.trap ? E.c_icur + E.n_lin + j : nullptr);
             }
             break;
         case S21_COMP_V:
@@ -714,11 +736,13 @@ __global__ void k_commit(EvalArgs E, double* c_q) {
     if (t < E.n_m1) {
         if (!E.m1_caps[t]) return;
         for (int f = 0; f < M1S; f++) E.m1_state[(size_t)f * E.n_m1 + t] = E.m1_state[(size_t)(M1S + f) * E.n_m1 + t];
+        for (int f = 0; f < 2; f++) E.m1_icur[(size_t)f * E.n_m1 + t] = E.m1_icur[(size_t)(2 + f) * E.n_m1 + t];
         return;
     }
     if (t < E.n_m1 + E.n_d) {
         const int j = t - E.n_m1;
         E.d_state[2 * E.n_d + j] = E.d_state[1 * E.n_d + j];
+        E.d_icur[j] = E.d_icur[E.n_d + j];
         return;
     }
     if (t < E.n_m1 + E.n_d + E.n_lin) {
@@ -726,7 +750,42 @@ __global__ void k_commit(EvalArgs E, double* c_q) {
         if (E.l_type[j] != S21_COMP_C) return;
         const int* nd = E.l_node + 3 * j;
         c_q[j] = E.l_val[j] * (xval(E.x, nd[0]) - xval(E.x, nd[1]));  // guess.q = c * vd
+        E.c_icur[j] = E.c_icur[E.n_lin + j];
     }
+}
+
+// Local truncation error of a trapezoidal step from the third divided difference DD3 of the node voltages
+// over the last four accepted points (x, h1, h2, h3 at t0 > t1 > t2 > t3): the error is dt^3 / 12 times the
+// third derivative, which is about 6 DD3.  *rmax = max over voltage unknowns of that error over
+// trtol (reltol max(|x|, |h1|) + abstol).
+__global__ void k_lte(int n, const int* __restrict__ kind, const double* __restrict__ x, const double* __restrict__ h1,
+                      const double* __restrict__ h2, const double* __restrict__ h3, double t0, double t1, double t2,
+                      double t3, double reltol, double abstol, double trtol, unsigned long long* rmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double r = 0.0;
+    if (i < n && kind[i] == S21_VAR_V) {
+        const double d01 = (x[i] - h1[i]) / (t0 - t1), d12 = (h1[i] - h2[i]) / (t1 - t2),
+                     d23 = (h2[i] - h3[i]) / (t2 - t3);
+        const double d012 = (d01 - d12) / (t0 - t2), d123 = (d12 - d23) / (t1 - t3);
+        const double d3 = (d012 - d123) / (t0 - t3);
+        const double dt = t0 - t1;
+        const double lte = dt * dt * dt / 2.0 * fabs(d3);
+        const double tol = trtol * (reltol * fmax(fabs(x[i]), fabs(h1[i])) + abstol);
+        r = lte / tol;
+        if (!(r == r)) r = 1e300;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_xor(r, o, 64);
+        r = u > r ? u : r;
+    }
+    if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(rmax, (unsigned long long)__double_as_longlong(r));
+}
+
+// out = a + w (b - a): an output-grid point between two accepted points
+__global__ void k_interp(int n, const double* __restrict__ a, const double* __restrict__ b, double w,
+                         double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] + w * (b[i] - a[i]);
 }
 
 }  // namespace lg
@@ -807,6 +866,10 @@ struct s21_large {
     s21::lg::Buf f_fo, f_first, f_rptr, f_ridx, f_chptr, f_ch, f_xptr, f_xmap, f_aptr, f_aelem, f_aoff, f_woff;
     s21::lg::Buf l_small, l_big, l_jobs, l_all, perm, prow, pool, yv, wv;
     s21::lg::Buf wave;
+    // trapezoidal integration: per-charge currents, unknown kinds, accepted-point history
+    s21::lg::Buf m1_icur, d_icur, c_icur, kinds_dev, hist[4];
+    int trap = 0;
+    int64_t accepted = 0, rejected = 0;
     // results
     int status = S21_OK;
     int64_t iters = 0, npts = 0, factorisations = 0;
@@ -918,6 +981,10 @@ static EvalArgs eval_args(const s21_large* L, bool tran, double dt) {
     E.c_q = L->c_q.as<double>();
     E.l_cb = L->l_cb.as<int>();
     E.cv = L->cv.as<double>();
+    E.trap = L->trap;
+    E.m1_icur = L->m1_icur.as<double>();
+    E.d_icur = L->d_icur.as<double>();
+    E.c_icur = L->c_icur.as<double>();
     return E;
 }
 
@@ -1113,6 +1180,10 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     L->m1_state.alloc(std::max<size_t>((size_t)2 * M1S * nm, 1) * sizeof(double));
     L->d_state.alloc(std::max<size_t>((size_t)DSF * nd, 1) * sizeof(double));
     L->c_q.alloc(std::max<size_t>(nl, 1) * sizeof(double));
+    L->m1_icur.alloc(std::max<size_t>((size_t)4 * nm, 1) * sizeof(double));
+    L->d_icur.alloc(std::max<size_t>((size_t)2 * nd, 1) * sizeof(double));
+    L->c_icur.alloc(std::max<size_t>((size_t)2 * nl, 1) * sizeof(double));
+    upload(L->kinds_dev, t.kinds);
     L->cv.alloc(std::max<size_t>(cells, 1) * sizeof(double));
     upload(L->g_ptr, cnt);
     upload(L->g_idx, gidx);
@@ -1252,6 +1323,14 @@ static void solve(s21_large* L) {
     LCHECK(hipGetLastError());
 }
 
+// Commit of every device's guess (Component::commit) at the accepted solution.
+static void commit_state(s21_large* L, bool tran, double dt) {
+    const EvalArgs E = eval_args(L, tran, dt);
+    hipLaunchKernelGGL(k_commit, dim3(blocks(L->n_m1 + L->n_d + L->n_lin)), dim3(TPB), 0, L->stream, E,
+                       L->c_q.as<double>());
+    LCHECK(hipGetLastError());
+}
+
 // Evaluation, gather and residual at the current x; flags[0] set when the residual fails the test.
 static void assemble_residual(s21_large* L, bool tran, double dt) {
     const int n = L->t.n, nslot = L->n_elem + n;
@@ -1274,7 +1353,7 @@ static void assemble_residual(s21_large* L, bool tran, double dt) {
 // iteration or time point -- while the FP64 residual still falls at least fourfold per iteration (Newton
 // as iterative refinement, converging to the same solution under the same test), and factors afresh
 // otherwise, or after 32 reuses.
-static int newton(s21_large* L, bool tran, double dt) {
+static int newton(s21_large* L, bool tran, double dt, bool commit = true) {
     const int n = L->t.n;
     bool prev_ok = true;  // dx starts at zero
     LCHECK(hipMemsetAsync(L->flags.p, 0, 8 * sizeof(int), L->stream));
@@ -1288,10 +1367,7 @@ static int newton(s21_large* L, bool tran, double dt) {
         if (hf[1]) return S21_PIVOT;  // the last factorisation met a zero pivot
         if (it > 0) prev_ok = hf[2] == 0;
         if (prev_ok && hf[0] == 0) {
-            const EvalArgs E = eval_args(L, tran, dt);
-            hipLaunchKernelGGL(k_commit, dim3(blocks(L->n_m1 + L->n_d + L->n_lin)), dim3(TPB), 0, L->stream, E,
-                               L->c_q.as<double>());
-            LCHECK(hipGetLastError());
+            if (commit) commit_state(L, tran, dt);
             return S21_OK;
         }
         double res_now;
@@ -1331,7 +1407,7 @@ static int newton(s21_large* L, bool tran, double dt) {
 }
 
 static void reset_state(s21_large* L) {
-    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x})
+    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x, &L->m1_icur, &L->d_icur, &L->c_icur})
         LCHECK(hipMemsetAsync(b->p, 0, b->bytes, L->stream));
     L->iters = 0;
     L->factorisations = 0;
@@ -1429,6 +1505,7 @@ int s21_large_op(s21_large* L) {
             L->prepared = "op";
         }
         reset_state(L);
+        L->trap = 0;
         const auto t0 = std::chrono::steady_clock::now();
         L->status = newton(L, false, 0.0);
         LCHECK(hipStreamSynchronize(L->stream));
@@ -1447,6 +1524,7 @@ int s21_large_tran(s21_large* L, const s21_tran_options* tr) {
         prepare(L, tr);
         L->prepared = "tran";
         reset_state(L);
+        L->trap = 0;
         const int n = L->t.n;
         const int64_t npts = s21_tran_point_count(tr->tstep, tr->tstop);
         L->wave.alloc((size_t)npts * n * sizeof(double));
@@ -1473,6 +1551,124 @@ int s21_large_tran(s21_large* L, const s21_tran_options* tr) {
             L->axis[p] = t;
             t += tr->tstep;
         }
+    });
+}
+
+int s21_large_tran_trap(s21_large* L, const s21_tran_options* tr, const s21_trap_options* o) {
+    return lguard([&] {
+        if (!L || !tr || !o) fail(S21_INVALID, "null argument");
+        if (!(tr->tstep > 0.0) || !(tr->tstop >= 0.0)) fail(S21_INVALID, "tstep must be positive");
+        check_current_device(L);
+        L->prepared.clear();
+        prepare(L, tr);
+        L->prepared = "tran";
+        reset_state(L);
+        L->trap = 0;  // the operating point integrates nothing
+        const int n = L->t.n;
+        const int64_t npts = s21_tran_point_count(tr->tstep, tr->tstop);
+        std::vector<double> axis(npts, 0.0);  // the reference's output grid (t += tstep)
+        {
+            double t = tr->tstep;
+            for (int64_t p = 1; p < npts; p++) {
+                axis[p] = t;
+                t += tr->tstep;
+            }
+        }
+        L->wave.alloc((size_t)npts * n * sizeof(double));
+        for (Buf& h : L->hist) h.alloc((size_t)std::max(n, 1) * sizeof(double));
+        L->accepted = L->rejected = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        L->status = newton(L, false, 0.0);
+        int64_t stored = 0;
+        double* W = L->wave.as<double>();
+        auto copy = [&](double* dst, const void* src) {
+            LCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, L->stream));
+        };
+        if (L->status == S21_OK) {
+            copy(W, L->x.p);
+            stored = 1;
+            const double dtmax = tr->tstep;
+            const double dtmin = o->dt_min > 0.0 ? o->dt_min : dtmax * 1e-9;
+            // hist[0] = x at the last accepted point (time th[0]), hist[1..3] the ones before
+            int h[4] = {0, 1, 2, 3};
+            double th[4] = {0.0, 0.0, 0.0, 0.0};
+            int nh = 1;
+            copy(L->hist[h[0]].as<double>(), L->x.p);
+            double t = 0.0, dt = o->lte ? dtmax / 4.0 : dtmax;
+            L->trap = 2;  // the first step is Backward Euler, as SPICE starts: the operating point's capacitor
+                          // currents (zero) do not describe the circuit released from its IC forcing
+            const double t_end = axis[npts - 1];
+            int64_t k = 1;
+            while (k < npts) {
+                if (o->lte) dt = std::min(dt, t_end - t);
+                const double tn = o->lte ? t + dt : axis[k];
+                L->status = newton(L, true, dt, false);
+                double r = 0.0;
+                if (L->status == S21_OK && o->lte && nh >= 3) {
+                    LCHECK(hipMemsetAsync(L->mxbits.p, 0, sizeof(unsigned long long), L->stream));
+                    hipLaunchKernelGGL(k_lte, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->kinds_dev.as<int>(),
+                                       L->x.as<double>(), L->hist[h[0]].as<double>(), L->hist[h[1]].as<double>(),
+                                       L->hist[h[2]].as<double>(), tn, th[0], th[1], th[2], o->reltol, o->abstol,
+                                       o->trtol, L->mxbits.as<unsigned long long>());
+                    unsigned long long bits = 0;
+                    LCHECK(hipMemcpyAsync(&bits, L->mxbits.p, sizeof(bits), hipMemcpyDeviceToHost, L->stream));
+                    LCHECK(hipStreamSynchronize(L->stream));
+                    r = __builtin_bit_cast(double, bits);
+                }
+                if (L->status != S21_OK || r > 1.0) {  // reject: back to the last accepted point
+                    if (!o->lte) break;
+                    L->rejected++;
+                    dt = L->status != S21_OK ? dt / 4.0 : dt * std::max(0.25, 0.9 * std::cbrt(1.0 / r));
+                    copy(L->x.as<double>(), L->hist[h[0]].p);
+                    if (dt < dtmin) {
+                        L->status = S21_CONVERGENCE;
+                        fail(S21_CONVERGENCE, "Timestep too small");
+                    }
+                    L->status = S21_OK;
+                    continue;
+                }
+                commit_state(L, true, dt);
+                L->accepted++;
+                L->trap = 1;
+                if (o->lte) {  // output-grid points in (t, tn], linearly between the accepted points
+                    while (k < npts && axis[k] <= tn * (1.0 + 1e-12)) {
+                        const double w = std::min(1.0, (axis[k] - t) / (tn - t));
+                        hipLaunchKernelGGL(k_interp, dim3(blocks(n)), dim3(TPB), 0, L->stream, n,
+                                           L->hist[h[0]].as<double>(), L->x.as<double>(), w, W + (size_t)k * n);
+                        k++;
+                    }
+                } else {
+                    copy(W + (size_t)k * n, L->x.p);
+                    k++;
+                }
+                stored = k;
+                // history: the new point in front
+                const int last = h[3];
+                for (int q = 3; q > 0; q--) {
+                    h[q] = h[q - 1];
+                    th[q] = th[q - 1];
+                }
+                h[0] = last;
+                th[0] = tn;
+                copy(L->hist[h[0]].as<double>(), L->x.p);
+                nh = std::min(nh + 1, 4);
+                t = tn;
+                if (o->lte && nh >= 4) dt = std::min(dtmax, dt * (r > 0.0 ? std::min(2.0, 0.9 * std::cbrt(1.0 / r)) : 2.0));
+            }
+        }
+        LCHECK(hipStreamSynchronize(L->stream));
+        L->trap = 0;
+        L->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        L->npts = stored;
+        L->axis.assign(axis.begin(), axis.begin() + stored);
+    });
+}
+
+int s21_large_trap_stats(const s21_large* L, int64_t* accepted, int64_t* rejected) {
+    return lguard([&] {
+        if (!L) fail(S21_INVALID, "null instance");
+        if (accepted) *accepted = L->accepted;
+        if (rejected) *rejected = L->rejected;
     });
 }
 

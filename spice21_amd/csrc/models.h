@@ -86,16 +86,36 @@ __host__ __device__ inline Integ be_integ(double dq, double dq_dv, double vguess
     double rhs = i - g * vguess;
     return Integ{g, i, rhs};
 }
+// Trapezoidal companion, the formula the reference keeps disabled (analysis.rs:429-435): ip is the charge's
+// current at the previous accepted point.
+__host__ __device__ inline Integ trap_integ(double dq, double dq_dv, double vguess, double dt, double ip) {
+    double g = 2.0 * dq_dv / dt;
+    double i = 2.0 * dq / dt - ip;
+    double rhs = i - g * vguess;
+    return Integ{g, i, rhs};
+}
+// Integration of one charge: Backward Euler, or trapezoidal when ip (the previous current) is given; the
+// new current goes to *i_out when given.
+__host__ __device__ inline Integ integ(double dq, double dq_dv, double vguess, double dt, const double* ip,
+                                       double* i_out) {
+    const Integ r = ip ? trap_integ(dq, dq_dv, vguess, dt, *ip) : be_integ(dq, dq_dv, vguess, dt);
+    if (i_out) *i_out = r.i;
+    return r;
+}
 
 // Mos1::op_stamp (comps/mos.rs:649-893).  Terminal voltages vd, vg, vs, vb.  Calls put(entry, value)
 // for the 22 matrix stamps and 4 RHS stamps in the reference's stamp order, stamp k going to entry
 // sigma(k) when source and drain are swapped.  TRAN selects the transient branch; with caps == nullptr
 // the device has no capacitance and the integration terms are zero.  guess (optional) receives the new
 // operating point.
+// trap_ip / trap_i (optional, transient only): the previous and new currents of the two integrated gate
+// charges (the gate-drain one the stamps carry in tr.gs, then the gate-bulk one) for trapezoidal
+// integration; without them the integration is Backward Euler.
 template <bool TRAN, class Put>
 __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* caps, const Mos1State* op, double vd_,
                                           double vg_, double vs_, double vb_, double gmin, double dt, Put&& put,
-                                          Mos1State* guess, Mos1AcOp* acop = nullptr) {
+                                          Mos1State* guess, Mos1AcOp* acop = nullptr, const double* trap_ip = nullptr,
+                                          double* trap_i = nullptr) {
     const double p = P.p;
     const bool reversed = p * (vd_ - vs_) < 0.0;
     const double vd = reversed ? vs_ : vd_;
@@ -172,10 +192,11 @@ __host__ __device__ inline void mos1_load(const Mos1Core& P, const Mos1Caps* cap
             const double cgd = cgd1 + caps->cgd_ov + (same ? op->cgd : op->cgs);
             const double cgb = cgb1 + caps->cgb_ov + op->cgb;
             const double dqgd = same ? (vgd - op->vgd) * cgd : (vgd - op->vgs) * cgd;
-            const Integ a = be_integ(dqgd, cgd, vgd, dt);
+            const Integ a = integ(dqgd, cgd, vgd, dt, trap_ip, trap_i);
             trgs_g = a.g;
             trgs_rhs = a.rhs;
-            const Integ b = be_integ((vgb - op->vgb) * cgb, cgb, vgb, dt);
+            const Integ b = integ((vgb - op->vgb) * cgb, cgb, vgb, dt, trap_ip ? trap_ip + 1 : nullptr,
+                                  trap_i ? trap_i + 1 : nullptr);
             trgb_g = b.g;
             trgb_rhs = b.rhs;
         }
@@ -351,9 +372,11 @@ __host__ __device__ inline double diode_limit(const DiodeP& P, double vnew, doub
     }
     return P.vte * log(vnew / P.vte);
 }
+// trap_ip / trap_i (optional): the junction charge's previous and new current for trapezoidal integration.
 template <bool TRAN, class Put>
 __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn, double gmin, double dt,
-                                           double prev_guess_vd, double op_charge, Put&& put, DiodeGuess* guess) {
+                                           double prev_guess_vd, double op_charge, Put&& put, DiodeGuess* guess,
+                                           const double* trap_ip = nullptr, double* trap_i = nullptr) {
     double vd = vr - vn;
     const bool has_bv = P.has_bv != 0.0;
     if (has_bv && vd < fmin(10.0 * P.vte - P.bv, 0.0)) {
@@ -383,7 +406,7 @@ __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn
         cd = P.tt + P.cz2 * P.f3 + P.m * vd / P.vj;
     }
     if (TRAN) {
-        const Integ c = be_integ(qd - op_charge, cd, vd, dt);
+        const Integ c = integ(qd - op_charge, cd, vd, dt, trap_ip, trap_i);
         id += c.i;
         gd += c.g;
     }
@@ -408,11 +431,13 @@ __host__ __device__ inline void diode_load(const DiodeP& P, double vr, double vn
 // Vsrc: (p,i) (i,p) (n,i) (i,n) plus RHS i.  Isrc: RHS p, n.
 constexpr int R_NE = 4, C_NE = 6, V_NE = 5, I_NE = 2;
 
-// Capacitor transient load: returns the companion (g, rhs) of the BE integration.
+// Capacitor transient load: the companion (g, rhs) of the BE integration, or of the trapezoidal one when
+// trap_ip (the previous current) is given; the new current goes to *trap_i.
 template <class Put>
-__host__ __device__ inline void cap_load_tran(double c, double vd, double op_q, double dt, Put&& put) {
+__host__ __device__ inline void cap_load_tran(double c, double vd, double op_q, double dt, Put&& put,
+                                              const double* trap_ip = nullptr, double* trap_i = nullptr) {
     const double q = c * vd;
-    const Integ a = be_integ(q - op_q, c, vd, dt);
+    const Integ a = integ(q - op_q, c, vd, dt, trap_ip, trap_i);
     put(0, a.g);
     put(1, a.g);
     put(2, -a.g);

@@ -53,6 +53,20 @@ class Large:
         check(lib().s21_large_tran(self.h, C.byref(t)))
         return self
 
+    def tran_trap(self, tstep, tstop, ics=(), lte=True, reltol=1e-3, abstol=1e-6, trtol=7.0, dt_min=0.0):
+        """Trapezoidal transient: fixed steps of tstep (lte=False) or steps chosen by the local truncation
+        error with tstep as the largest, interpolated onto the output grid t = k tstep."""
+        t = make_tran_options(tstep, tstop, ics, self.cnet.var_names)
+        o = A.s21_trap_options(1 if lte else 0, reltol, abstol, trtol, dt_min)
+        check(lib().s21_large_tran_trap(self.h, C.byref(t), C.byref(o)))
+        return self
+
+    def trap_stats(self):
+        """(accepted, rejected) steps of the last trapezoidal transient."""
+        a, r = C.c_int64(), C.c_int64()
+        check(lib().s21_large_trap_stats(self.h, C.byref(a), C.byref(r)))
+        return a.value, r.value
+
     def n_vars(self):
         return lib().s21_large_n_vars(self.h)
 
