@@ -193,6 +193,11 @@ S21_API int s21_batch_tran(s21_batch* b, const s21_tran_options* t);
 /* Operating point, then complex small-signal sweep (analysis.rs:761-832). */
 S21_API int s21_batch_ac(s21_batch* b, const s21_ac_options* a);
 
+/* The reference's AC result files for one instance of the last sweep, written into `dir` (analysis.rs:
+ * 783-827): `stream.ac.json` (per frequency a flat array [f, re0, im0, re1, im1, ...]) and `ac.json` (the
+ * serialised AcResult: signals, freq, data as [re, im] pairs, and the map, empty as the reference writes
+ * it).  s21_ac_bytes writes them into the working directory after its sweep, as the reference's `ac` does. */
+S21_API int s21_batch_write_ac_results(const s21_batch* b, int32_t group, int32_t inst, const char* dir);
 S21_API int32_t s21_batch_n_groups(const s21_batch* b);
 S21_API int32_t s21_batch_n_inst(const s21_batch* b, int32_t group);
 /* Unknowns of the group after the last analysis (includes IC variables after a transient). */

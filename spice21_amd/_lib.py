@@ -49,6 +49,7 @@ def lib():
     _sig(L, "s21_batch_free", None, vp)
     _sig(L, "s21_batch_op", C.c_int, vp)
     _sig(L, "s21_batch_op_noalloc", C.c_int, vp, P(d), P(i32), P(i64))
+    _sig(L, "s21_batch_write_ac_results", C.c_int, vp, i32, i32, C.c_char_p)
     _sig(L, "s21_batch_tran", C.c_int, vp, P(A.s21_tran_options))
     _sig(L, "s21_batch_ac", C.c_int, vp, P(A.s21_ac_options))
     _sig(L, "s21_batch_n_groups", i32, vp)

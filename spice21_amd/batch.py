@@ -164,6 +164,10 @@ class Batch:
         check(lib().s21_batch_get_wave(self.h, g, inst, _dp(out)))
         return out
 
+    def write_ac_results(self, directory, g=0, inst=0):
+        """The reference's AC result files (stream.ac.json, ac.json) of one instance, into `directory`."""
+        check(lib().s21_batch_write_ac_results(self.h, g, inst, str(directory).encode()))
+
     def timing(self):
         ms, nl, avg = C.c_double(), C.c_int64(), C.c_double()
         check(lib().s21_batch_timing(self.h, C.byref(ms), C.byref(nl), C.byref(avg)))

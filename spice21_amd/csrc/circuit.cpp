@@ -841,6 +841,8 @@ int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len) 
         engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
         engine_check(s21_batch_ac(bp.b, &ao));
         check_status(bp.b);
+        // the reference's `ac` writes its result files into the working directory (analysis.rs:783-827)
+        engine_check(s21_batch_write_ac_results(bp.b, 0, 0, "."));
         const int64_t np = s21_batch_n_points(bp.b);
         const int nv = s21_batch_n_vars(bp.b, 0);
         std::vector<double> f(np), wv((size_t)np * nv * 2);

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -1440,6 +1442,102 @@ int s21_batch_ac(s21_batch* b, const s21_ac_options* a) {
         for (size_t gi = 0; gi < b->groups.size(); gi++)
             for (int i = 0; i < b->groups[gi].n_inst; i++)
                 if (dc_status[gi][i] != S21_OK) b->groups[gi].status[i] = dc_status[gi][i];
+    });
+}
+
+// The reference's AC result files (analysis.rs:783-827), for one instance of the last sweep: the streamed
+// sequence `stream.ac.json`, one flat array [f, re0, im0, re1, im1, ...] per frequency, and the in-memory
+// result `ac.json`, serialised before the reference fills its map: {"signals", "freq", "data" (per
+// frequency, [re, im] per signal, as num-complex serialises a Complex), "map": {}}.  Numbers are written
+// in the shortest form that reads back to the same double; non-finite ones as null, as serde_json does.
+// Unlike the reference, which streams as it sweeps, the files are written once the sweep has succeeded.
+static void json_num(std::string& o, double v) {
+    if (!std::isfinite(v)) {
+        o += "null";
+        return;
+    }
+    char buf[40];
+    for (int prec = 1; prec <= 17; prec++) {  // the shortest text that reads back to the same double
+        std::snprintf(buf, sizeof buf, "%.*g", prec, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    o += buf;
+}
+static void json_str(std::string& o, const std::string& s) {
+    o += '"';
+    for (char c : s) {
+        if (c == '"' || c == '\\') {
+            o += '\\';
+            o += c;
+        } else if ((unsigned char)c < 0x20) {
+            char buf[8];
+            std::snprintf(buf, sizeof buf, "\\u%04x", (unsigned)(unsigned char)c);
+            o += buf;
+        } else {
+            o += c;
+        }
+    }
+    o += '"';
+}
+static void write_file(const std::string& path, const std::string& text) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) fail(S21_ERR, "cannot write " + path);
+    const size_t w = std::fwrite(text.data(), 1, text.size(), f);
+    const int c = std::fclose(f);
+    if (w != text.size() || c != 0) fail(S21_ERR, "cannot write " + path);
+}
+
+int s21_batch_write_ac_results(const s21_batch* b, int32_t g, int32_t inst, const char* dir) {
+    return guard([&] {
+        if (!b || g < 0 || g >= (int)b->groups.size() || b->last_mode != MODE_AC) fail(S21_INVALID, "no AC sweep to write");
+        const GroupHost& gh = b->groups[g];
+        if (inst < 0 || inst >= gh.n_inst) fail(S21_INVALID, "instance out of range");
+        const int n = gh.topo.n;
+        const int64_t np = b->npts;
+        std::vector<double> wv((size_t)np * n * 2);
+        if (s21_batch_get_wave(b, g, inst, wv.data()) != S21_OK) fail(S21_ERR, "AC waveform unavailable");
+        const std::string base = dir && *dir ? std::string(dir) + "/" : std::string();
+        std::string st = "[";
+        for (int64_t p = 0; p < np; p++) {
+            if (p) st += ',';
+            st += '[';
+            json_num(st, b->axis[p]);
+            for (int k = 0; k < n; k++) {
+                st += ',';
+                json_num(st, wv[((size_t)p * n + k) * 2]);
+                st += ',';
+                json_num(st, wv[((size_t)p * n + k) * 2 + 1]);
+            }
+            st += ']';
+        }
+        st += ']';
+        std::string js = "{\"signals\":[";
+        for (int k = 0; k < n; k++) {
+            if (k) js += ',';
+            json_str(js, gh.topo.names[k]);
+        }
+        js += "],\"freq\":[";
+        for (int64_t p = 0; p < np; p++) {
+            if (p) js += ',';
+            json_num(js, b->axis[p]);
+        }
+        js += "],\"data\":[";
+        for (int64_t p = 0; p < np; p++) {
+            if (p) js += ',';
+            js += '[';
+            for (int k = 0; k < n; k++) {
+                if (k) js += ',';
+                js += '[';
+                json_num(js, wv[((size_t)p * n + k) * 2]);
+                js += ',';
+                json_num(js, wv[((size_t)p * n + k) * 2 + 1]);
+                js += ']';
+            }
+            js += ']';
+        }
+        js += "],\"map\":{}}";
+        write_file(base + "stream.ac.json", st);
+        write_file(base + "ac.json", js);
     });
 }
 

@@ -1,6 +1,8 @@
 """The reference's Python-binding tests (spice21py/spice21py/tests/test_spice21.py), run through the MI355X
 engine's byte ABI, with the oracle as the checker where the reference's own test leaves values unchecked.
 Needs the GPU."""
+import json
+
 import numpy as np
 import pytest
 
@@ -54,7 +56,8 @@ def test_tran1():  # test_spice21.py:42-50
     assert all(v == 1.0 for v in res["1"])
 
 
-def test_ac1():  # test_spice21.py:53-70
+def test_ac1(tmp_path, monkeypatch):  # test_spice21.py:53-70
+    monkeypatch.chdir(tmp_path)  # the AC analysis writes its result files into the working directory
     c = circuit(Resistor(p="inp", n="out", g=1e-3), Capacitor(p="out", c=1e-9), Vsrc(p="inp", dc=1e-3, acm=1))
     for args in (None, protos.AcOptions(fstart=1, fstop=1_000_000_000, npts=90)):
         res = api.ac(c, args=args)
@@ -67,6 +70,32 @@ def test_ac1():  # test_spice21.py:53-70
         for k in range(1, len(res["out"])):  # low-pass: the output magnitude falls
             assert abs(res["out"][k]) <= abs(res["out"][k - 1])
     assert len(res["out"]) == 91
+
+
+def test_ac_result_files(tmp_path, monkeypatch):
+    """The reference's `ac` writes stream.ac.json (per frequency [f, re0, im0, ...] in variable order) and
+    ac.json (signals, freq, data as [re, im] pairs, and an empty map: it is serialised before the map is
+    filled) into the working directory (analysis.rs:783-827); both hold the same numbers as the result."""
+    monkeypatch.chdir(tmp_path)
+    c = circuit(Resistor(p="inp", n="out", g=1e-3), Capacitor(p="out", c=1e-9), Vsrc(name="v1", p="inp", acm=1))
+    res = api.ac(c, args=protos.AcOptions(fstart=10, fstop=100_000_000, npts=35))
+    stream = json.load(open(tmp_path / "stream.ac.json"))
+    full = json.load(open(tmp_path / "ac.json"))
+    assert set(full) == {"signals", "freq", "data", "map"} and full["map"] == {}
+    names = full["signals"]
+    assert sorted(names) == sorted(res)
+    assert len(stream) == len(full["freq"]) == len(full["data"]) == len(res["out"]) == 36
+    for p, (row, f, d) in enumerate(zip(stream, full["freq"], full["data"])):
+        assert row[0] == f and len(row) == 1 + 2 * len(names) and len(d) == len(names)
+        for k, nm in enumerate(names):
+            z = res[nm][p]
+            assert row[1 + 2 * k] == z.real and row[2 + 2 * k] == z.imag  # shortest round-trip text
+            assert d[k] == [z.real, z.imag]
+    assert full["freq"][0] == 10.0 and full["freq"][-1] == 100_000_000.0
+    # the RC low-pass's closed form at every frequency
+    w = 2 * np.pi * np.array(full["freq"])
+    out = np.array([complex(*d[names.index("out")]) for d in full["data"]])
+    assert np.abs(out - 1.0 / (1.0 + 1j * w * 1e-9 / 1e-3)).max() <= 1e-9
 
 
 def test_mos1_ro_matches_oracle():
