@@ -57,6 +57,9 @@ static inline Cx cx_scale_mul(Cx a, double s) { return {a.re * s, a.im * s}; }
 static inline Cx cx_scale_div(Cx a, double s) { return {a.re / s, a.im / s}; }
 static inline double absv(double x) { return std::fabs(x); }
 static inline double absv(Cx z) { return std::hypot(z.re, z.im); }
+// Diagnostics (not part of the restated algorithm): factorisations, and those whose pivot search chose
+// other elements than the same matrix's previous factorisation did.
+static std::atomic<long long> g_factorisations{0}, g_pivot_changes{0};
 static inline bool is_zero(double x) { return x == 0.0; }
 static inline bool is_zero(Cx z) { return z.re == 0.0 && z.im == 0.0; }
 template <class T> static inline T zero();
@@ -392,16 +395,23 @@ template <class T> struct Matrix : Maker {
         state = FACTORING;
         axes[ROWS].setup_factoring();
         axes[COLS].setup_factoring();
+        std::vector<Eidx> seq;
         for (size_t n = 0; n < diag.size() - 1; n++) {
             Eidx pivot = search_for_pivot(n);
             if (pivot == NONE) sperror("Pivot Search Fail");
+            seq.push_back(pivot);
             swap(ROWS, el[pivot].row, n);
             swap(COLS, el[pivot].col, n);
             row_col_elim(pivot, n);
         }
         state = FACTORED;
         factorizations++;
+        // diagnostics: did this factorisation's search pick other elements than the previous one's?
+        if (factorizations > 1 && seq != last_pivots) g_pivot_changes++;
+        g_factorisations++;
+        last_pivots.swap(seq);
     }
+    std::vector<Eidx> last_pivots;
     Eidx search_for_pivot(size_t n) const {  // :676-686
         Eidx ei = markowitz_search_diagonal(n);
         if (ei != NONE) return ei;
@@ -1606,6 +1616,14 @@ static int code_of(const std::string& d) {
 extern "C" {
 
 void oracle_plant(int k) { g_plant = k; }
+void oracle_pivot_stats(long long* factorisations, long long* changes, int reset) {
+    if (factorisations) *factorisations = g_factorisations.load();
+    if (changes) *changes = g_pivot_changes.load();
+    if (reset) {
+        g_factorisations = 0;
+        g_pivot_changes = 0;
+    }
+}
 
 /* DC operating point (analysis.rs:383-388).  x_out[n_vars]; iters_out: Newton iterations. */
 int oracle_dcop(const s21_netlist* net, const s21_options* o, double* x_out, int64_t* iters_out, int64_t* facts_out,

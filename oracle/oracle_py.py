@@ -58,6 +58,8 @@ def lib():
         L.oracle_mat_check.argtypes = [C.c_void_p]
         L.oracle_plant.argtypes = [C.c_int]
         L.oracle_plant.restype = None
+        L.oracle_pivot_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
+        L.oracle_pivot_stats.restype = None
         _lib = L
     return _lib
 
@@ -282,3 +284,11 @@ class planted:
     def __exit__(self, *exc):
         lib().oracle_plant(0)
         return False
+
+
+def pivot_stats(reset=True):
+    """(factorisations, factorisations whose pivot search chose other elements than the previous
+    factorisation of the same matrix) since the last reset."""
+    f, c = C.c_longlong(), C.c_longlong()
+    lib().oracle_pivot_stats(C.byref(f), C.byref(c), int(reset))
+    return f.value, c.value

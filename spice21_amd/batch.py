@@ -173,6 +173,13 @@ class Batch:
         check(lib().s21_batch_timing(self.h, C.byref(ms), C.byref(nl), C.byref(avg)))
         return ms.value, nl.value, avg.value
 
+    def resumptions(self):
+        """Instance-launches of the last operating point / transient that resumed an instance on a new
+        pivot order (its kept order met a zero pivot, or one so small that a multiplier overflowed)."""
+        n = C.c_int64()
+        check(lib().s21_batch_resumptions(self.h, C.byref(n)))
+        return n.value
+
     def set_profiling(self, on=True):
         check(lib().s21_batch_set_profiling(self.h, 1 if on else 0))
         return self

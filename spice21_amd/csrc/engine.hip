@@ -96,6 +96,7 @@ struct s21_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, kernel_avg_ms = 0.0;
     int64_t launches = 0;
+    int64_t resumed = 0;  // instance resumptions of the last analysis (launches of recover_pivots)
     int64_t npts = 0;
     int profile = 0;
     int last_waves = 1, last_ipw = 1;
@@ -816,11 +817,11 @@ static void group_from(GroupHost& g, const s21_group& sg) {
     }
 }
 
-// ---- zero-pivot recovery ----
+// ---- pivot recovery ----
 // The reference searches its pivots again at every factorisation (Matrix::solve after each reset,
-// sparse21/mod.rs:647-674, 929-932); the engine keeps the first search's order, which is the order those
-// searches return for as long as it stays acceptable.  An instance whose kept order meets an exact zero
-// pivot (status S21_PIVOT) resumes the interrupted iteration -- of the operating point or of a transient
+// sparse21/mod.rs:647-674, 929-932); the engine keeps the first search's order for as long as every pivot
+// of it is non-zero and its multipliers stay finite (factor_forward).  An instance whose kept order meets
+// a pivot that fails this (status S21_PIVOT) resumes the interrupted iteration -- of the operating point or of a transient
 // time point -- in a launch of its own: a mixed-precision instance first in FP64 on the same order, then
 // on the order the reference's search picks on that iteration's matrix (assembled on the GPU from the
 // instance's state), searched from the arrangement and fill its earlier factorisations left.  The
@@ -930,7 +931,8 @@ static size_t wave_offset(const s21_batch* b, int gi, int64_t npts) {
     return off;
 }
 
-static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_t npts) {
+static void recover_pivots(s21_batch* b, const s21_tran_options* tr, int64_t npts) {
+    b->resumed = 0;
     std::vector<Stalled> todo;
     for (size_t gi = 0; gi < b->groups.size(); gi++) {
         GroupHost& g = b->groups[gi];
@@ -1038,6 +1040,7 @@ static void recover_zero_pivots(s21_batch* b, const s21_tran_options* tr, int64_
             A.npts = npts;
         }
         run(&sb, A, tr ? (double*)wave->p : nullptr, tr ? npts : 1);
+        b->resumed += (int64_t)todo.size();
         sb.fetched = false;
         fetch_results(&sb);
         std::vector<Stalled> next;
@@ -1239,7 +1242,7 @@ int s21_batch_op(s21_batch* b) {
         Args A = base_args(b, MODE_OP);
         run(b, A, nullptr, 1);
         b->fetched = false;
-        recover_zero_pivots(b, nullptr, 1);
+        recover_pivots(b, nullptr, 1);
         b->npts = 0;
         b->axis.clear();
         b->last_mode = MODE_OP;
@@ -1264,7 +1267,7 @@ int s21_batch_tran(s21_batch* b, const s21_tran_options* tr) {
         A.npts = npts;
         run(b, A, (double*)b->d_wave->p, npts);
         b->fetched = false;
-        recover_zero_pivots(b, tr, npts);
+        recover_pivots(b, tr, npts);
         b->npts = npts;
         b->axis.resize(npts);
         double t = 0.0;
@@ -1658,6 +1661,12 @@ int s21_batch_get_wave(const s21_batch* b, int32_t g, int32_t inst, double* out)
         const size_t per = (size_t)b->npts * gh.topo.n * vw;
         HIPCHECK(hipMemcpy(out, (double*)b->d_wave->p + off + (size_t)inst * per, per * sizeof(double),
                            hipMemcpyDeviceToHost));
+    });
+}
+int s21_batch_resumptions(const s21_batch* b, int64_t* instance_rounds) {
+    return guard([&] {
+        if (!b || !instance_rounds) fail(S21_INVALID, "bad arguments");
+        *instance_rounds = b->resumed;
     });
 }
 int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, double* kernel_avg_ms) {

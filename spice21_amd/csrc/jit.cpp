@@ -78,20 +78,20 @@ std::string emit_steps(const std::vector<GenStep>& st) {
 void gen_straight(std::ostringstream& o, const Topology& t, const std::string& fn, const std::string& sfx) {
     o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
       << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
-         "    V y, l, ck;\n";
+         "    V y, l, ck, mx = V(0);\n";
     for (int i = 0; i < t.n; i++) {
         if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;  // no L entries
         o << "    y = (V)c[" << i << "];\n";
         for (int q = t.rp[i]; q < t.diag[i]; q++) {
             const int k = t.ci[q];
-            o << "    l = ldiv<V>(lu[" << q << "], lu[" << t.diag[k] << "]); lu[" << q << "] = l; ck = (V)c[" << k
+            o << "    l = ldiv<V>(lu[" << q << "], lu[" << t.diag[k] << "]); lu[" << q << "] = l; mx = fmax(mx, fabs(l)); ck = (V)c[" << k
               << "]; if (ck != V(0)) y = y - ck * l;\n";
             for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++)
                 o << "    lu[" << t.upd_dst[x] << "] = lu[" << t.upd_dst[x] << "] - lu[" << t.upd_src[x] << "] * l;\n";
         }
         o << "    c[" << i << "] = (double)y;\n";
     }
-    o << "    bool z = false;\n";
+    o << "    bool z = __builtin_isinf(mx);\n";  // an overflowed multiplier: a pivot too small to keep
     for (int i = 0; i + 1 < t.n; i++) o << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
     o << "    return z;\n}\n";
     o << "template <typename V>\n" << fn << " void bw_gen" << sfx
@@ -152,8 +152,8 @@ void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, 
                 for (int q = q0; q < q1; q++) {
                     const int k = t.ci[q];
                     const std::string a = use(q), rpiv = use(t.diag[k]), yk = usec(k);  // pivot reciprocal
-                    body << "    l = " << a << " * " << rpiv << "; " << yi << " = __builtin_fmaf(-" << yk << ", l, "
-                         << yi << ");\n";
+                    body << "    l = " << a << " * " << rpiv << "; mx = fmaxf(mx, fabsf(l)); " << yi
+                         << " = __builtin_fmaf(-" << yk << ", l, " << yi << ");\n";
                     for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++) {
                         const std::string src = use(t.upd_src[x]), d = use(t.upd_dst[x]);
                         body << "    " << d << " = __builtin_fmaf(-" << src << ", l, " << d << ");\n";
@@ -180,12 +180,12 @@ void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, 
         }
         o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
           << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
-             "    V l, m = __builtin_inff();\n";
+             "    V l, m = __builtin_inff(), mx = V(0);\n";
         for (int q = 0; q < t.nnz; q++)
             if (used[q]) o << "    V v" << q << ";\n";
         for (int k = 0; k < t.n; k++)
             if (cused[k]) o << "    V y" << k << ";\n";
-        o << pre.str() << emit_steps(steps) << "    return m == V(0);\n}\n";
+        o << pre.str() << emit_steps(steps) << "    return m == V(0) || __builtin_isinf(mx);\n}\n";
     }
     // ---- backward sweep ----
     {

@@ -533,11 +533,16 @@ __device__ inline float ldiv<float>(float a, float b) {
 // Factorisation with the forward sweep fused in.  A chunk of row i processes L entries (i, k) once row k
 // is final: l_ik = a_ik / u_kk, a_ij -= l_ik u_kj over row k's U entries, and c_i -= c_k l_ik -- for each
 // target in increasing k, the order of the reference's right-looking elimination and column-oriented
-// forward substitution (sparse21/mod.rs:865-919, 946-964).  Returns whether this lane saw a zero pivot
-// other than the last (the assertion at sparse21/mod.rs:872).
+// forward substitution (sparse21/mod.rs:865-919, 946-964).  Returns whether this lane saw a pivot the
+// kept order cannot use: a zero one other than the last (the assertion at sparse21/mod.rs:872), or one so
+// small against its column that a multiplier l_ik = a_ik / u_kk overflows to infinity (in the factors'
+// precision).  The reference's own search exchanges a diagonal below 1e-3 of its column
+// (sparse21/mod.rs:213-216, 757-762); the engine keeps such a pivot, as its Newton loop tests a residual
+// computed from the assembled matrix in FP64 and needs the factors only for the update.
 template <typename V>
 __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
     const Idx& I = C.I;
+    bool small_pivot = false;
     for (int l = 0; l < C.T.n_clvl; l++) {
         const int e = I.clvl[l + 1];
         for (int ch = act ? I.clvl[l] + C.sl : e; ch < e; ch += C.L) {
@@ -548,6 +553,7 @@ __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
                 const R4 L = r4(I.fl[j]);
                 const V lv = ldiv<V>(lu[L.x], lu[L.y]);
                 lu[L.x] = lv;
+                small_pivot = small_pivot || __builtin_isinf(lv);
                 const V ck = (V)C.c[L.z];
                 if (ck != V(0)) y = y - ck * lv;  // the sweep skips zero entries (sparse21/mod.rs:953)
                 for (int m = 0; m < L.w; m++, u++) {
@@ -560,7 +566,7 @@ __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
         }
         wave_sync();
     }
-    bool zero_pivot = false;
+    bool zero_pivot = small_pivot;
     for (int i = act ? C.sl : C.T.n; i < C.T.n - 1; i += C.L)
         if (lu[I.diag[i]] == V(0)) zero_pivot = true;
     return zero_pivot;
@@ -1233,7 +1239,7 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
         }
         __syncthreads();
         if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
-        if (act && s_act[slot] == 2) {  // a zero pivot other than the last (sparse21/mod.rs:872)
+        if (act && s_act[slot] == 2) {  // a pivot the reference's search would not take (factor_forward)
             st = S21_PIVOT;
             act = false;
         }

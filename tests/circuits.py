@@ -165,3 +165,31 @@ def random_ac_circuit(seed):
     for nd in nodes:  # keep every node connected
         n.r(1e-6, nd, "")
     return n
+
+
+def switching_diode(I=3.0, G=0.1, C=1e-7, isat=1e-3):
+    """A diode from a 0 V source node b to an RC node a that a current source charges toward I/G.  At the
+    first Newton iteration (x = 0) the diode's conductance isat/vte passes the reference's pivot threshold,
+    so the first search pivots on it; as a charges, the diode turns off and its conductance falls below
+    1e-3 of the source's unit entry in its column (a above ~0.1 V), below 1e-6 of it (a above ~0.3 V) and,
+    with gmin = 0 and no junction charge, to exactly zero (exp underflow, a above ~19.3 V).  The
+    reference's later searches then pivot on the source's entries instead."""
+    from spice21_amd.netlist import Netlist
+
+    n = Netlist()
+    n.v("vb", 0.0, "b", "")
+    n.r(G, "a", "")
+    n.c(C, "a", "")
+    n.i(I, "a", "")
+    dm = n.diode_model(**{"is": isat})
+    di = n.diode_inst()
+    n.diode("d1", "b", "a", dm, di)
+    return n
+
+
+def no_gmin():
+    from spice21_amd import _abi as A
+
+    o = A.s21_options.default()
+    o.gmin = 0.0
+    return o

@@ -302,3 +302,28 @@ def test_ac_common_source_gain_closed_form():
         out = X[:, n.names.index("d")]
         assert np.max(np.abs(out - gain)) <= 1e-9 * abs(gain), (lam, rd, out[0], gain)
         assert np.all(X[:, n.names.index("g")] == 1.0 + 0.0j)
+
+
+def test_switching_diode_repivots_and_follows_the_off_state_recurrence():
+    """The switching-diode circuit with gmin = 0: the reference's pivot searches choose other elements as the
+    diode turns off (pivot_stats counts factorisations whose search differs from the previous one's), and
+    once the diode's conductance has underflowed to zero (a above 19.3 V) the node follows the Backward
+    Euler recurrence of the RC with the reverse saturation current and the IC forcing conductance left
+    after the operating point: C (a1 - a0)/dt = I - isat - (G + gic) a1."""
+    from tests.circuits import no_gmin, switching_diode
+
+    I, G, C, isat, dt, gic = 3.0, 0.1, 1e-7, 1e-3, 1e-8, 1e-9
+    cn = switching_diode(I, G, C, isat).to_c()
+    O.pivot_stats()
+    t, X, names, _ = O.tran(cn, dt, 3e-6, [("a", 0.0)], opts=no_gmin())
+    fact, changes = O.pivot_stats()
+    assert fact > 300 and changes >= 1
+    a = X[:, names.index("a")]
+    off = np.nonzero(a > 19.5)[0]
+    assert len(off) > 100
+    k = off[1:]
+    pred = (C / dt * a[k - 1] + I - isat) / (C / dt + G + gic)
+    assert np.abs(a[k] - pred).max() <= 1e-10
+    # the source holds b at 0 V and carries the diode's reverse current
+    assert np.all(X[:, names.index("b")] == 0.0)
+    assert np.abs(X[k, names.index("vb")] - isat).max() <= 1e-12
