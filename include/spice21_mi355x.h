@@ -183,8 +183,8 @@ S21_API int s21_batch_op(s21_batch* b);
  *     group.  x's contents are ignored: the solve starts from x = 0 as the reference's does.
  *   - The Newton state lives in a fixed 64 MiB device array of the library (one per GPU, kept until
  *     the process ends, cleared by each call); a batch whose state needs more returns S21_INVALID.
- *   - An instance whose frozen pivot order meets a pivot it cannot keep (zero, or so small that a
- *     multiplier overflows) reports S21_PIVOT (s21_batch_op resumes such instances
+ *   - An instance whose frozen pivot order meets a pivot it cannot keep (zero, or below the smallest
+ *     normal number of the factors' precision) reports S21_PIVOT (s21_batch_op resumes such instances
  *     on a new order in a launch of its own, which allocates).
  *   - The batch's own results (s21_batch_get_*) are left unchanged; s21_batch_timing reports this call.
  *   - Every call on one GPU shares that GPU's array: calls that overlap on one GPU race. */
@@ -220,8 +220,8 @@ S21_API int s21_batch_get_wave(const s21_batch* b, int32_t group, int32_t inst, 
  * number of those launches, and the average duration of the dominant kernel. */
 S21_API int s21_batch_timing(const s21_batch* b, double* total_ms, int64_t* launches, double* kernel_avg_ms);
 /* Resumptions of the last operating point or transient: the instance-launches that continued an instance
- * whose kept pivot order met a pivot it cannot keep (zero, sparse21/mod.rs:872, or so small that a
- * multiplier overflows), each on the order the reference's search picks at that factorisation
+ * whose kept pivot order met a pivot it cannot keep (zero, sparse21/mod.rs:872, or below the smallest
+ * normal number of the factors' precision), each on the order the reference's search picks there
  * (sparse21/mod.rs:647-919).  0 when every instance kept its order throughout. */
 S21_API int s21_batch_resumptions(const s21_batch* b, int64_t* instance_rounds);
 

@@ -175,7 +175,7 @@ class Batch:
 
     def resumptions(self):
         """Instance-launches of the last operating point / transient that resumed an instance on a new
-        pivot order (its kept order met a zero pivot, or one so small that a multiplier overflowed)."""
+        pivot order (its kept order met a pivot that is zero or below the smallest normal number)."""
         n = C.c_int64()
         check(lib().s21_batch_resumptions(self.h, C.byref(n)))
         return n.value

@@ -566,7 +566,7 @@ static void prepare_jit(s21_batch* b, bool ac = false, int64_t n_freq = 0) {
         group_pattern.push_back(it->second);
     }
     if (ops > JIT_MAX_TOTAL_OPS) return;
-    b->jitk = jit_compile(jit_source(pats, group_pattern, b->precision == S21_MIXED, 2, ac));
+    b->jitk = jit_compile(jit_source(pats, group_pattern, b->precision == S21_MIXED, 2, ac, b->profile != 0));
     b->has_jit = true;
 }
 
@@ -591,7 +591,7 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
 
 template <bool MIXED>
 static void launch(s21_batch* b, const GroupGPU* dgroups, int n_groups, int n_blocks, size_t smem, const Args& A) {
-    auto kern = newton_kernel<MIXED>;
+    auto kern = A.prof ? newton_kernel_prof<MIXED> : newton_kernel<MIXED>;
     if (smem > 64 * 1024)
         HIPCHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(WAVE * A.waves), smem, b->stream, dgroups, n_groups, A);
@@ -820,7 +820,7 @@ static void group_from(GroupHost& g, const s21_group& sg) {
 // ---- pivot recovery ----
 // The reference searches its pivots again at every factorisation (Matrix::solve after each reset,
 // sparse21/mod.rs:647-674, 929-932); the engine keeps the first search's order for as long as every pivot
-// of it is non-zero and its multipliers stay finite (factor_forward).  An instance whose kept order meets
+// of it is a normal number, neither zero nor subnormal (factor_forward).  An instance whose kept order meets
 // a pivot that fails this (status S21_PIVOT) resumes the interrupted iteration -- of the operating point or of a transient
 // time point -- in a launch of its own: a mixed-precision instance first in FP64 on the same order, then
 // on the order the reference's search picks on that iteration's matrix (assembled on the GPU from the

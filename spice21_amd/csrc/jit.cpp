@@ -78,21 +78,21 @@ std::string emit_steps(const std::vector<GenStep>& st) {
 void gen_straight(std::ostringstream& o, const Topology& t, const std::string& fn, const std::string& sfx) {
     o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
       << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
-         "    V y, l, ck, mx = V(0);\n";
+         "    V y, l, ck;\n";
     for (int i = 0; i < t.n; i++) {
         if (t.diag[i] < 0 || t.diag[i] == t.rp[i]) continue;  // no L entries
         o << "    y = (V)c[" << i << "];\n";
         for (int q = t.rp[i]; q < t.diag[i]; q++) {
             const int k = t.ci[q];
-            o << "    l = ldiv<V>(lu[" << q << "], lu[" << t.diag[k] << "]); lu[" << q << "] = l; mx = fmax(mx, fabs(l)); ck = (V)c[" << k
+            o << "    l = ldiv<V>(lu[" << q << "], lu[" << t.diag[k] << "]); lu[" << q << "] = l; ck = (V)c[" << k
               << "]; if (ck != V(0)) y = y - ck * l;\n";
             for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++)
                 o << "    lu[" << t.upd_dst[x] << "] = lu[" << t.upd_dst[x] << "] - lu[" << t.upd_src[x] << "] * l;\n";
         }
         o << "    c[" << i << "] = (double)y;\n";
     }
-    o << "    bool z = __builtin_isinf(mx);\n";  // an overflowed multiplier: a pivot too small to keep
-    for (int i = 0; i + 1 < t.n; i++) o << "    z = z || lu[" << t.diag[i] << "] == V(0);\n";
+    o << "    bool z = false;\n";  // a pivot zero or below the smallest normal number (factor_forward)
+    for (int i = 0; i + 1 < t.n; i++) o << "    z = z || pivot_unusable(lu[" << t.diag[i] << "]);\n";
     o << "    return z;\n}\n";
     o << "template <typename V>\n" << fn << " void bw_gen" << sfx
       << "(LDS const V* __restrict__ lu, LDS V* __restrict__ c) {\n"
@@ -152,7 +152,7 @@ void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, 
                 for (int q = q0; q < q1; q++) {
                     const int k = t.ci[q];
                     const std::string a = use(q), rpiv = use(t.diag[k]), yk = usec(k);  // pivot reciprocal
-                    body << "    l = " << a << " * " << rpiv << "; mx = fmaxf(mx, fabsf(l)); " << yi
+                    body << "    l = " << a << " * " << rpiv << "; " << yi
                          << " = __builtin_fmaf(-" << yk << ", l, " << yi << ");\n";
                     for (int x = t.upd_ptr[q]; x < t.upd_ptr[q + 1]; x++) {
                         const std::string src = use(t.upd_src[x]), d = use(t.upd_dst[x]);
@@ -180,12 +180,12 @@ void gen_steps(std::ostringstream& o, const Topology& t, const std::string& fn, 
         }
         o << "template <typename V>\n" << fn << " bool lu_gen" << sfx
           << "(LDS V* __restrict__ lu, LDS V* __restrict__ c) {\n"
-             "    V l, m = __builtin_inff(), mx = V(0);\n";
+             "    V l, m = __builtin_inff();\n";
         for (int q = 0; q < t.nnz; q++)
             if (used[q]) o << "    V v" << q << ";\n";
         for (int k = 0; k < t.n; k++)
             if (cused[k]) o << "    V y" << k << ";\n";
-        o << pre.str() << emit_steps(steps) << "    return m == V(0) || __builtin_isinf(mx);\n}\n";
+        o << pre.str() << emit_steps(steps) << "    return pivot_unusable(m);\n}\n";
     }
     // ---- backward sweep ----
     {
@@ -281,7 +281,7 @@ void gen_cx(std::ostringstream& o, const Topology& t, const std::string& fn, con
 }  // namespace
 
 std::string jit_source(const std::vector<const Topology*>& ts, const std::vector<int>& group_pattern, bool mixed,
-                       int waves_per_simd, bool complex) {
+                       int waves_per_simd, bool complex, bool profile) {
     std::ostringstream o;
     o << "#include \"kernel.h\"\nnamespace s21 {\n";
     // One pattern: the functions inline into the kernel.  Several: each stays a function of its own, so
@@ -316,6 +316,7 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
          "    static constexpr bool lane_per_instance = true;\n"
          "    static constexpr int kind = "
       << (complex ? 2 : 1) << ";\n"
+         "    static constexpr bool profile = false;\n"
          "    template <typename V>\n"
          "    static __device__ bool factor_lane(int g, LDS V* lu, LDS V* c) {\n"
       << dispatch("return ", "lu_gen")
@@ -333,7 +334,8 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
       << ") void s21_newton_jit("
          "const s21::GroupGPU* __restrict__ groups, int n_groups, s21::Args A) {\n"
          "    s21::newton_body<"
-      << (mixed ? "true" : "false") << ", s21::LuGen>(groups, n_groups, A);\n}\n";
+      << (mixed ? "true" : "false") << (profile ? ", s21::Profiled<s21::LuGen>>" : ", s21::LuGen>")
+      << "(groups, n_groups, A);\n}\n";
     return o.str();
 }
 

@@ -431,9 +431,11 @@ __device__ inline void eval_lin(const Ctx& C, int j, LDS const u16* dv, LDS cons
 __device__ inline long long cycles() { return (long long)__builtin_readcyclecounter(); }
 
 // Assembly of A and the RHS at the lane segment's x: colours in order, then the hub gather.  With
-// profiling on, pc[0] gets the device evaluation and pc[1] the hub gather.
-__device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A, long long* pc = nullptr) {
-    const long long t0 = pc ? cycles() : 0;
+// profiling on (prof, uniform), pg[0] gets the device evaluation's cycles and pg[1] the hub gather's; pg
+// is the instance's counters in global memory, given to one lane of the instance (nullptr elsewhere).
+__device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A, bool prof = false,
+                                long long* pg = nullptr) {
+    const long long t0 = prof ? cycles() : 0;
     const TopoGPU& T = C.T;
     const Idx& I = C.I;
     const int ns = T.nnz + T.n;
@@ -460,7 +462,7 @@ __device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A
         }
         wave_sync();
     }
-    const long long t1 = pc ? cycles() : 0;
+    const long long t1 = prof ? cycles() : 0;
     // hub slots: ordered sums of their contributions (long slots in chunks whose partials add in order)
     for (int t = act ? C.sl : T.n_gt; t < T.n_gt; t += C.L) {
         const R4 r = r4(I.gt[t]);
@@ -478,11 +480,13 @@ __device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A
         for (int p = r.y + 1; p < r.z; p++) acc = acc + C.part[p];
         C.a[r.x] = acc;
     }
-    if (pc) {
+    if (prof) {
         wave_sync();
         const long long t2 = cycles();
-        pc[0] += t1 - t0;
-        pc[1] += t2 - t1;
+        if (pg) {
+            pg[0] += t1 - t0;
+            pg[1] += t2 - t1;
+        }
     }
 }
 
@@ -534,15 +538,23 @@ __device__ inline float ldiv<float>(float a, float b) {
 // is final: l_ik = a_ik / u_kk, a_ij -= l_ik u_kj over row k's U entries, and c_i -= c_k l_ik -- for each
 // target in increasing k, the order of the reference's right-looking elimination and column-oriented
 // forward substitution (sparse21/mod.rs:865-919, 946-964).  Returns whether this lane saw a pivot the
-// kept order cannot use: a zero one other than the last (the assertion at sparse21/mod.rs:872), or one so
-// small against its column that a multiplier l_ik = a_ik / u_kk overflows to infinity (in the factors'
-// precision).  The reference's own search exchanges a diagonal below 1e-3 of its column
-// (sparse21/mod.rs:213-216, 757-762); the engine keeps such a pivot, as its Newton loop tests a residual
-// computed from the assembled matrix in FP64 and needs the factors only for the update.
+// kept order cannot use (other than the last): a zero one (the assertion at sparse21/mod.rs:872) or one
+// below the smallest normal number of the factors' precision, whose multipliers l_ik = a_ik / u_kk
+// overflow.  The reference's own search exchanges a diagonal below 1e-3 of its column
+// (sparse21/mod.rs:213-216, 757-762); the engine keeps such a pivot while it is normal, as its Newton
+// loop tests a residual computed from the assembled matrix in FP64 and needs the factors only for the
+// update.
+template <typename V>
+__device__ constexpr V min_normal() {
+    return sizeof(V) == 8 ? V(2.2250738585072014e-308) : V(1.17549435e-38f);
+}
+template <typename V>
+__device__ inline bool pivot_unusable(V u) {  // false for NaN, which the reference's search passes over too
+    return u < min_normal<V>() && u > -min_normal<V>();
+}
 template <typename V>
 __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
     const Idx& I = C.I;
-    bool small_pivot = false;
     for (int l = 0; l < C.T.n_clvl; l++) {
         const int e = I.clvl[l + 1];
         for (int ch = act ? I.clvl[l] + C.sl : e; ch < e; ch += C.L) {
@@ -553,7 +565,6 @@ __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
                 const R4 L = r4(I.fl[j]);
                 const V lv = ldiv<V>(lu[L.x], lu[L.y]);
                 lu[L.x] = lv;
-                small_pivot = small_pivot || __builtin_isinf(lv);
                 const V ck = (V)C.c[L.z];
                 if (ck != V(0)) y = y - ck * lv;  // the sweep skips zero entries (sparse21/mod.rs:953)
                 for (int m = 0; m < L.w; m++, u++) {
@@ -566,9 +577,9 @@ __device__ inline bool factor_forward(const Ctx& C, bool act, LDS V* lu) {
         }
         wave_sync();
     }
-    bool zero_pivot = small_pivot;
+    bool zero_pivot = false;
     for (int i = act ? C.sl : C.T.n; i < C.T.n - 1; i += C.L)
-        if (lu[I.diag[i]] == V(0)) zero_pivot = true;
+        if (pivot_unusable(lu[I.diag[i]])) zero_pivot = true;
     return zero_pivot;
 }
 
@@ -629,6 +640,9 @@ struct LuLevels {
     // Analyses a policy serves: 0 every one; a generated policy 1 (operating points and transients of its
     // real patterns) or 2 (AC launches of its complex patterns).
     static constexpr int kind = 0;
+    // Whether the kernel carries the phase-cycle counters (Profiled<>); without them no profiling code is
+    // compiled in.
+    static constexpr bool profile = false;
     template <typename V>
     static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {
         return factor_forward<V>(C, act, lu);
@@ -639,23 +653,31 @@ struct LuLevels {
     }
 };
 
+// A Newton solve's outcome: its status and the loads (iterations) it ran.  Returned by value, as are the
+// phase cycles (added to the instance's counters in global memory), so that no counter lives in private
+// memory.
+struct SolveOut {
+    int st, loads;
+};
+
 template <bool MIXED, class LU>
-__device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
-                      int it_left, bool prev_ok) {
+__device__ SolveOut newton(const Ctx& C, bool act, bool tran, const Args& A, long long* pg, int it_left,
+                           bool prev_ok) {
     const TopoGPU& T = C.T;
-    const bool prof = A.prof != 0;
+    const bool prof = LU::profile && A.prof != 0;
     long long t0 = 0, t1 = 0;
     int st = act ? RUNNING : S21_OK;
+    int loads = 0;
     for (int it = 0; it < A.max_iter; it++) {
         if (it >= it_left) act = false;  // the segment's share of the iteration limit is used up
         if (!wave_any(act)) break;
-        if (act) iters++;
-        assemble(C, act, tran, A, prof ? pc : nullptr);
+        if (act) loads++;
+        assemble(C, act, tran, A, prof, pg);
         wave_sync();
         if (prof) t0 = cycles();
         const bool bad = seg_any(C, residual(C, act, A.iabstol));
         wave_sync();
-        if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[2] += t1 - t0; t0 = t1; }
         const bool conv = act && prev_ok && !bad;
         if (wave_any(conv)) {
             commit(C, conv);
@@ -669,7 +691,7 @@ __device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long lon
             st = T.sym_status;
             act = false;
         }
-        if (prof) { t1 = cycles(); pc[6] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[6] += t1 - t0; t0 = t1; }
         if (!wave_any(act)) break;
         bool zero;
         if (MIXED) {
@@ -683,12 +705,12 @@ __device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long lon
             st = S21_PIVOT;
             act = false;
         }
-        if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[3] += t1 - t0; t0 = t1; }
         if (MIXED)
             LU::template backward<float>(C, act, C.lu32);
         else
             LU::template backward<double>(C, act, C.a);
-        if (prof) { t1 = cycles(); pc[4] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[4] += t1 - t0; t0 = t1; }
         // clamp and update (analysis.rs:196-207)
         double mx = 0.0;
         for (int k = act ? C.sl : T.n; k < T.n; k += C.L) {
@@ -705,9 +727,9 @@ __device__ int newton(const Ctx& C, bool act, bool tran, const Args& A, long lon
         }
         prev_ok = !seg_any(C, !ok);
         wave_sync();
-        if (prof) { t1 = cycles(); pc[5] += t1 - t0; }
+        if (prof) { t1 = cycles(); if (pg) pg[5] += t1 - t0; }
     }
-    return st == RUNNING ? S21_CONVERGENCE : st;
+    return SolveOut{st == RUNNING ? S21_CONVERGENCE : st, loads};
 }
 
 // ============================================================================================
@@ -1033,13 +1055,14 @@ __device__ inline void backward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS 
 // iteration would assemble the same values and factor them into the same LU: the first iteration does
 // both (the factors into a copy of A, which the residual keeps reading) and later iterations reuse them.
 template <bool MIXED>
-__device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, long long& iters) {
+__device__ SolveOut newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega) {
     const TopoGPU& T = C.T;
     int st = act ? RUNNING : S21_OK;
+    int loads = 0;
     bool prev_ok = true;  // dx starts at zero
     for (int it = 0; it < 20; it++) {
         if (!wave_any(act)) break;
-        if (act) iters++;
+        if (act) loads++;
         if (it == 0) assemble_ac(C, Z_, act, omega);
         wave_sync();
         const bool bad = seg_any(C, residual_ac(C, Z_, act));
@@ -1094,21 +1117,22 @@ __device__ int newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega, 
         prev_ok = !seg_any(C, !ok);
         wave_sync();
     }
-    return st == RUNNING ? S21_CONVERGENCE : st;
+    return SolveOut{st == RUNNING ? S21_CONVERGENCE : st, loads};
 }
 
 // The complex Newton of newton_ac for every instance of the workgroup in lockstep, with a factorisation
 // run by one lane per instance (LU::factor_lane / backward_lane on the interleaved complex arrays; the
 // mixed-precision copies of A and of the residual are complex64).
 template <bool MIXED, class LU>
-__device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omega, long long& iters, LDS int* s_act,
-                            int slot, int n_slots, LDS char* region0, int stride, int gidx) {
+__device__ SolveOut newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omega, LDS int* s_act, int slot,
+                                 int n_slots, LDS char* region0, int stride, int gidx) {
     const TopoGPU& T = C.T;
     int st = act ? RUNNING : S21_OK;
+    int loads = 0;
     bool prev_ok = true;  // dx starts at zero
     for (int it = 0; it < 20; it++) {
         if (!__syncthreads_or(act)) break;
-        if (act) iters++;
+        if (act) loads++;
         if (it == 0) assemble_ac(C, Z_, act, omega);  // the same matrix at every iteration (newton_ac)
         wave_sync();
         const bool bad = seg_any(C, residual_ac(C, Z_, act));
@@ -1178,7 +1202,7 @@ __device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omeg
         prev_ok = !seg_any(C, !ok);
         wave_sync();
     }
-    return st == RUNNING ? S21_CONVERGENCE : st;
+    return SolveOut{st == RUNNING ? S21_CONVERGENCE : st, loads};
 }
 
 // One Newton solve of every instance of the workgroup in lockstep, for an LU policy that factors one
@@ -1186,18 +1210,19 @@ __device__ int newton_ac_wg(const Ctx& C, const AcCtx& Z_, bool act, double omeg
 // barriers the first n_slots threads of the workgroup each factor and sweep one instance's matrix,
 // so one instruction stream serves all of them.  s_act: per-slot flag words in LDS.
 template <bool MIXED, class LU>
-__device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long long& iters, long long* pc,
-                         int it_left, bool prev_ok, LDS int* s_act, int slot, int n_slots, LDS char* region0,
-                         int stride, int gidx) {
+__device__ SolveOut newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long long* pg, int it_left,
+                              bool prev_ok, LDS int* s_act, int slot, int n_slots, LDS char* region0, int stride,
+                              int gidx) {
     const TopoGPU& T = C.T;
-    const bool prof = A.prof != 0;
+    const bool prof = LU::profile && A.prof != 0;
     long long t0 = 0, t1 = 0;
     int st = act ? RUNNING : S21_OK;
+    int loads = 0;
     for (int it = 0; it < A.max_iter; it++) {
         if (it >= it_left) act = false;
         if (!__syncthreads_or(act)) break;
-        if (act) iters++;
-        assemble(C, act, tran, A, prof ? pc : nullptr);
+        if (act) loads++;
+        assemble(C, act, tran, A, prof, pg);
         wave_sync();
         if (prof) t0 = cycles();
         const bool bad = seg_any(C, residual(C, act, A.iabstol));
@@ -1220,7 +1245,7 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
             for (int k = act ? C.sl : T.n; k < T.n; k += C.L) C.lu32[T.nnz + k] = (float)C.c[k];
         }
         if (C.sl == 0) s_act[slot] = act ? 1 : 0;
-        if (prof) { t1 = cycles(); pc[2] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[2] += t1 - t0; t0 = t1; }
         __syncthreads();
         for (int sl = threadIdx.x; sl < n_slots; sl += blockDim.x) {
             if (!s_act[sl]) continue;
@@ -1238,7 +1263,7 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
             s_act[sl] = z ? 2 : 1;
         }
         __syncthreads();
-        if (prof) { t1 = cycles(); pc[3] += t1 - t0; t0 = t1; }
+        if (prof) { t1 = cycles(); if (pg) pg[3] += t1 - t0; t0 = t1; }
         if (act && s_act[slot] == 2) {  // a pivot the reference's search would not take (factor_forward)
             st = S21_PIVOT;
             act = false;
@@ -1263,9 +1288,9 @@ __device__ int newton_wg(const Ctx& C, bool act, bool tran, const Args& A, long 
         }
         prev_ok = !seg_any(C, !ok);
         wave_sync();
-        if (prof) { t1 = cycles(); pc[5] += t1 - t0; }
+        if (prof) { t1 = cycles(); if (pg) pg[5] += t1 - t0; }
     }
-    return st == RUNNING ? S21_CONVERGENCE : st;
+    return SolveOut{st == RUNNING ? S21_CONVERGENCE : st, loads};
 }
 
 // The AC sweep of newton_body: slot's instance (C.inst) for every frequency of run u (first pass), its
@@ -1329,14 +1354,16 @@ __device__ inline void ac_sweep(Ctx& C, const Args& A, int g, int u, int slot, b
     auto point = [&](bool act, long long pt, int& its) {
         const long long i0 = iters;
         const double om = act ? A.omega[pt] : 0.0;
-        int s2;
+        SolveOut r;
         if constexpr (LU::lane_per_instance) {
             __shared__ int s_act[WAVE * MAX_WAVES];
-            s2 = newton_ac_wg<MIXED, LU>(C, Z_, act, om, iters, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
-                                         A.inst_bytes, g);
+            r = newton_ac_wg<MIXED, LU>(C, Z_, act, om, (LDS int*)s_act, slot, A.waves * A.ipw, region0,
+                                        A.inst_bytes, g);
         } else {
-            s2 = newton_ac<MIXED>(C, Z_, act, om, iters);
+            r = newton_ac<MIXED>(C, Z_, act, om);
         }
+        const int s2 = r.st;
+        iters += r.loads;
         its = (int)(iters - i0);
         if (act && C.sl == 0) rec[pt] = (s2 << 8) | its;
         if (act && s2 == S21_OK) store_x(true, wv + (size_t)pt * 2 * T.n);
@@ -1509,8 +1536,12 @@ __device__ inline void op_tran(Ctx& C, const Args& A, int g, int slot, bool vali
         return;
     }
 
-    long long pc[NPROF] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const long long tstart = A.prof ? cycles() : 0;
+    // phase cycle counters of this instance (profiling mode): one lane adds to them in global memory
+    long long* const pg =
+        (LU::profile && A.prof && G.prof && valid && C.sl == 0) ? G.prof + (size_t)inst * NPROF : nullptr;
+    if (pg)
+        for (int k = 0; k < NPROF; k++) pg[k] = 0;
+    const long long tstart = LU::profile && A.prof ? cycles() : 0;
     long long iters = 0;
     int pt0 = 0, used0 = 0;  // a resumed instance: the solve it continues and the iterations that solve used
     if (A.resume && valid) {
@@ -1521,13 +1552,16 @@ __device__ inline void op_tran(Ctx& C, const Args& A, int g, int slot, bool vali
     int last_pt = 0;
     long long last_start = 0;  // iteration count when the last solve began
     auto solve = [&](bool act, bool tran, int left, bool prev_ok) {
+        SolveOut r;
         if constexpr (LU::lane_per_instance) {
             __shared__ int s_act[WAVE * MAX_WAVES];
-            return newton_wg<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok, (LDS int*)s_act, slot,
-                                        A.waves * A.ipw, region0, A.inst_bytes, g);
+            r = newton_wg<MIXED, LU>(C, act, tran, A, pg, left, prev_ok, (LDS int*)s_act, slot, A.waves * A.ipw,
+                                     region0, A.inst_bytes, g);
         } else {
-            return newton<MIXED, LU>(C, act, tran, A, iters, pc, left, prev_ok);
+            r = newton<MIXED, LU>(C, act, tran, A, pg, left, prev_ok);
         }
+        iters += r.loads;
+        return r.st;
     };
     last_start = iters - used0;
     const bool op_here = pt0 == 0;  // a resumed instance may start inside the time loop
@@ -1563,10 +1597,7 @@ __device__ inline void op_tran(Ctx& C, const Args& A, int g, int slot, bool vali
             G.resume_at[2 * inst] = last_pt;
             G.resume_at[2 * inst + 1] = (int)(iters - last_start);
         }
-        if (A.prof && G.prof) {
-            pc[7] = cycles() - tstart;
-            for (int k = 0; k < NPROF; k++) G.prof[(size_t)inst * NPROF + k] = pc[k];
-        }
+        if (pg) pg[7] = cycles() - tstart;
     }
 }
 
@@ -1639,11 +1670,22 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     }
 }
 
+// An LU policy with the phase-cycle counters compiled in (profiling launches).
+template <class LU>
+struct Profiled : LU {
+    static constexpr bool profile = true;
+};
+
 // Precompiled kernel: the level-scheduled factorisation of the pattern's records.
 template <bool MIXED>
 __global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel(const GroupGPU* __restrict__ groups, int n_groups,
                                                                   Args A) {
     newton_body<MIXED, LuLevels>(groups, n_groups, A);
+}
+template <bool MIXED>
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel_prof(const GroupGPU* __restrict__ groups,
+                                                                       int n_groups, Args A) {
+    newton_body<MIXED, Profiled<LuLevels>>(groups, n_groups, A);
 }
 
 }  // namespace s21

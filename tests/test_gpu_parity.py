@@ -789,3 +789,21 @@ def test_closed_form_charge_circuits_match_oracle():
     err = float((np.abs(W_ - X) / np.maximum(np.abs(X), 1e-12)).max())
     log("  max relative |dx|", err)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("jit", [-1, 1])
+def test_profiling_kernels_count_phases_and_leave_results_unchanged(jit):
+    """Profiling launches run kernels with the phase-cycle counters compiled in (the normal kernels carry
+    none): the same solutions and iteration counts, bit for bit, and per instance non-negative phase
+    cycles whose sum stays within the instance's total (the eighth counter)."""
+    nl, over = W.chain_mc(n_inst=64)
+    cn = nl.to_c()
+    for precision in (FP64, MIXED):
+        ref = Batch(Group(cn, 64, over), precision=precision).set_jit(jit).op()
+        b = Batch(Group(cn, 64, over), precision=precision).set_jit(jit).set_profiling(True).op()
+        assert np.array_equal(ref.x(), b.x()) and np.array_equal(ref.status()[1], b.status()[1])
+        pr = b.profile()
+        log("profile", precision, jit, pr[0])
+        assert np.all(pr >= 0) and np.all(pr[:, 7] > 0)
+        assert np.all(pr[:, :7].sum(axis=1) <= pr[:, 7])
+        assert np.all(pr[:, 0] > 0) and np.all(pr[:, 3] > 0)  # device evaluation, factorisation

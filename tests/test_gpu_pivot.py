@@ -1,9 +1,9 @@
 """Pivot recovery on the GPU.  The engine keeps the first factorisation's pivot order while every pivot of
-it is non-zero (sparse21/mod.rs:872) and its multipliers stay finite; an instance whose order fails that
+it is a normal number (not zero, sparse21/mod.rs:872, nor subnormal); an instance whose order fails that
 resumes the interrupted Newton iteration -- of the operating point, or of a time point in the middle of a
 transient -- on the order the reference's search picks there.  The switching diode (tests/circuits.py)
 turns off as its node charges: its conductance, a pivot of the first order, underflows to zero part-way
-through each analysis (without the resumption the multiplier 1/g overflows and the update turns NaN)."""
+through each analysis (kept, the pivot's multiplier 1/g overflows first and the update turns NaN)."""
 import numpy as np
 import pytest
 
