@@ -349,7 +349,8 @@ def main():
         res[name] = dict(batch=b, elapsed=elapsed, kernel_ms=kms, status=st, iters=it, x=b.x(),
                          structure=b.structure(), launch={"instances_per_wavefront": ipw, "wavefronts_per_workgroup": waves,
                                                           "generated_factorisation": jit,
-                                                          "autotune_ms": {str(k): v for k, v in tune.items()}})
+                                                          "tuned": getattr(b, "tuned", None),
+                                                          "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()}})
         print(f"[bench] {name}: {args.steps} steps in {elapsed:.4f} s, kernel {np.mean(kms):.3f} ms/launch, "
               f"iterations/instance {it.min()}..{it.max()}, failed {int(np.sum(st != 0))}, launch {res[name]['launch']}",
               file=sys.stderr, flush=True)

@@ -255,6 +255,16 @@ S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_
  * wavefronts per workgroup, generated factorisation used. */
 S21_API int s21_batch_set_ipw(s21_batch* b, int32_t ipw);
 S21_API int s21_batch_set_jit(s21_batch* b, int32_t mode);
+/* Launch-shape and structure knobs the autotuner searches (no counterpart in the reference, whose
+ * solver is serial): wavefronts per workgroup, at most `waves` (1..8; 0 = the most that still leave a
+ * workgroup for every CU), and the hub policy of the stamp scatter (1 = every slot that several
+ * components stamp is gathered in order after the device rounds -- one colour round per device class,
+ * for latency-bound launches of few instances; 2 = only slots of many contributors, colour rounds for the
+ * rest; 0 = chosen by batch size).  The wavefronts change only time; the hub policy also the order in
+ * which a shared slot's contributions are added (1 follows the reference's component order exactly; 2
+ * adds those of few contributors in colour order, which agrees to rounding). */
+S21_API int s21_batch_set_waves(s21_batch* b, int32_t waves);
+S21_API int s21_batch_set_hub(s21_batch* b, int32_t policy);
 S21_API int s21_batch_config(const s21_batch* b, int32_t* ipw, int32_t* waves, int32_t* jit);
 /* AC sweep shape (no counterpart in the reference, which sweeps one instance's frequencies in sequence,
  * each solve starting from the previous frequency's solution, analysis.rs:797-818): each instance's

@@ -63,6 +63,8 @@ def lib():
     _sig(L, "s21_batch_get_wave", C.c_int, vp, i32, i32, P(d))
     _sig(L, "s21_batch_timing", C.c_int, vp, P(d), P(i64), P(d))
     _sig(L, "s21_batch_resumptions", C.c_int, vp, P(i64))
+    _sig(L, "s21_batch_set_waves", C.c_int, vp, i32)
+    _sig(L, "s21_batch_set_hub", C.c_int, vp, i32)
     _sig(L, "s21_batch_pivots", C.c_int, vp, i32, P(i32), P(i32))
     _sig(L, "s21_batch_set_profiling", C.c_int, vp, i32)
     _sig(L, "s21_batch_set_ipw", C.c_int, vp, i32)

@@ -204,22 +204,58 @@ class Batch:
         """Runs of frequencies per instance of the last AC sweep."""
         return int(lib().s21_batch_last_ac_runs(self.h))
 
-    def autotune(self, run, candidates=(1, 2, 4), repeats=2):
-        """Times ``run(self)`` (an analysis call) at each instances-per-wavefront candidate and keeps the
-        fastest; returns {ipw: kernel milliseconds}.  Each candidate runs once untimed first, which also
-        builds its kernel."""
-        times = {}
-        for ipw in candidates:
-            self.set_ipw(ipw)
+    def set_waves(self, waves):
+        """Wavefronts per workgroup, at most this many (1..8), or 0 to let each launch choose."""
+        check(lib().s21_batch_set_waves(self.h, int(waves)))
+        return self
+
+    def set_hub(self, policy):
+        """Hub policy of the stamp scatter: 1 every shared slot, 2 only slots of many contributors, 0 auto."""
+        check(lib().s21_batch_set_hub(self.h, int(policy)))
+        return self
+
+    def _time(self, run, repeats):
+        run(self)  # untimed: builds the structure and, for a generated factorisation, its kernel
+        ms = []
+        for _ in range(repeats):
             run(self)
-            ms = []
-            for _ in range(repeats):
-                run(self)
-                ms.append(self.timing()[0])
-            times[ipw] = min(ms)
-        best = min(times, key=times.get)
-        self.set_ipw(best)
+            ms.append(self.timing()[0])
+        return min(ms)
+
+    def autotune(self, run, candidates=(1, 2, 4, 8, 16), waves=(0, 2, 4), jit=(0, -1), hub=(0, 1, 2), repeats=2):
+        """Times ``run(self)`` (an analysis call) over the launch shape and structure and keeps the fastest:
+        instances per wavefront (all candidates, at the automatic waves), then, at the best of those,
+        wavefronts per workgroup, the factorisation (0: generated where the batch qualifies, -1:
+        record-driven) and the hub policy, each in turn at the best so far.  Results are bit-identical
+        across the shapes and factorisations; the hub policy changes the order shared slots are summed in
+        (agreeing to rounding).  Returns {"ipw=..,waves=..,jit=..,hub=..": kernel ms}
+        for every configuration timed; ``self`` is left at the fastest."""
+        best = {"ipw": 0, "waves": 0, "jit": 0, "hub": 0}
+        setters = {"ipw": self.set_ipw, "waves": self.set_waves, "jit": self.set_jit, "hub": self.set_hub}
+        times = {}
+
+        def key(c):
+            return ",".join(f"{k}={c[k]}" for k in ("ipw", "waves", "jit", "hub"))
+
+        def apply(c):
+            for k, v in c.items():
+                setters[k](v)
+
+        for dim, values in (("ipw", candidates), ("waves", waves), ("jit", jit), ("hub", hub)):
+            for v in values:
+                c = dict(best, **{dim: v})
+                if key(c) in times:
+                    continue
+                apply(c)
+                try:
+                    times[key(c)] = self._time(run, repeats)
+                except Exception:  # a shape the circuit does not fit (LDS) is skipped
+                    continue
+            best = min(({k: int(x.split("=")[1]) for k, x in zip(("ipw", "waves", "jit", "hub"), t.split(","))}
+                        for t in times), key=lambda c: times[key(c)])
+        apply(best)
         run(self)
+        self.tuned = best
         return times
 
     def config(self):

@@ -102,6 +102,8 @@ struct s21_batch {
     int last_waves = 1, last_ipw = 1;
     int ipw = 0;  // instances per wavefront; 0 = chosen per launch
     int jit = 0;  // generated factorisation: 0 = for large single-pattern batches, 1 = always, -1 = never
+    int waves = 0;  // wavefronts per workgroup, at most this many (0 = chosen per launch)
+    int hub = 0;    // hub policy: 0 = chosen per batch, 1 = every shared slot, 2 = slots of many contributors
     int ac_runs = 0;  // AC: runs of frequencies per instance; 0 = chosen per sweep
     int last_ac_runs = 0;
     bool last_jit = false;
@@ -383,7 +385,8 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const bool latency = total <= LATENCY_INST || (b->precision == S21_MIXED &&
                                                    total <= (int64_t)LATENCY_INST_PER_SIMD * 4 * std::max(cus, 1));
-    host_prepare(b->opts, g, tr, hp, ac, latency && !ac ? 1 : HUB_CONTRIB);
+    const bool every_shared = b->hub == 1 || (b->hub == 0 && latency);
+    host_prepare(b->opts, g, tr, hp, ac, every_shared && !ac ? 1 : HUB_CONTRIB);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
 
@@ -650,9 +653,10 @@ static void pick_config(const s21_batch* b, size_t blob, size_t raw, int total_i
         long largest = 0;  // launch units of the largest group: no wavefront beyond them has work
         for (const GroupHost& g : b->groups) largest = std::max(largest, (long)g.n_inst * ac_units(A));
         int best = 0;
-        for (int k = 1; k <= MAX_WAVES; k++) {
+        const int kmax = b->waves > 0 ? std::min(b->waves, MAX_WAVES) : MAX_WAVES;
+        for (int k = 1; k <= kmax; k++) {
             if (blob + (size_t)k * ipw * stride > max_block) break;
-            if (k > 1 && (workgroups(k) < target || (long)(k - 1) * ipw >= largest)) break;
+            if (k > 1 && b->waves <= 0 && (workgroups(k) < target || (long)(k - 1) * ipw >= largest)) break;
             best = k;
         }
         if (best > 0) {
@@ -1692,6 +1696,22 @@ int s21_batch_set_jit(s21_batch* b, int32_t mode) {
         if (mode != b->jit)
             for (GroupHost& g : b->groups) g.prepared.clear();
         b->jit = mode;
+    });
+}
+int s21_batch_set_waves(s21_batch* b, int32_t waves) {
+    return guard([&] {
+        if (!b) fail(S21_INVALID, "null batch");
+        if (waves < 0 || waves > MAX_WAVES) fail(S21_INVALID, "wavefronts per workgroup must be 0 (auto) to 8");
+        b->waves = waves;
+    });
+}
+int s21_batch_set_hub(s21_batch* b, int32_t policy) {
+    return guard([&] {
+        if (!b) fail(S21_INVALID, "null batch");
+        if (policy < 0 || policy > 2) fail(S21_INVALID, "hub policy must be 0 (auto), 1 or 2");
+        if (policy != b->hub)
+            for (GroupHost& g : b->groups) g.prepared.clear();
+        b->hub = policy;
     });
 }
 int s21_batch_set_ac_runs(s21_batch* b, int32_t runs) {

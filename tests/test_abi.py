@@ -47,3 +47,9 @@ def test_ac_point_count(f0, f1, n):
     nl.v("v", 0.0, "a", "", acm=1.0)
     f, _, _ = O.ac(nl.to_c(), f0, f1, n)
     assert lib().s21_ac_point_count(C.byref(A.s21_ac_options(f0, f1, n))) == len(f)
+
+
+def test_tuning_knobs_reject_bad_arguments():
+    from spice21_amd._lib import lib
+    assert lib().s21_batch_set_waves(None, 2) == A.S21_INVALID
+    assert lib().s21_batch_set_hub(None, 1) == A.S21_INVALID

@@ -807,3 +807,39 @@ def test_profiling_kernels_count_phases_and_leave_results_unchanged(jit):
         assert np.all(pr >= 0) and np.all(pr[:, 7] > 0)
         assert np.all(pr[:, :7].sum(axis=1) <= pr[:, 7])
         assert np.all(pr[:, 0] > 0) and np.all(pr[:, 3] > 0)  # device evaluation, factorisation
+
+
+def test_tuning_knobs_keep_results_bit_identical():
+    """Wavefronts per workgroup and instances per wavefront change only the time: at one hub policy and
+    factorisation kind the 64-instance chain's operating points are bit-identical across them.  The hub
+    policy changes the order in which a shared slot's contributions are added (policy 1 gathers every
+    shared slot in component order, the reference's order; policy 2 adds those of few contributors in
+    colour order) and the generated factorisation rounds like the record-driven one only to 1e-12
+    (test_generated_factorisation_matches_level_schedule), so across those the solutions agree to
+    rounding.  The autotuner leaves the batch at one of the configurations it timed."""
+    nl, over = W.chain_mc(n_inst=64)
+    cn = nl.to_c()
+    v = np.asarray(nl.kinds) == 0
+    for precision in (FP64, MIXED):
+        refs = {}
+        for ipw, waves, hub, jit in [(1, 1, 1, -1), (4, 2, 1, -1), (2, 4, 1, 1), (16, 1, 1, 1), (4, 2, 2, -1),
+                                     (16, 1, 2, -1), (8, 8, 2, 1), (2, 1, 2, 1)]:
+            b = Batch(Group(cn, 64, over), precision=precision)
+            b.set_ipw(ipw).set_waves(waves).set_hub(hub).set_jit(jit).op()
+            x, (st, it) = b.x(), b.status()
+            assert np.all(st == 0)
+            log("knobs", precision, (ipw, waves, hub, jit), "config", b.config())
+            if (hub, jit) not in refs:
+                refs[(hub, jit)] = (x, it)
+            else:
+                k = refs[(hub, jit)]
+                assert np.array_equal(x, k[0]) and np.array_equal(it, k[1]), (ipw, waves, hub, jit)
+        for k, (x, _) in refs.items():
+            d = np.abs(x - refs[(1, -1)][0])[:, v].max()
+            log("hub, jit", k, "differs from (1, -1) by", d)
+            assert d <= (1e-12 if precision == FP64 else 1e-6)
+        b = Batch(Group(cn, 64, over), precision=precision)
+        times = b.autotune(lambda bb: bb.op(), repeats=1)
+        log("autotune", precision, b.tuned, times)
+        assert ",".join(f"{k}={b.tuned[k]}" for k in ("ipw", "waves", "jit", "hub")) in times
+        assert np.abs(b.x() - refs[(1, -1)][0])[:, v].max() <= (1e-12 if precision == FP64 else 1e-6)
