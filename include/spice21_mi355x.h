@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define S21_ABI_VERSION 1
+#define S21_ABI_VERSION 2
 #define S21_GND (-1)
 #define S21_API __attribute__((visibility("default")))
 
@@ -52,7 +52,9 @@ enum {                                                      /* comps/mod.rs:60-7
     S21_COMP_V = 3,      /* Vsrc, value = dc, acm = AC magnitude      comps/mod.rs:95-150  */
     S21_COMP_MOS1 = 4,   /* Mos level 1                               comps/mos.rs:625-969 */
     S21_COMP_MOS0 = 5,   /* Mos level 0 (model = 0 NMOS / 1 PMOS)     comps/mos.rs:1007-1100 */
-    S21_COMP_DIODE = 6   /* Diode                                     comps/diode.rs:215-355 */
+    S21_COMP_DIODE = 6,  /* Diode                                     comps/diode.rs:215-355 */
+    S21_COMP_BSIM4 = 7   /* BSIM4 (comps/bsim4/bsim4solver.rs:3757-3769), single-instance
+                            analyses only (the large-instance path and the byte ABI)      */
 };
 
 /* Optional-double parameter set keyed by proto field number (protos/mos.proto:22-76,
@@ -69,9 +71,11 @@ typedef struct {
     int32_t type;     /* S21_COMP_* */
     int32_t node[6];  /* variable indices, S21_GND for ground:
                          R, C, I: {p, n}; V: {p, n, ivar}; MOS1: {d, g, s, b, dp, sp};
-                         MOS0: {d, g, s, b}; DIODE: {p, n, r}                          */
-    int32_t model;    /* MOS1: index into mos1_models; DIODE: into diode_models; MOS0: 0|1 */
-    int32_t inst;     /* MOS1: index into mos1_insts; DIODE: into diode_insts             */
+                         MOS0: {d, g, s, b}; DIODE: {p, n, r}; BSIM4: {d, g, s, b}     */
+    int32_t model;    /* MOS1: index into mos1_models; DIODE: into diode_models; MOS0: 0|1;
+                         BSIM4: into bsim4_models                                          */
+    int32_t inst;     /* MOS1: index into mos1_insts; DIODE: into diode_insts;
+                         BSIM4: into bsim4_insts                                           */
     double value;     /* R: g; C: c; I: dc; V: dc                                         */
     double acm;       /* V: AC magnitude                                                  */
 } s21_comp;
@@ -86,6 +90,12 @@ typedef struct {
     int32_t n_mos1_insts;   const s21_params* mos1_insts;
     int32_t n_diode_models; const s21_params* diode_models;
     int32_t n_diode_insts;  const s21_params* diode_insts;
+    /* Bsim4Model (protos/bsim4.proto:44-46: slot 1 = mos_type, 1 PMOS) and Bsim4InstParams keyed by
+     * field number (protos/bsim4.proto:6-41).  Only the polarity reaches the model
+     * (Bsim4ModelSpecs::from, comps/bsim4/model/mod.rs:21-30); every mode selector is at its default,
+     * so a BSIM4 device has no internal nodes (bsim4ports.rs:31-112). */
+    int32_t n_bsim4_models; const s21_params* bsim4_models;
+    int32_t n_bsim4_insts;  const s21_params* bsim4_insts;
 } s21_netlist;
 
 /* Simulation options, defaults as analysis.rs:665-693:
@@ -296,7 +306,8 @@ S21_API int s21_batch_assemble(s21_batch* b, int32_t group, int32_t inst, const 
  * lives in HBM and is factored by the multifrontal method over a fill-reducing order fixed once per
  * analysis (static pivot rows from a maximum transversal, approximate minimum degree) in place of the
  * reference's per-iteration Markowitz search (sparse21/mod.rs:647-991, which grows about as n^2.5).
- * Components: R, C, V, I, Mos1, Diode. */
+ * Components: R, C, V, I, Mos1, Diode, BSIM4 (one thread per BSIM4 device evaluates Bsim4::op, tran_op and
+ * stamp, bsim4solver.rs:145-3754, tran.rs:11-550, stamp.rs:8-554). */
 typedef struct s21_large s21_large;
 S21_API int s21_large_create(const s21_netlist* net, const s21_options* opts, int32_t precision, s21_large** out);
 S21_API void s21_large_free(s21_large* l);
@@ -335,8 +346,10 @@ S21_API int s21_large_structure(const s21_large* l, int64_t* nnz_a, int64_t* nnz
  * Newton iteration; 1 lets an iteration keep the factors of an earlier Jacobian (of an earlier iteration or
  * time point) while the FP64 residual still falls at least fourfold per iteration (at most 32 iterations
  * in a row), so that Newton converges
- * to the same solution under the same test as iterative refinement.  Default: 1 in S21_MIXED, 0 in
- * S21_FP64. */
+ * to the same solution under the same test as iterative refinement; a reused solve whose update is not
+ * finite is discarded and the iteration repeated with fresh factors.  Default: 1 in S21_MIXED, 0 in
+ * S21_FP64.  Not available with BSIM4 devices (default 0, and 1 is rejected): with reused factors a 51-stage
+ * BSIM4 ring met non-finite updates, and a three-stage circuit converged to a different trajectory. */
 S21_API int s21_large_set_reuse(s21_large* l, int32_t on);
 /* Wall time of the last analysis (milliseconds) and factorisations it ran. */
 S21_API int s21_large_timing(const s21_large* l, double* total_ms, int64_t* factorisations);

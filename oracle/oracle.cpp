@@ -13,12 +13,20 @@
  *   spice21/src/comps/mod.rs     -- Vsrc, Capacitor, Resistor, Isrc, constants
  *   spice21/src/comps/mos.rs     -- Mos1 resolve/derive/op_stamp/load_ac, Mos0
  *   spice21/src/comps/diode.rs   -- Diode derive/limit/load
+ *   spice21/src/comps/bsim4/     -- BSIM4: the Component (create_matps, load, commit,
+ *                                   bsim4solver.rs:28-145, 3757-3769) is the oracle's own; its device
+ *                                   functions (parameter derivation, Bsim4::op, tran_op, stamp) are the
+ *                                   product's host-and-device source (spice21_amd/csrc/bsim4*.h,
+ *                                   bsim4_derive.cpp), compiled here by g++ for the CPU.  They are not an
+ *                                   independent restatement: the reference's three BSIM4 ring-oscillator
+ *                                   waveforms and its dcop known answers pin them, through this oracle.
  * Complex arithmetic follows num-complex 0.3 (spice21/Cargo.toml:23), which the reference uses
  * for AC: naive division by |z|^2, norm = hypot(re, im).
  *
  * Input is the flattened netlist of include/spice21_mi355x.h (elaboration is the product's).
  */
 #include "../include/spice21_mi355x.h"
+#include "../spice21_amd/csrc/bsim4_eval.h"
 
 #include <algorithm>
 #include <atomic>
@@ -647,6 +655,8 @@ struct AnalysisInfo {
 //   3  Diode: depletion charge without the transit-time factor (diode.rs:307-309 multiplies by tt)
 //   4  Meyer gate-bulk capacitance below threshold over phi instead of 2 phi (mos.rs:727-735)
 //   5  Meyer saturation gate-source capacitance cox/2 instead of cox/3 (mos.rs:740-743)
+//   6  BSIM4 commit skipped: charges integrate against the device's initial point (bsim4solver.rs:3765-3767)
+//   7  BSIM4 bulk-row right-hand-side stamp dropped (stamp.rs: the bNodePrime RHS entry)
 static int g_plant = 0;
 
 static inline ChargeInteg integq(const TranState& st, double dq, double dq_dv, double vguess) {
@@ -1422,6 +1432,49 @@ struct Diode : Component {
 /* ------------------------------------------------------------------------------------------
  * Solver (analysis.rs:136-346)
  * ------------------------------------------------------------------------------------------ */
+/* BSIM4 at the default mode selectors: no internal nodes, so dNodePrime = dNode, gNodePrime = gNodeExt,
+ * sNodePrime = sNode, bNodePrime = dbNode = sbNode = bNode and no charge node (bsim4ports.rs:31-112). */
+struct Bsim4 : Component {
+    s21::b4::B4Par P;
+    VarIdx t[4];  // d, g, s, b
+    s21::b4::B4Op op{}, guess{};
+    Eidx m[4][4];
+    Bsim4(const s21::b4::B4Par& par, const VarIdx nd[4]) : P(par) {
+        for (int k = 0; k < 4; k++) t[k] = nd[k];
+        for (auto& r : m)
+            for (auto& c : r) c = NONE;
+    }
+    void create_matrix_elems(Maker& mk) override {  // create_matps, bsim4solver.rs:28-116 (Q entries: no node)
+        enum { D = 0, G = 1, S = 2, B = 3 };
+        static const int order[22][2] = {{D, B}, {G, B}, {S, B}, {B, D}, {B, G}, {B, S}, {B, B}, {D, D},
+                                         {G, G}, {S, S}, {D, D}, {S, S}, {D, D}, {G, D}, {G, S}, {S, S},
+                                         {D, S}, {D, D}, {D, G}, {S, G}, {S, S}, {S, D}};
+        for (auto& q : order) m[q[0]][q[1]] = make_matrix_elem(mk, t[q[0]], t[q[1]]);
+    }
+    void commit() override {
+        if (g_plant != 6) op = guess;
+    }
+    Stamps<double> load(const std::vector<double>& x, const AnalysisInfo& an, const Options& opts) override {
+        // load_dc_tr, bsim4solver.rs:134-144: op() from the port voltages, tran_op in a transient, stamp()
+        s21::b4::B4Op N;
+        s21::b4::op_eval(P, guess, var_get(x, t[0]), var_get(x, t[1]), var_get(x, t[2]), var_get(x, t[3]), opts.gmin,
+                         N);
+        s21::b4::B4Tran T{};
+        if (an.kind == AN_TRAN) s21::b4::tran_eval(P, op, N, an.tran->dt, T);
+        guess = N;
+        Stamps<double> st;
+        s21::b4::stamp(P, N, T, [&](int k, double v) {
+            const VarIdx r = t[s21::b4::b4_row(k)];
+            const int c = s21::b4::b4_col(k);
+            if (c < 0)
+                st.b.push_back({g_plant == 7 && s21::b4::b4_row(k) == 3 ? -1 : r, v});
+            else
+                st.g.push_back({r < 0 || t[c] < 0 ? NONE : m[s21::b4::b4_row(k)][c], v});
+        });
+        return st;
+    }
+};
+
 struct Circuit {
     std::vector<std::unique_ptr<Component>> comps;
     std::vector<int> kinds;
@@ -1460,6 +1513,19 @@ static void build_components(const s21_netlist* net, const Options& opts, Circui
             DiodeModel m = diode_model_from(net->diode_models[c.model]);
             DiodeIntp ip = diode_derive(m, net->diode_insts[c.inst], opts);
             ck.comps.emplace_back(new Diode(m, ip, nd[0], nd[1], nd[2]));
+            break;
+        }
+        case S21_COMP_BSIM4: {
+            if (c.model < 0 || c.model >= net->n_bsim4_models || c.inst < 0 || c.inst >= net->n_bsim4_insts)
+                sperror("Parameters not defined");
+            s21::b4::B4Par P;
+            try {
+                P = s21::b4::derive_params(net->bsim4_models[c.model], net->bsim4_insts[c.inst]);
+            } catch (const std::domain_error& e) {
+                sperror(e.what());
+            }
+            VarIdx pr[4] = {nd[0], nd[1], nd[2], nd[3]};
+            ck.comps.emplace_back(new Bsim4(P, pr));
             break;
         }
         default: sperror("Invalid Component");

@@ -23,7 +23,7 @@ STATUS_NAMES = {
 
 # variable and component kinds (analysis.rs:33-38, comps/mod.rs:60-72)
 S21_VAR_V, S21_VAR_I = 0, 1
-S21_COMP_R, S21_COMP_C, S21_COMP_I, S21_COMP_V, S21_COMP_MOS1, S21_COMP_MOS0, S21_COMP_DIODE = range(7)
+S21_COMP_R, S21_COMP_C, S21_COMP_I, S21_COMP_V, S21_COMP_MOS1, S21_COMP_MOS0, S21_COMP_DIODE, S21_COMP_BSIM4 = range(8)
 S21_OVR_VALUE, S21_OVR_MODEL, S21_OVR_INST = 0, 1, 2
 S21_FP64, S21_MIXED = 0, 1
 
@@ -38,6 +38,12 @@ DIODE_MODEL_FIELDS = dict(
     tnom=2, is_=3, n=4, tt=5, vj=6, m=7, eg=8, xti=9, kf=10, af=11, fc=12, bv=13, ibv=14, rs=15, cj0=16,
 )
 DIODE_INST_FIELDS = dict(area=4, temp=5)
+# protos/bsim4.proto:44-46 (Bsim4Model) and :6-41 (Bsim4InstParams)
+BSIM4_MODEL_FIELDS = dict(mos_type=1)
+BSIM4_INST_FIELDS = dict(
+    l=1, w=2, nf=3, sa=4, sb=5, sd=6, sca=7, scb=8, scc=9, sc=10, ad=11, as_=12, pd=13, ps=14, nrd=15, nrs=16,
+    min=17, rgeomod=18, rbdb=19, rbsb=20, rbpb=21, rbps=22, rbpd=23, delvto=24, xgw=25, ngcon=26, trnqsmod=27,
+    acnqsmod=28, rbodymod=29, rgatemod=30, geomod=31)
 
 
 def _field(table, name):
@@ -88,6 +94,10 @@ class s21_netlist(C.Structure):
         ("diode_models", C.POINTER(s21_params)),
         ("n_diode_insts", C.c_int32),
         ("diode_insts", C.POINTER(s21_params)),
+        ("n_bsim4_models", C.c_int32),
+        ("bsim4_models", C.POINTER(s21_params)),
+        ("n_bsim4_insts", C.c_int32),
+        ("bsim4_insts", C.POINTER(s21_params)),
     ]
 
 

@@ -166,7 +166,7 @@ struct s21_ckt {
     std::vector<s21::CompSpec> comps;
     std::map<std::string, s21_params> mos1_models, mos1_insts, diode_models, diode_insts;
     std::map<std::string, int> mos0;
-    std::map<std::string, int> bsim4_models;
+    std::map<std::string, s21_params> bsim4_models, bsim4_insts;
     std::map<std::string, s21::ModuleSpec> modules;
 };
 
@@ -184,6 +184,43 @@ static s21_params params_msg(Reader r, int int64_field) {
         } else if (f == 2 && wt == 0) {  // Mos1Model.mos_type (enum)
             p.v[2] = (double)r.varint();
             p.set |= 1ull << 2;
+        } else {
+            r.skip(wt);
+        }
+    }
+    return p;
+}
+// Bsim4Model (protos/bsim4.proto:44-46, name 900): only the polarity reaches the model
+// (Bsim4ModelSpecs::from, comps/bsim4/model/mod.rs:21-30; mos_type 1 is PMOS, anything else NMOS).
+static s21_params bsim4_model_msg(Reader r, std::string& name) {
+    s21_params p;
+    memset(&p, 0, sizeof p);
+    int f, wt;
+    while (r.next(f, wt)) {
+        if (f == 1 && wt == 0) {
+            p.v[1] = r.varint() == 1 ? 1.0 : 0.0;
+            p.set |= 1ull << 1;
+        } else if (f == 900 && wt == 2) {
+            name = r.str();
+        } else {
+            r.skip(wt);
+        }
+    }
+    return p;
+}
+// Bsim4InstParams (protos/bsim4.proto:6-42): fields 1-31 are DoubleValue / UInt64Value wrappers, name 40.
+static s21_params bsim4_inst_msg(Reader r, std::string& name) {
+    s21_params p;
+    memset(&p, 0, sizeof p);
+    int f, wt;
+    while (r.next(f, wt)) {
+        if (f >= 1 && f <= 31 && wt == 2) {
+            Reader sub = r.sub();
+            const bool integer = f == 17 || f == 18 || f >= 27;
+            p.v[f] = integer ? (double)wrapped_int64(sub) : wrapped_double(sub);
+            p.set |= 1ull << f;
+        } else if (f == 40 && wt == 2) {
+            name = r.str();
         } else {
             r.skip(wt);
         }
@@ -375,8 +412,18 @@ static void decode_circuit(Reader r, s21_ckt& ck) {
                 }
                 case 2: ck.diode_models[name_of(m)] = params_msg(m, -1); break;
                 case 3: ck.diode_insts[name_of(m)] = params_msg(m, -1); break;
-                case 4: ck.bsim4_models[name_of(m)] = 1; break;
-                case 5: break;
+                case 4: {  // Defs::add_model keyed by name (circuit.rs:297-301)
+                    std::string nm;
+                    s21_params p = bsim4_model_msg(m, nm);
+                    ck.bsim4_models[nm] = p;
+                    break;
+                }
+                case 5: {  // Bsim4Cache::add_inst keyed by name (cache.rs:55-57)
+                    std::string nm;
+                    s21_params p = bsim4_inst_msg(m, nm);
+                    ck.bsim4_insts[nm] = p;
+                    break;
+                }
                 case 6: ck.mos1_models[name_of(m)] = params_msg(m, 28); break;
                 case 7: ck.mos1_insts[name_of(m)] = params_msg(m, -1); break;
                 default: any = false; break;
@@ -402,7 +449,7 @@ struct s21_elab {
     std::vector<int32_t> kinds;
     std::vector<const char*> name_ptrs;
     std::vector<s21_comp> comps;
-    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts;
+    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts, bsim4_models, bsim4_insts;
     std::vector<std::string> comp_names;
 };
 
@@ -413,7 +460,7 @@ struct Elaborator {
     s21_elab& e;
     std::vector<std::string> path;
     std::unordered_map<std::string, int> first;  // name -> first variable with that name
-    std::map<std::string, int> m1_model_idx, m1_inst_idx, d_model_idx, d_inst_idx;
+    std::map<std::string, int> m1_model_idx, m1_inst_idx, d_model_idx, d_inst_idx, b4_model_idx, b4_inst_idx;
     using Ns = std::map<std::string, int>;  // namespace: node name -> variable (S21_GND for ground)
 
     Elaborator(const s21_ckt& c, s21_elab& out) : ck(c), e(out) {}
@@ -516,8 +563,13 @@ struct Elaborator {
             int nd[6] = {node_var(c.d, on_top(), ns), node_var(c.g, on_top(), ns), node_var(c.s, on_top(), ns),
                          node_var(c.b, on_top(), ns), 0, 0};
             path.push_back(c.name);
-            if (ck.bsim4_models.count(c.model)) fail(S21_ERR, "BSIM4 devices are not supported by this engine");
-            if (ck.mos1_models.count(c.model)) {
+            if (ck.bsim4_models.count(c.model)) {  // BSIM4 definitions take precedence (elab.rs:143-147)
+                if (!ck.bsim4_insts.count(c.params)) fail(S21_ERR, "Parameters not defined: " + c.params);
+                const int mi = def_index(c.model, ck.bsim4_models, b4_model_idx, e.bsim4_models);
+                const int ii = def_index(c.params, ck.bsim4_insts, b4_inst_idx, e.bsim4_insts);
+                path.pop_back();  // no internal nodes at the default mode selectors (bsim4ports.rs:31-112)
+                push(S21_COMP_BSIM4, nd, 4, mi, ii, 0.0, 0.0, c.name);
+            } else if (ck.mos1_models.count(c.model)) {
                 if (!ck.mos1_insts.count(c.params)) fail(S21_ERR, "Parameters not defined: " + c.params);
                 const s21_params& mp = ck.mos1_models.at(c.model);
                 const int mi = def_index(c.model, ck.mos1_models, m1_model_idx, e.mos1_models);
@@ -578,6 +630,16 @@ static void view(const s21_elab& e, s21_netlist& n) {
     n.diode_models = e.diode_models.data();
     n.n_diode_insts = (int32_t)e.diode_insts.size();
     n.diode_insts = e.diode_insts.data();
+    n.n_bsim4_models = (int32_t)e.bsim4_models.size();
+    n.bsim4_models = e.bsim4_models.data();
+    n.n_bsim4_insts = (int32_t)e.bsim4_insts.size();
+    n.bsim4_insts = e.bsim4_insts.data();
+}
+
+static bool has_bsim4(const s21_netlist& nl) {
+    for (int k = 0; k < nl.n_comps; k++)
+        if (nl.comps[k].type == S21_COMP_BSIM4) return true;
+    return false;
 }
 
 // SimOptions (protos/spice21.proto:136-142) -> Options (analysis.rs:665-688)
@@ -686,6 +748,20 @@ struct BatchPtr {
         if (b) s21_batch_free(b);
     }
 };
+struct LargePtr {
+    s21_large* l = nullptr;
+    ~LargePtr() {
+        if (l) s21_large_free(l);
+    }
+};
+static void check_status(s21_large* l) {
+    int32_t st = 0;
+    int64_t it = 0;
+    if (s21_large_get_status(l, &st, &it) != S21_OK) fail(S21_ERR, "status unavailable");
+    static const char* txt[] = {"", "Error", "Convergence Failed", "Singular Matrix", "Pivot Search Fail",
+                                "Decode Error", "Invalid Argument", "Device Error", "Convergence Failed"};
+    if (st != S21_OK) fail(st, st >= 0 && st <= 8 ? txt[st] : "Error");
+}
 
 static void engine_check(int rc) {
     if (rc != S21_OK) {
@@ -749,13 +825,21 @@ int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len) 
         Elaborator(c.ckt, e).run();
         s21_netlist nl;
         view(e, nl);
-        s21_group g{&nl, 1, 0, nullptr};
-        BatchPtr bp;
-        engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
-        engine_check(s21_batch_op(bp.b));
-        check_status(bp.b);
         std::vector<double> x(nl.n_vars);
-        engine_check(s21_batch_get_x(bp.b, 0, x.data()));
+        if (has_bsim4(nl)) {  // BSIM4 circuits run on the single-instance path
+            LargePtr lp;
+            engine_check(s21_large_create(&nl, &c.opts, S21_FP64, &lp.l));
+            engine_check(s21_large_op(lp.l));
+            check_status(lp.l);
+            engine_check(s21_large_get_x(lp.l, x.data()));
+        } else {
+            s21_group g{&nl, 1, 0, nullptr};
+            BatchPtr bp;
+            engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
+            engine_check(s21_batch_op(bp.b));
+            check_status(bp.b);
+            engine_check(s21_batch_get_x(bp.b, 0, x.data()));
+        }
         std::map<std::string, double> vals;  // OpResult::from keeps the last value of a repeated name
         for (int k = 0; k < nl.n_vars; k++) vals[e.names[k]] = x[k];
         Writer w;
@@ -792,16 +876,38 @@ int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len
             ic_name.push_back(kv.first.c_str());
         }
         s21_tran_options to{c.tstep, c.tstop, (int32_t)ic_var.size(), ic_var.data(), ic_val.data(), ic_name.data()};
-        s21_group g{&nl, 1, 0, nullptr};
         BatchPtr bp;
-        engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
-        engine_check(s21_batch_tran(bp.b, &to));
-        check_status(bp.b);
-        const int64_t np = s21_batch_n_points(bp.b);
-        const int nv = s21_batch_n_vars(bp.b, 0);
-        std::vector<double> t(np), wv((size_t)np * nv);
-        engine_check(s21_batch_get_axis(bp.b, t.data()));
-        engine_check(s21_batch_get_wave(bp.b, 0, 0, wv.data()));
+        LargePtr lp;
+        int64_t np;
+        int nv;
+        std::vector<double> t, wv;
+        std::vector<std::string> vnames;
+        if (has_bsim4(nl)) {  // BSIM4 circuits run on the single-instance path
+            engine_check(s21_large_create(&nl, &c.opts, S21_FP64, &lp.l));
+            engine_check(s21_large_tran(lp.l, &to));
+            check_status(lp.l);
+            np = s21_large_n_points(lp.l);
+            nv = s21_large_n_vars(lp.l);
+            t.resize(np);
+            wv.resize((size_t)np * nv);
+            std::vector<int32_t> all(nv);
+            for (int k = 0; k < nv; k++) all[k] = k;
+            engine_check(s21_large_get_axis(lp.l, t.data()));
+            engine_check(s21_large_get_wave(lp.l, nv, all.data(), wv.data()));
+            for (int k = 0; k < nv; k++) vnames.push_back(s21_large_var_name(lp.l, k));
+        } else {
+            s21_group g{&nl, 1, 0, nullptr};
+            engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
+            engine_check(s21_batch_tran(bp.b, &to));
+            check_status(bp.b);
+            np = s21_batch_n_points(bp.b);
+            nv = s21_batch_n_vars(bp.b, 0);
+            t.resize(np);
+            wv.resize((size_t)np * nv);
+            engine_check(s21_batch_get_axis(bp.b, t.data()));
+            engine_check(s21_batch_get_wave(bp.b, 0, 0, wv.data()));
+            for (int k = 0; k < nv; k++) vnames.push_back(s21_batch_var_name(bp.b, 0, k));
+        }
         Writer w;
         {
             Writer arr;
@@ -813,7 +919,7 @@ int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len
         for (int k = 0; k < nv; k++) {
             std::vector<double> col(np);
             for (int64_t p = 0; p < np; p++) col[p] = wv[(size_t)p * nv + k];
-            vals[s21_batch_var_name(bp.b, 0, k)] = col;
+            vals[vnames[k]] = col;
         }
         for (auto& kv : vals) {
             Writer arr, ent;
@@ -835,6 +941,8 @@ int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len) 
         Elaborator(c.ckt, e).run();
         s21_netlist nl;
         view(e, nl);
+        // the reference's BSIM4 has no AC stamps: Component::load_ac's default (comps/mod.rs:86-88)
+        if (has_bsim4(nl)) fail(S21_ERR, "AC Not Implemented For This Component!");
         s21_ac_options ao{c.fstart, c.fstop, c.npts};
         s21_group g{&nl, 1, 0, nullptr};
         BatchPtr bp;

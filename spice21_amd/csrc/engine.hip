@@ -802,6 +802,9 @@ static void group_from(GroupHost& g, const s21_group& sg) {
         g.kinds.push_back(nl.var_kind ? nl.var_kind[k] : S21_VAR_V);
     }
     g.comps.assign(nl.comps, nl.comps + nl.n_comps);
+    for (const s21_comp& c : g.comps)
+        if (c.type == S21_COMP_BSIM4)
+            fail(S21_INVALID, "BSIM4 devices run on the single-instance path (s21_large_*), not in a batch");
     if (nl.n_mos1_models) g.mos1_models.assign(nl.mos1_models, nl.mos1_models + nl.n_mos1_models);
     if (nl.n_mos1_insts) g.mos1_insts.assign(nl.mos1_insts, nl.mos1_insts + nl.n_mos1_insts);
     if (nl.n_diode_models) g.diode_models.assign(nl.diode_models, nl.diode_models + nl.n_diode_models);

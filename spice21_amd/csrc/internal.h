@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/spice21_mi355x.h"
+#include "bsim4.h"
 #include "layout.h"
 #include "models.h"
 
@@ -127,6 +128,7 @@ void topo_records(Topology& t);
 struct HostInstance {
     std::vector<Mos1Derived> m1;
     std::vector<DiodeP> dio;
+    std::vector<b4::B4Par> b4;      // per BSIM4 in component order
     std::vector<double> lin_value;  // per linear component, operating-point value
     std::vector<double> dio_vd;     // per diode, the guess its limiting starts from (empty: 0, as at x = 0)
 };

@@ -26,6 +26,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "bsim4_eval.h"
 #include "internal.h"
 #include "large.h"
 #include "models.h"
@@ -190,6 +191,47 @@ __global__ void __launch_bounds__(TPB) k_eval(EvalArgs E) {
             cv[1] = -val;
         }
     }
+}
+
+// BSIM4, one thread per device (bsim4solver.rs:134-145, 3757-3769): Bsim4::op from the port voltages with
+// limiting against the device's last guess, tran_op's Backward-Euler charge companions against the committed
+// point in a transient, then the 42 stamps into the device's contribution cells.  Parameters and states are
+// arrays of structures in HBM; the new guess is written in place once the old one's limiting fields are in
+// registers.
+struct B4Args {
+    int n, tran;
+    double gmin, dt;
+    const double* x;
+    const int* node;          // [n][4] d g s b (-1 = ground)
+    const b4::B4Par* par;     // [n]
+    b4::B4Op* guess;          // [n] the last load's point
+    const b4::B4Op* op;       // [n] the committed point
+    const int* cb;            // [n] first contribution cell
+    double* cv;
+};
+
+__global__ void __launch_bounds__(128) k_eval_b4(B4Args A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n) return;
+    const int* nd = A.node + 4 * j;
+    const b4::B4Par& P = A.par[j];
+    b4::B4Op G;  // only the limiting fields of the old guess are read (vds, vgs, vbs, vbd, von)
+    G.vds = A.guess[j].vds;
+    G.vgs = A.guess[j].vgs;
+    G.vbs = A.guess[j].vbs;
+    G.vbd = A.guess[j].vbd;
+    G.von = A.guess[j].von;
+    b4::B4Op& N = A.guess[j];
+    b4::op_eval(P, G, xval(A.x, nd[0]), xval(A.x, nd[1]), xval(A.x, nd[2]), xval(A.x, nd[3]), A.gmin, N);
+    b4::B4Tran T{};
+    if (A.tran) b4::tran_eval(P, A.op[j], N, A.dt, T);
+    double* cv = A.cv + A.cb[j];
+    b4::stamp(P, N, T, [&](int k, double v) { cv[k] = v; });
+}
+
+__global__ void k_commit_b4(int n, const b4::B4Op* __restrict__ guess, b4::B4Op* __restrict__ op) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) op[j] = guess[j];
 }
 
 // Each matrix element (then each RHS entry) sums its contributions in component-then-stamp order.
@@ -704,13 +746,15 @@ __global__ void k_permute_out(int n, const int* __restrict__ perm, const T* __re
     if (b < n) dx[perm[b]] = (double)y[b];
 }
 
-// max |dx| over the non-NaN entries (the reference's fold with f64::max), as ordered bit patterns
-__global__ void k_maxabs(int n, const double* __restrict__ dx, unsigned long long* out) {
+// max |dx| over the non-NaN entries (the reference's fold with f64::max), as ordered bit patterns;
+// *nonfinite = 1 when some entry is NaN or infinite
+__global__ void k_maxabs(int n, const double* __restrict__ dx, unsigned long long* out, int* nonfinite) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
     if (i < n) {
         const double a = fabs(dx[i]);
         if (a == a) v = a;
+        if (!(a <= 1.7976931348623157e308)) atomicOr(nonfinite, 1);
     }
     for (int o = 32; o > 0; o >>= 1) {
         const double u = __shfl_xor(v, o, 64);
@@ -718,11 +762,17 @@ __global__ void k_maxabs(int n, const double* __restrict__ dx, unsigned long lon
     }
     if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
-// clamp and update (analysis.rs:196-207); flags[0] |= 4 when some |dx| > reltol after the clamp
+// clamp and update (analysis.rs:196-207); flags[0] |= 4 when some |dx| > reltol after the clamp.  A
+// solve with reused factors whose dx is not finite is discarded whole (x unchanged, not converged): the
+// iteration is repeated with fresh factors.
 __global__ void k_update(int n, const double* __restrict__ dx, const unsigned long long* mxbits, double reltol,
-                         double* __restrict__ x, int* flags) {
+                         double* __restrict__ x, int* flags, int reused, const int* __restrict__ nonfinite) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (reused && *nonfinite) {
+        if (i == 0) atomicOr(flags, 4);
+        return;
+    }
     const double mx = __longlong_as_double((long long)*mxbits);
     double d = dx[i];
     if (mx > 1.0) d = d * 1.0 / mx;
@@ -844,7 +894,7 @@ struct s21_large {
     std::vector<std::string> names;
     std::vector<int> kinds;
     std::vector<s21_comp> comps;
-    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts;
+    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts, bsim4_models, bsim4_insts;
     s21_options opts;
     int precision = S21_FP64;
     int device = 0;
@@ -855,10 +905,11 @@ struct s21_large {
     std::string prepared;
     s21::Topology t;
     s21::LargeSym S;
-    int n_m1 = 0, n_d = 0, n_lin = 0, n_cells = 0, n_elem = 0;
+    int n_m1 = 0, n_d = 0, n_lin = 0, n_b4 = 0, n_cells = 0, n_elem = 0;
     std::vector<s21::lg::Level> levels;
     std::vector<int> big_host;  // the big-front launch list (host copy)
     int reuse = -1;             // factor reuse: 1 on, 0 off (factor every iteration), -1 = on in mixed precision
+    bool has_bsim4 = false;     // BSIM4 devices present: no factor reuse
     bool have_factors = false;  // the pool holds factors of this structure
     int reused = 0;             // iterations since the last factorisation
     s21::lg::Buf m1_node, m1_par, m1_caps, m1_state, m1_cb, d_node, d_par, d_state, d_cb, l_node, l_type, l_val, c_q,
@@ -866,6 +917,7 @@ struct s21_large {
     s21::lg::Buf f_fo, f_first, f_rptr, f_ridx, f_chptr, f_ch, f_xptr, f_xmap, f_aptr, f_aelem, f_aoff, f_woff;
     s21::lg::Buf l_small, l_big, l_jobs, l_all, perm, prow, pool, yv, wv;
     s21::lg::Buf wave;
+    s21::lg::Buf b4_node, b4_par, b4_guess, b4_op, b4_cb;
     // trapezoidal integration: per-charge currents, unknown kinds, accepted-point history
     s21::lg::Buf m1_icur, d_icur, c_icur, kinds_dev, hist[4];
     int trap = 0;
@@ -930,7 +982,13 @@ static int entry_targets(const Comp& c, const std::unordered_map<int64_t, int>& 
         tgt[0] = B(nd[0]);
         tgt[1] = B(nd[1]);
         return I_NE;
-    default: fail(S21_INVALID, "the large-instance path takes R, C, V, I, Mos1 and Diode components");
+    case S21_COMP_BSIM4:
+        for (int k = 0; k < b4::B4_NS; k++) {
+            const int r = nd[b4::b4_row(k)];
+            tgt[k] = b4::b4_col(k) < 0 ? B(r) : M(r, nd[b4::b4_col(k)]);
+        }
+        return b4::B4_NS;
+    default: fail(S21_INVALID, "the large-instance path takes R, C, V, I, Mos1, Diode and BSIM4 components");
     }
 }
 
@@ -1004,23 +1062,35 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     HostInstance h0;
     std::vector<int> m1_node, d_node, l_node, l_type, m1_caps;
     std::vector<double> m1p, dpar, lval0, lval1;
-    std::vector<int> m1_comp, d_comp, l_comp;
+    std::vector<int> m1_comp, d_comp, l_comp, b4_comp, b4_node;
     for (size_t ci = 0; ci < t.comps.size(); ci++) {
         const Comp& c = t.comps[ci];
-        if (c.type == S21_COMP_MOS1) {
+        if (c.type == S21_COMP_BSIM4) {
+            if (c.model < 0 || c.model >= (int)L->bsim4_models.size() || c.inst < 0 ||
+                c.inst >= (int)L->bsim4_insts.size())
+                fail(S21_ERR, "Parameters not defined");
+            try {
+                h0.b4.push_back(b4::derive_params(L->bsim4_models[c.model], L->bsim4_insts[c.inst]));
+            } catch (const std::domain_error& e) {  // the reference panics here (bsim4inst.rs:200-223)
+                fail(S21_ERR, e.what());
+            }
+            b4_comp.push_back((int)ci);
+            for (int r = 0; r < 4; r++) b4_node.push_back(c.node[r]);
+        } else if (c.type == S21_COMP_MOS1) {
             h0.m1.push_back(derive_mos1(L->mos1_models[c.model], L->mos1_insts[c.inst], L->opts));
             m1_comp.push_back((int)ci);
         } else if (c.type == S21_COMP_DIODE) {
             h0.dio.push_back(derive_diode(L->diode_models[c.model], L->diode_insts[c.inst], L->opts));
             d_comp.push_back((int)ci);
         } else if (c.type == S21_COMP_MOS0) {
-            fail(S21_INVALID, "the large-instance path takes R, C, V, I, Mos1 and Diode components");
+            fail(S21_INVALID, "the large-instance path takes R, C, V, I, Mos1, Diode and BSIM4 components");
         } else {
             h0.lin_value.push_back(c.value);
             l_comp.push_back((int)ci);
         }
     }
-    const int nm = (int)m1_comp.size(), nd = (int)d_comp.size(), nl = (int)l_comp.size();
+    const int nm = (int)m1_comp.size(), nd = (int)d_comp.size(), nl = (int)l_comp.size(), nb4 = (int)b4_comp.size();
+    L->n_b4 = nb4;
     L->n_m1 = nm;
     L->n_d = nd;
     L->n_lin = nl;
@@ -1062,8 +1132,13 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     std::unordered_map<int64_t, int> elem;
     elem.reserve((size_t)ne * 2);
     for (int e = 0; e < ne; e++) elem[ekey(t.erow[e], t.ecol[e], n)] = e;
-    std::vector<int> m1_cb(nm), d_cb(nd), l_cb(nl), comp_cb(t.comps.size(), -1);
+    std::vector<int> m1_cb(nm), d_cb(nd), l_cb(nl), b4_cb(nb4), comp_cb(t.comps.size(), -1);
     int cells = 0;
+    for (int j = 0; j < nb4; j++) {
+        b4_cb[j] = cells;
+        comp_cb[b4_comp[j]] = cells;
+        cells += b4::B4_NS;
+    }
     for (int j = 0; j < nm; j++) {
         m1_cb[j] = cells;
         comp_cb[m1_comp[j]] = cells;
@@ -1083,7 +1158,7 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     const int nslot = ne + n;
     std::vector<int> cnt(nslot + 1, 0);
     std::vector<std::pair<int, int>> contrib;  // (slot, cell) in component order
-    int tg[MOS1_NE];
+    int tg[b4::B4_NS > MOS1_NE ? b4::B4_NS : MOS1_NE];
     for (size_t ci = 0; ci < t.comps.size(); ci++) {
         const int k = entry_targets(t.comps[ci], elem, n, ne, tg);
         for (int e = 0; e < k; e++)
@@ -1177,6 +1252,11 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     upload(L->l_type, l_type);
     upload(L->l_val, lval);
     upload(L->l_cb, l_cb);
+    upload(L->b4_node, b4_node);
+    upload(L->b4_par, h0.b4);
+    upload(L->b4_cb, b4_cb);
+    L->b4_guess.alloc(std::max<size_t>(nb4, 1) * sizeof(b4::B4Op));
+    L->b4_op.alloc(std::max<size_t>(nb4, 1) * sizeof(b4::B4Op));
     L->m1_state.alloc(std::max<size_t>((size_t)2 * M1S * nm, 1) * sizeof(double));
     L->d_state.alloc(std::max<size_t>((size_t)DSF * nd, 1) * sizeof(double));
     L->c_q.alloc(std::max<size_t>(nl, 1) * sizeof(double));
@@ -1194,7 +1274,7 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     L->x.alloc((size_t)std::max(n, 1) * sizeof(double));
     L->res.alloc((size_t)std::max(n, 1) * sizeof(double));
     L->dx.alloc((size_t)std::max(n, 1) * sizeof(double));
-    L->flags.alloc(8 * sizeof(int));  // 3 flags, padding, the residual's max |.| (8 bytes)
+    L->flags.alloc(8 * sizeof(int));  // 3 flags, the non-finite-dx flag, the residual's max |.| (8 bytes)
     L->mxbits.alloc(sizeof(unsigned long long));
     upload(L->f_fo, S.front_off);
     upload(L->f_first, S.sn_first);
@@ -1328,6 +1408,9 @@ static void commit_state(s21_large* L, bool tran, double dt) {
     const EvalArgs E = eval_args(L, tran, dt);
     hipLaunchKernelGGL(k_commit, dim3(blocks(L->n_m1 + L->n_d + L->n_lin)), dim3(TPB), 0, L->stream, E,
                        L->c_q.as<double>());
+    if (L->n_b4)
+        hipLaunchKernelGGL(k_commit_b4, dim3(blocks(L->n_b4)), dim3(TPB), 0, L->stream, L->n_b4,
+                           L->b4_guess.as<b4::B4Op>(), L->b4_op.as<b4::B4Op>());
     LCHECK(hipGetLastError());
 }
 
@@ -1336,6 +1419,20 @@ static void assemble_residual(s21_large* L, bool tran, double dt) {
     const int n = L->t.n, nslot = L->n_elem + n;
     const EvalArgs E = eval_args(L, tran, dt);
     hipLaunchKernelGGL(k_eval, dim3(blocks(L->n_m1 + L->n_d + L->n_lin)), dim3(TPB), 0, L->stream, E);
+    if (L->n_b4) {
+        B4Args A{L->n_b4,
+                 tran ? 1 : 0,
+                 L->opts.gmin,
+                 dt,
+                 L->x.as<double>(),
+                 L->b4_node.as<int>(),
+                 L->b4_par.as<b4::B4Par>(),
+                 L->b4_guess.as<b4::B4Op>(),
+                 L->b4_op.as<b4::B4Op>(),
+                 L->b4_cb.as<int>(),
+                 L->cv.as<double>()};
+        hipLaunchKernelGGL(k_eval_b4, dim3((L->n_b4 + 127) / 128), dim3(128), 0, L->stream, A);
+    }
     hipLaunchKernelGGL(k_gather, dim3(blocks(nslot)), dim3(TPB), 0, L->stream, nslot, L->g_ptr.as<int>(),
                        L->g_idx.as<int>(), L->cv.as<double>(), L->slots.as<double>());
     LCHECK(hipMemsetAsync(L->flags.p, 0, sizeof(int), L->stream));
@@ -1358,6 +1455,7 @@ static int newton(s21_large* L, bool tran, double dt, bool commit = true) {
     bool prev_ok = true;  // dx starts at zero
     LCHECK(hipMemsetAsync(L->flags.p, 0, 8 * sizeof(int), L->stream));
     double res_prev = 0.0;
+    bool last_reused = false;
     for (int it = 0; it < 100; it++) {  // analysis.rs:173
         L->iters++;
         assemble_residual(L, tran, dt);
@@ -1376,9 +1474,10 @@ static int newton(s21_large* L, bool tran, double dt, bool commit = true) {
             std::memcpy(&bits, hf + 4, sizeof(bits));
             res_now = __builtin_bit_cast(double, bits);
         }
-        // the first iteration of a solve has no residual of its own to compare with: it keeps the factors
-        const bool refactor =
-            !L->reuse || !L->have_factors || L->reused >= 32 || (it > 0 && !(res_now <= 0.25 * res_prev));
+        // the first iteration of a solve has no residual of its own to compare with: it keeps the factors;
+        // a reused solve that came out non-finite (hf[3]) was discarded and is repeated with fresh factors
+        const bool refactor = !L->reuse || !L->have_factors || L->reused >= 32 || (hf[3] && last_reused) ||
+                              (it > 0 && !(res_now <= 0.25 * res_prev));
         res_prev = res_now;
         LCHECK(hipMemsetAsync(L->flags.as<int>() + 1, 0, 3 * sizeof(int), L->stream));
         if (refactor) {
@@ -1391,23 +1490,25 @@ static int newton(s21_large* L, bool tran, double dt, bool commit = true) {
         } else {
             L->reused++;
         }
+        last_reused = !refactor;
         if (L->precision == S21_MIXED)
             solve<float>(L);
         else
             solve<double>(L);
         LCHECK(hipMemsetAsync(L->mxbits.p, 0, sizeof(unsigned long long), L->stream));
         hipLaunchKernelGGL(k_maxabs, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->dx.as<double>(),
-                           L->mxbits.as<unsigned long long>());
+                           L->mxbits.as<unsigned long long>(), L->flags.as<int>() + 3);
         hipLaunchKernelGGL(k_update, dim3(blocks(n)), dim3(TPB), 0, L->stream, n, L->dx.as<double>(),
                            L->mxbits.as<unsigned long long>(), L->opts.reltol, L->x.as<double>(),
-                           L->flags.as<int>() + 2);
+                           L->flags.as<int>() + 2, last_reused ? 1 : 0, L->flags.as<int>() + 3);
         LCHECK(hipGetLastError());
     }
     return S21_CONVERGENCE;
 }
 
 static void reset_state(s21_large* L) {
-    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x, &L->m1_icur, &L->d_icur, &L->c_icur})
+    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x, &L->m1_icur, &L->d_icur, &L->c_icur, &L->b4_guess,
+                   &L->b4_op})
         LCHECK(hipMemsetAsync(b->p, 0, b->bytes, L->stream));
     L->iters = 0;
     L->factorisations = 0;
@@ -1476,6 +1577,8 @@ int s21_large_create(const s21_netlist* net, const s21_options* opts, int32_t pr
             if (k > 0 && !p) fail(S21_INVALID, "malformed netlist");
             v.assign(p, p + std::max(k, 0));
         };
+        take(nl.n_bsim4_models, nl.bsim4_models, L->bsim4_models);
+        take(nl.n_bsim4_insts, nl.bsim4_insts, L->bsim4_insts);
         take(nl.n_mos1_models, nl.mos1_models, L->mos1_models);
         take(nl.n_mos1_insts, nl.mos1_insts, L->mos1_insts);
         take(nl.n_diode_models, nl.diode_models, L->diode_models);
@@ -1485,7 +1588,10 @@ int s21_large_create(const s21_netlist* net, const s21_options* opts, int32_t pr
         else
             s21_default_options(&L->opts);
         L->precision = precision == S21_MIXED ? S21_MIXED : S21_FP64;
-        L->reuse = L->precision == S21_MIXED ? 1 : 0;
+        bool bsim4 = false;
+        for (const s21_comp& c : L->comps) bsim4 = bsim4 || c.type == S21_COMP_BSIM4;
+        L->has_bsim4 = bsim4;
+        L->reuse = L->precision == S21_MIXED && !bsim4 ? 1 : 0;
         LCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
         LCHECK(hipEventCreate(&L->ev0));
         LCHECK(hipEventCreate(&L->ev1));
@@ -1559,6 +1665,9 @@ int s21_large_tran_trap(s21_large* L, const s21_tran_options* tr, const s21_trap
         if (!L || !tr || !o) fail(S21_INVALID, "null argument");
         if (!(tr->tstep > 0.0) || !(tr->tstop >= 0.0)) fail(S21_INVALID, "tstep must be positive");
         check_current_device(L);
+        for (const s21_comp& c : L->comps)
+            if (c.type == S21_COMP_BSIM4)  // tran_op bakes ag0 = 1 / dt into its companions (tran.rs:86)
+                fail(S21_INVALID, "BSIM4 devices integrate by Backward Euler only");
         L->prepared.clear();
         prepare(L, tr);
         L->prepared = "tran";
@@ -1743,6 +1852,7 @@ int s21_large_timing(const s21_large* L, double* total_ms, int64_t* factorisatio
 int s21_large_set_reuse(s21_large* L, int32_t on) {
     return lguard([&] {
         if (!L) fail(S21_INVALID, "null instance");
+        if (on && L->has_bsim4) fail(S21_INVALID, "factor reuse is not available with BSIM4 devices");
         L->reuse = on ? 1 : 0;
     });
 }

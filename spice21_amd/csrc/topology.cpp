@@ -2,7 +2,8 @@
 //
 //  * topo_elements   -- matrix elements in the order Component::create_matrix_elems makes them
 //                       (comps/mod.rs:127-132, 190-195, 291-298; comps/mos.rs:896-903, 1044-1050;
-//                       comps/diode.rs:248-256), deduplicated as Matrix::make does (sparse21/mod.rs:265).
+//                       comps/diode.rs:248-256; comps/bsim4/bsim4solver.rs:28-116), deduplicated as
+//                       Matrix::make does (sparse21/mod.rs:265).
 //  * topo_pivot      -- the reference's pivot choice on the first Newton iteration's matrix: Markowitz
 //                       diagonal search with relative threshold 1e-3 and the x5 tie cut-off, the
 //                       column-n submatrix search, then the global maximum; the last index is never
@@ -16,6 +17,7 @@
 #include <numeric>
 #include <unordered_map>
 
+#include "bsim4_eval.h"
 #include "internal.h"
 
 namespace s21 {
@@ -82,6 +84,14 @@ void topo_elements(Topology& t) {
             make(nd[2], nd[1]);
             make(nd[1], nd[1]);
             break;
+        case S21_COMP_BSIM4: {  // create_matps with every internal node on its terminal, no charge node
+            enum { D = 0, G = 1, S = 2, B = 3 };
+            static const int order[22][2] = {{D, B}, {G, B}, {S, B}, {B, D}, {B, G}, {B, S}, {B, B}, {D, D},
+                                             {G, G}, {S, S}, {D, D}, {S, S}, {D, D}, {G, D}, {G, S}, {S, S},
+                                             {D, S}, {D, D}, {D, G}, {S, G}, {S, S}, {S, D}};
+            for (auto& q : order) make(nd[q[0]], nd[q[1]]);
+            break;
+        }
         default: fail(S21_INVALID, "Invalid Component");
         }
     }
@@ -165,10 +175,23 @@ void host_assemble_op(const Topology& t, const HostInstance& h, const s21_option
     auto addb = [&](int r, double v) {
         if (r >= 0) rhs[r] += v;
     };
-    int im1 = 0, id = 0, il = 0;
+    int im1 = 0, id = 0, il = 0, ib4 = 0;
     for (const Comp& c : t.comps) {
         const int* nd = c.node;
         switch (c.type) {
+        case S21_COMP_BSIM4: {  // the first load of a fresh device: its guess is all zero
+            const b4::B4Par& P = h.b4[ib4++];
+            b4::B4Op N;
+            b4::op_eval(P, b4::B4Op{}, X(nd[0]), X(nd[1]), X(nd[2]), X(nd[3]), o.gmin, N);
+            b4::stamp(P, N, b4::B4Tran{}, [&](int k, double v) {
+                const int r = nd[b4::b4_row(k)];
+                if (b4::b4_col(k) < 0)
+                    addb(r, v);
+                else
+                    addm(r, nd[b4::b4_col(k)], v);
+            });
+            break;
+        }
         case S21_COMP_MOS1: {
             const Mos1Derived& D = h.m1[im1++];
             mos1_load<false>(D.core, nullptr, nullptr, X(nd[RD]), X(nd[RG]), X(nd[RS]), X(nd[RB]), o.gmin, 0.0,

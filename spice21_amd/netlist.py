@@ -22,6 +22,8 @@ class Netlist:
         self.mos1_insts = []
         self.diode_models = []
         self.diode_insts = []
+        self.bsim4_models = []
+        self.bsim4_insts = []
         self._mos1_model_r = []  # (has_rd_or_rsh, has_rs_or_rsh)
         self._diode_has_rs = []
         for s in signals:
@@ -76,6 +78,15 @@ class Netlist:
         self.diode_insts.append(A.s21_params.make(A.DIODE_INST_FIELDS, **kw))
         return len(self.diode_insts) - 1
 
+    def bsim4_model(self, mos_type=0):
+        """Bsim4Model: only the polarity (0 NMOS, 1 PMOS) reaches the model (comps/bsim4/model/mod.rs:21-30)."""
+        self.bsim4_models.append(A.s21_params.make(A.BSIM4_MODEL_FIELDS, mos_type=mos_type))
+        return len(self.bsim4_models) - 1
+
+    def bsim4_inst(self, **kw):
+        self.bsim4_insts.append(A.s21_params.make(A.BSIM4_INST_FIELDS, **kw))
+        return len(self.bsim4_insts) - 1
+
     # ---- components (elab.rs:55-173) ----
     def _push(self, typ, nodes, model=0, inst=0, value=0.0, acm=0.0, name=""):
         nodes = list(nodes) + [A.S21_GND] * (6 - len(nodes))
@@ -102,6 +113,11 @@ class Netlist:
         dp = self.addv(f"{name}.dp") if has_rd else nd[0]
         sp = self.addv(f"{name}.sp") if has_rs else nd[2]
         return self._push(A.S21_COMP_MOS1, nd + [dp, sp], model=model, inst=inst, name=name)
+
+    def bsim4(self, name, d, g, s, b, model, inst):
+        """A BSIM4 device: no internal nodes at the default mode selectors (bsim4ports.rs:31-112)."""
+        nd = [self.node(d), self.node(g), self.node(s), self.node(b)]
+        return self._push(A.S21_COMP_BSIM4, nd, model=model, inst=inst, name=name)
 
     def mos0(self, name, d, g, s, b, mos_type=0):
         nd = [self.node(d), self.node(g), self.node(s), self.node(b)]
@@ -183,10 +199,12 @@ class CNetlist:
 
         self.mm, self.mi = arr(nl.mos1_models), arr(nl.mos1_insts)
         self.dm, self.di = arr(nl.diode_models), arr(nl.diode_insts)
+        self.bm, self.bi = arr(nl.bsim4_models), arr(nl.bsim4_insts)
         self.net = A.s21_netlist(
             n, self.kinds, self.names, m, self.comps,
             len(nl.mos1_models), self.mm, len(nl.mos1_insts), self.mi,
             len(nl.diode_models), self.dm, len(nl.diode_insts), self.di,
+            len(nl.bsim4_models), self.bm, len(nl.bsim4_insts), self.bi,
         )
         self.n_vars = n
         self.var_names = list(nl.names)

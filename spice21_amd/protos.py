@@ -3,7 +3,8 @@
 The schema restates the messages of spice21/protos/spice21.proto and spice21/protos/mos.proto that the
 analyses take and return, with the same package, message and field names and field numbers, so bytes
 produced here decode in the reference and vice versa.  (The image has no protoc; the descriptors are
-assembled directly.)  BSIM4 messages carry only their names here: BSIM4 devices are not simulated.
+assembled directly.)  The BSIM4 messages follow spice21/protos/bsim4.proto: Bsim4Model carries the polarity
+(its other parameters are commented out there) and Bsim4InstParams fields 1-31.
 """
 from google.protobuf import descriptor_pb2, descriptor_pool, message_factory, wrappers_pb2
 
@@ -12,6 +13,7 @@ _D, _U64, _I64, _S, _M, _E = F.TYPE_DOUBLE, F.TYPE_UINT64, F.TYPE_INT64, F.TYPE_
 _OPT, _REP = F.LABEL_OPTIONAL, F.LABEL_REPEATED
 _DV = ".google.protobuf.DoubleValue"
 _IV = ".google.protobuf.Int64Value"
+_UV = ".google.protobuf.UInt64Value"
 
 # message name -> list of (field name, number, type, label, type name | None, oneof | None)
 _SCHEMA = {
@@ -38,8 +40,15 @@ _SCHEMA = {
                                           ("mjsw", 21), ("js", 22), ("tox", 23), ("ld", 24), ("u0", 25), ("fc", 26),
                                           ("nsub", 27), ("nss", 29), ("tnom", 30), ("kf", 31), ("af", 32)]]
     + [("tpg", 28, _M, _OPT, _IV)],
-    "Bsim4Model": [("name", 1, _S), ("mos_type", 2, _E, _OPT, ".spice21.MosType")],
-    "Bsim4InstParams": [("name", 1, _S)],
+    "Bsim4Model": [("mos_type", 1, _E, _OPT, ".spice21.MosType"), ("name", 900, _S)],
+    "Bsim4InstParams": [(n, k, _M, _OPT, _UV if k in (17, 18, 27, 28, 29, 30, 31) else _DV)
+                        for n, k in [("l", 1), ("w", 2), ("nf", 3), ("sa", 4), ("sb", 5), ("sd", 6), ("sca", 7),
+                                     ("scb", 8), ("scc", 9), ("sc", 10), ("ad", 11), ("as", 12), ("pd", 13),
+                                     ("ps", 14), ("nrd", 15), ("nrs", 16), ("min", 17), ("rgeomod", 18),
+                                     ("rbdb", 19), ("rbsb", 20), ("rbpb", 21), ("rbps", 22), ("rbpd", 23),
+                                     ("delvto", 24), ("xgw", 25), ("ngcon", 26), ("trnqsmod", 27),
+                                     ("acnqsmod", 28), ("rbodymod", 29), ("rgatemod", 30), ("geomod", 31)]]
+    + [("name", 40, _S)],
     "Instance": [("r", 1, _M, _OPT, ".spice21.Resistor", 0), ("c", 2, _M, _OPT, ".spice21.Capacitor", 0),
                  ("m", 3, _M, _OPT, ".spice21.Mos", 0), ("i", 4, _M, _OPT, ".spice21.Isrc", 0),
                  ("v", 5, _M, _OPT, ".spice21.Vsrc", 0), ("d", 6, _M, _OPT, ".spice21.Diode", 0),

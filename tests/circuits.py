@@ -10,11 +10,22 @@ def mos1_defaults(n):
     return nmos, pmos, inst
 
 
+def bsim4_defaults(n):
+    """add_bsim4_defaults (tests.rs:1463-1474): default BSIM4 NMOS and PMOS models and instance parameters."""
+    return n.bsim4_model(0), n.bsim4_model(1), n.bsim4_inst()
+
+
 def ro3(kind):
     """cmos_ro3 / nmos_ro3 / pmos_ro3 (tests.rs:889-912, 966-990, 1009-1033) flattened by hand: module
-    instances x1..x3 map inp/out onto "1"->"2", "2"->"3", "3"->"1"; vss is ground."""
+    instances x1..x3 map inp/out onto "1"->"2", "2"->"3", "3"->"1"; vss is ground.  Kinds "cmos", "nmos",
+    "pmos" use the Mos1 defaults, "bsim4_cmos", "bsim4_nmos", "bsim4_pmos" the BSIM4 ones."""
     n = Netlist(signals=["1", "2", "3", "vdd"])
     stages = [("1", "2"), ("2", "3"), ("3", "1")]
+    if kind.startswith("bsim4_"):
+        nm, pm, ip = bsim4_defaults(n)
+        dev, kind = n.bsim4, kind[len("bsim4_"):]
+    else:
+        dev = n.mos1
     if kind == "mos0":
         n.v("v1", 1.0, "vdd", "")
         for x, (a, b) in enumerate(stages):
@@ -22,21 +33,22 @@ def ro3(kind):
             n.mos0(f"x{x+1}.n", b, a, "", "", 0)
             n.c(1e-15, b, "")
         return n
-    nm, pm, ip = mos1_defaults(n)
+    if dev == n.mos1:
+        nm, pm, ip = mos1_defaults(n)
     n.v("v1", 1.0, "vdd", "")
     for x, (a, b) in enumerate(stages):
         if kind == "cmos":
-            n.mos1(f"x{x+1}.p", b, a, "vdd", "vdd", pm, ip)
-            n.mos1(f"x{x+1}.n", b, a, "", "", nm, ip)
+            dev(f"x{x+1}.p", b, a, "vdd", "vdd", pm, ip)
+            dev(f"x{x+1}.n", b, a, "", "", nm, ip)
             n.c(1e-15, b, "")
         elif kind == "nmos":
-            n.mos1(f"x{x+1}.m", b, a, "", "", nm, ip)
+            dev(f"x{x+1}.m", b, a, "", "", nm, ip)
             n.r(1e-6, b, "vdd")
             n.c(0.5e-15, b, "vdd")
         elif kind == "pmos":
             n.c(1e-16, b, "")
             n.r(1e-6, b, "")
-            n.mos1(f"x{x+1}.m", b, a, "vdd", "vdd", pm, ip)
+            dev(f"x{x+1}.m", b, a, "vdd", "vdd", pm, ip)
         else:
             raise ValueError(kind)
     return n
@@ -48,6 +60,12 @@ GOLDEN_RUNS = {
     "test_mos1_nmos_ro_tran.json": ("nmos", 1e-11, 1e-8),
     "test_mos1_pmos_ro_tran.json": ("pmos", 1e-11, 1e-8),
     "test_mos0_cmos_ro_tran.json": ("mos0", 1e-15, 1e-12),
+}
+BSIM4_GOLDEN_RUNS = {
+    # tests.rs:948-965, 1053-1071, 1360-1377
+    "test_bsim4_cmos_ro_tran.json": ("bsim4_cmos", 1e-10, 3e-7),
+    "test_bsim4_pmos_ro_tran.json": ("bsim4_pmos", 1e-11, 1e-8),
+    "test_bsim4_nmos_ro_tran.json": ("bsim4_nmos", 1e-9, 1e-6),
 }
 
 

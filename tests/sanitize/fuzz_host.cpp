@@ -72,8 +72,16 @@ void symbolic(const s21_netlist& nl, bool tran) {
         s21_default_options(&o);
         HostInstance h;
         std::vector<uint8_t> caps;
+        bool bsim4 = false;
         for (const Comp& c : t.comps) {
-            if (c.type == S21_COMP_MOS1) {
+            if (c.type == S21_COMP_BSIM4) {  // single-instance path only: derivation and assembly, no schedule
+                try {
+                    h.b4.push_back(b4::derive_params(nl.bsim4_models[c.model], nl.bsim4_insts[c.inst]));
+                } catch (const std::domain_error&) {
+                    fail(S21_ERR, "BSIM4 derivation");
+                }
+                bsim4 = true;
+            } else if (c.type == S21_COMP_MOS1) {
                 h.m1.push_back(derive_mos1(nl.mos1_models[c.model], nl.mos1_insts[c.inst], o));
                 caps.push_back(h.m1.back().has_caps ? 1 : 0);
             } else if (c.type == S21_COMP_DIODE) {
@@ -87,7 +95,7 @@ void symbolic(const s21_netlist& nl, bool tran) {
         topo_pivot(t, vals);
         for (double& v : x) v = 2.0 * uniform() - 0.5;
         host_assemble_op(t, h, o, x, vals, rhs);
-        if (t.sym_status == S21_OK) {
+        if (t.sym_status == S21_OK && !bsim4) {
             topo_schedule(t, caps);
             topo_records(t);
         }

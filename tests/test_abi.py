@@ -27,7 +27,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version_and_defaults():
-    assert lib().s21_abi_version() == 1
+    assert lib().s21_abi_version() == 2
     o = A.s21_options()
     lib().s21_default_options(C.byref(o))
     assert (o.temp, o.tnom, o.gmin, o.iabstol, o.reltol) == (300.15, 300.15, 1e-12, 1e-12, 1e-3)

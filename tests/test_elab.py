@@ -28,7 +28,10 @@ def mos(name, g, d, s, b, model, params="default"):
 
 
 def ro3_proto(kind):
-    """cmos_ro3 / nmos_ro3 / pmos_ro3 (tests.rs:889-912, 966-990, 1009-1033) as protobuf."""
+    """cmos_ro3 / nmos_ro3 / pmos_ro3 (tests.rs:889-912, 966-990, 1009-1033) as protobuf, with the Mos1
+    defaults, or with the BSIM4 ones (add_bsim4_defaults, tests.rs:1463-1474) for kinds "bsim4_*"."""
+    bsim4 = kind.startswith("bsim4_")
+    kind = kind[len("bsim4_"):] if bsim4 else kind
     if kind == "cmos":
         body = [mos("p", "inp", "out", "vdd", "vdd", "pmos"), mos("n", "inp", "out", "vss", "vss", "nmos"),
                 inst(c=P.Capacitor(name="c", p="out", n="vss", c=1e-15))]
@@ -44,7 +47,12 @@ def ro3_proto(kind):
     module = P.Module(name="inv", ports=["inp", "out", "vdd", "vss"], signals=[], comps=body)
     c = P.Circuit(name="ro", signals=["1", "2", "3", "vdd"])
     c.defs.append(P.Def(module=module))
-    if kind != "mos0":
+    if bsim4:
+        c.defs.append(P.Def(bsim4model=P.Bsim4Model(name="default")))
+        c.defs.append(P.Def(bsim4model=P.Bsim4Model(name="nmos")))
+        c.defs.append(P.Def(bsim4model=P.Bsim4Model(name="pmos", mos_type=P.MosType.PMOS)))
+        c.defs.append(P.Def(bsim4inst=P.Bsim4InstParams(name="default")))
+    elif kind != "mos0":
         c.defs.append(P.Def(mos1model=P.Mos1Model(name="default")))
         c.defs.append(P.Def(mos1model=P.Mos1Model(name="nmos")))
         c.defs.append(P.Def(mos1model=P.Mos1Model(name="pmos", mos_type=P.MosType.PMOS)))
@@ -60,7 +68,7 @@ def ro3_proto(kind):
 MOS0 = {"default": 0, "nmos": 0, "pmos": 1}
 
 
-@pytest.mark.parametrize("kind", ["cmos", "nmos", "pmos", "mos0"])
+@pytest.mark.parametrize("kind", ["cmos", "nmos", "pmos", "mos0", "bsim4_cmos", "bsim4_nmos", "bsim4_pmos"])
 def test_elaboration_matches_hand_flattening(kind):
     e = elaborate(ro3_proto(kind), mos0=MOS0 if kind == "mos0" else None)
     h = ro3(kind)
@@ -71,6 +79,51 @@ def test_elaboration_matches_hand_flattening(kind):
         c = e.comps[k]
         assert c.type == typ
         assert list(c.node)[: 6] == nodes
+    if kind.startswith("bsim4_"):  # each device points at a model of its polarity and the default instance
+        for k, (typ, nodes, model, inst, *_r) in enumerate(h.comps):
+            if typ != A.S21_COMP_BSIM4:
+                continue
+            c = e.comps[k]
+            em, hm = e.net.bsim4_models[c.model], h.bsim4_models[model]
+            assert (em.set >> 1 & 1 and em.v[1] == 1.0) == (hm.set >> 1 & 1 and hm.v[1] == 1.0)
+            assert e.net.bsim4_insts[c.inst].set == 0
+
+
+def test_bsim4_definitions_decode_every_instance_field():
+    """Bsim4InstParams fields 1-31 arrive in their slots (UInt64Value ones as integers), the model's name is
+    field 900 and its polarity field 1 (protos/bsim4.proto:6-46)."""
+    ip = P.Bsim4InstParams(name="big")
+    fields = [f for f in P.Bsim4InstParams.DESCRIPTOR.fields if f.name != "name"]
+    assert [f.number for f in fields] == list(range(1, 32))
+    for f in fields:
+        getattr(ip, f.name).value = f.number if f.message_type.name == "UInt64Value" else f.number * 0.5e-6
+    c = P.Circuit(name="one", signals=["d"])
+    c.defs.append(P.Def(bsim4model=P.Bsim4Model(name="pm", mos_type=P.MosType.PMOS)))
+    c.defs.append(P.Def(bsim4inst=ip))
+    c.comps.append(mos("m1", "d", "d", "", "", "pm", "big"))
+    e = elaborate(c)
+    assert e.comps[0].type == A.S21_COMP_BSIM4 and list(e.comps[0].node)[:4] == [0, 0, -1, -1]
+    m, i = e.net.bsim4_models[0], e.net.bsim4_insts[0]
+    assert m.set == 1 << 1 and m.v[1] == 1.0
+    assert i.set == sum(1 << k for k in range(1, 32))
+    for f in fields:
+        want = f.number if f.message_type.name == "UInt64Value" else f.number * 0.5e-6
+        assert i.v[f.number] == want, f.name
+
+
+def test_bsim4_missing_parameters_and_precedence():
+    """A Mos naming a BSIM4 model with undefined parameters fails as the reference's unwrap does; a name
+    defined both as a BSIM4 and a Mos1 model elaborates as BSIM4 (elab.rs:143-147)."""
+    c = P.Circuit(name="one", signals=["d"])
+    c.defs.append(P.Def(bsim4model=P.Bsim4Model(name="m")))
+    c.defs.append(P.Def(mos1model=P.Mos1Model(name="m")))
+    c.defs.append(P.Def(mos1inst=P.Mos1InstParams(name="p")))
+    c.comps.append(mos("m1", "d", "d", "", "", "m", "p"))
+    with pytest.raises(S21Error, match="Parameters not defined: p"):
+        elaborate(c)
+    c.defs.append(P.Def(bsim4inst=P.Bsim4InstParams(name="p")))
+    e = elaborate(c)  # keep the owner of the netlist alive while reading it
+    assert e.comps[0].type == A.S21_COMP_BSIM4
 
 
 @pytest.mark.parametrize("fixture", sorted(GOLDEN_RUNS))

@@ -1,0 +1,193 @@
+"""BSIM4 in the oracle: the reference's three BSIM4 ring-oscillator waveforms (spice21/resources/
+test_bsim4_{cmos,nmos,pmos}_ro_tran.json, tests.rs:948-965, 1053-1071, 1360-1377) and its dcop known answers
+(comps/bsim4/tests.rs:56-166).
+
+The oracle's BSIM4 Component (matrix elements, load, commit, charge bookkeeping) is its own; the device
+functions (derivation, Bsim4::op, tran_op, stamp) are the product's host-and-device source compiled for the
+CPU.  These checks pin those functions: every sample of the three waveforms within 5e-14 of the reference's
+(1e-6 is the reference's own tolerance, assert.rs:124-143), bit for bit over the first 30 samples of each,
+and the known-answer currents.  The waveforms depart from the fixtures by one unit in the last place after
+30-112 bit-exact samples and stay there; the dcop points and the first transient steps, which exercise
+every stamp, the limiting, both charge-partition branches and the junctions, are identical.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from spice21_amd import _abi as A
+from spice21_amd.elab import elaborate
+from spice21_amd.netlist import Netlist
+from tests.circuits import BSIM4_GOLDEN_RUNS, ro3
+from tests.test_elab import ro3_proto
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+BIT_EXACT_PREFIX = 30
+
+
+def _check(X, names, g, fixture):
+    for k, nm in enumerate(names):
+        ref = np.array(g[nm])
+        assert X.shape[0] == len(ref)
+        err = np.max(np.abs(X[:, k] - ref))
+        assert err <= 1e-6, f"{fixture}:{nm} beyond the reference's tolerance"
+        assert err <= 5e-14, f"{fixture}:{nm} {err}"
+        assert np.array_equal(X[:BIT_EXACT_PREFIX, k], ref[:BIT_EXACT_PREFIX]), f"{fixture}:{nm}"
+
+
+@pytest.mark.parametrize("fixture", sorted(BSIM4_GOLDEN_RUNS))
+def test_oracle_bsim4_rings_match_golden(fixture):
+    kind, tstep, tstop = BSIM4_GOLDEN_RUNS[fixture]
+    g = json.load(open(os.path.join(GOLDEN, fixture)))
+    t, X, names, _ = O.tran(ro3(kind).to_c(), tstep, tstop, [("1", 0.0)])
+    assert sorted(names + ["time"]) == sorted(g.keys())
+    assert np.array_equal(t, np.array(g["time"]))
+    _check(X, names, g, fixture)
+
+
+@pytest.mark.parametrize("fixture", sorted(BSIM4_GOLDEN_RUNS))
+def test_elaborated_bsim4_rings_match_golden(fixture):
+    """The same rings as protobuf Modules with Bsim4Model / Bsim4InstParams definitions, through the
+    product's decoder and elaborator."""
+    kind, tstep, tstop = BSIM4_GOLDEN_RUNS[fixture]
+    g = json.load(open(os.path.join(GOLDEN, fixture)))
+    e = elaborate(ro3_proto(kind))
+    t, X, names, _ = O.tran(e, tstep, tstop, [("1", 0.0)])
+    _check(X, names, g, fixture)
+
+
+@pytest.mark.parametrize("plant", [6, 7])
+def test_planted_bsim4_mistakes_are_caught(plant):
+    """A skipped commit (stale charges) or a dropped bulk-row stamp moves the waveform far beyond 1e-6."""
+    kind, tstep, tstop = BSIM4_GOLDEN_RUNS["test_bsim4_pmos_ro_tran.json"]
+    g = json.load(open(os.path.join(GOLDEN, "test_bsim4_pmos_ro_tran.json")))
+    with O.planted(plant):
+        try:
+            t, X, names, _ = O.tran(ro3(kind).to_c(), tstep, tstop, [("1", 0.0)])
+        except O.OracleError:
+            return  # failing to converge is caught too
+    errs = [np.max(np.abs(X[:, k] - np.array(g[nm]))) for k, nm in enumerate(names)]
+    assert max(errs) > 1e-3
+
+
+def diode_connected(mos_type, v, **inst):
+    """test_bsim4_{nmos,pmos}_dcop1 (comps/bsim4/tests.rs:56-118): drain and gate on a source, source and
+    bulk grounded, a 1e-10 S resistor across."""
+    n = Netlist()
+    m, ip = n.bsim4_model(mos_type), n.bsim4_inst(**inst)
+    n.bsim4("bsim4", "gd", "gd", "", "", m, ip)
+    n.v("v1", v, "gd", "")
+    n.r(1e-10, "gd", "")
+    return n
+
+
+def test_bsim4_nmos_dcop_known_answer():
+    x, _, _ = O.dcop(diode_connected(0, 1.0).to_c())
+    assert x[0] == 1.0
+    assert abs(abs(x[1]) - 150e-6) <= 1e-6  # tests.rs:86
+
+
+def test_bsim4_pmos_dcop_known_answer():
+    x, _, _ = O.dcop(diode_connected(1, -1.0).to_c())
+    assert x[0] == -1.0
+    assert abs(abs(x[1]) - 57e-6) <= 1e-6  # tests.rs:115
+
+
+def inverter(vin):
+    n = Netlist()
+    nm, pm, ip = n.bsim4_model(0), n.bsim4_model(1), n.bsim4_inst()
+    n.bsim4("p", "d", "inp", "vdd", "vdd", pm, ip)
+    n.bsim4("n", "d", "inp", "", "", nm, ip)
+    n.v("vinp", vin, "inp", "")
+    n.v("vvdd", 1.0, "vdd", "")
+    return n
+
+
+def test_bsim4_inverter_dcop_known_answer():  # comps/bsim4/tests.rs:119-166
+    n = inverter(0.0)
+    x, _, _ = O.dcop(n.to_c())
+    v = dict(zip(n.names, x))
+    assert v["vdd"] == 1.0 and v["inp"] == 0.0
+    assert v["d"] > 0.95
+    assert abs(v["vinp"]) < 1e-6 and abs(v["vvdd"]) < 1e-6
+
+
+def test_bsim4_inverter_transfer_is_monotone_and_rail_to_rail():
+    outs = []
+    for vin in np.linspace(0.0, 1.0, 11):
+        n = inverter(float(vin))
+        x, _, _ = O.dcop(n.to_c())
+        outs.append(dict(zip(n.names, x))["d"])
+    assert outs[0] > 0.95 and outs[-1] < 0.05
+    assert all(b <= a + 1e-12 for a, b in zip(outs, outs[1:]))
+
+
+def bias(vd, vs, mos_type=0, **inst):
+    """One device with its gate at 1 V (-1 V for PMOS) and drain / source driven, bulk grounded."""
+    n = Netlist()
+    m, ip = n.bsim4_model(mos_type), n.bsim4_inst(**inst)
+    g = 1.0 if mos_type == 0 else -1.0
+    n.bsim4("m", "d", "g", "s", "", m, ip)
+    n.v("vg", g, "g", "")
+    n.v("vd", vd, "d", "")
+    n.v("vs", vs, "s", "")
+    x, _, _ = O.dcop(n.to_c())
+    return dict(zip(n.names, x))
+
+
+@pytest.mark.parametrize("mos_type", [0, 1])
+def test_bsim4_source_drain_exchange(mos_type):
+    """With the default, symmetric source/drain geometry, driving the source terminal instead of the drain
+    runs the reverse-mode branch (mode = -1: vgd for vgs, the exchanged charge partition and stamps) at the
+    same intrinsic bias: the drain and source currents exchange."""
+    s = 1.0 if mos_type == 0 else -1.0
+    fwd = bias(0.5 * s, 0.0, mos_type)
+    rev = bias(0.0, 0.5 * s, mos_type)
+    assert abs(fwd["vd"]) > 1e-6
+    assert rev["vs"] == pytest.approx(fwd["vd"], rel=1e-9, abs=1e-18)
+    assert rev["vd"] == pytest.approx(fwd["vs"], rel=1e-6, abs=1e-15)
+
+
+def test_bsim4_width_and_length_scale_the_channel_current():
+    """Id grows about in proportion to W and in inverse proportion to L in this long-channel device
+    (5 um default): a wrong slot for w or l, or a lost nf, fails these bounds."""
+    base = abs(bias(1.0, 0.0)["vd"])
+    wide = abs(bias(1.0, 0.0, w=10e-6)["vd"])
+    long_ = abs(bias(1.0, 0.0, l=10e-6)["vd"])
+    fingers = abs(bias(1.0, 0.0, w=10e-6, nf=2)["vd"])
+    assert 1.9 < wide / base < 2.1
+    assert 0.4 < long_ / base < 0.6
+    assert 0.95 < fingers / wide < 1.05
+
+
+def test_bsim4_explicit_defaults_change_nothing():
+    """Instance parameters given at their default values (bsim4inst.rs:103-118) reproduce the default device
+    bit for bit."""
+    a = bias(0.7, 0.0)
+    b = bias(0.7, 0.0, l=5e-6, w=5e-6, nf=1.0, sa=0.0, sb=0.0, sd=0.0, delvto=0.0, min=0)
+    assert a == b
+
+
+def test_bsim4_delvto_raises_the_threshold():
+    lo = abs(bias(1.0, 0.0)["vd"])
+    hi = abs(bias(1.0, 0.0, delvto=0.1)["vd"])
+    assert hi < 0.9 * lo
+
+
+def test_bsim4_nonpositive_length_is_an_error():
+    """bsim4inst.rs:200-203 panics on Leff <= 0; the oracle (like the engine) reports it.  (L = 0 itself
+    gives 0 / 0 in the length-dependence terms, a NaN Leff that passes the reference's test.)"""
+    with pytest.raises(O.OracleError, match="Effective channel length <= 0"):
+        bias(0.5, 0.0, l=-1e-6)
+
+
+def test_bsim4_netlist_slots():
+    n = Netlist()
+    ip = n.bsim4_inst(l=1e-6, as_=2e-12, min=1)
+    p = n.bsim4_insts[ip]
+    assert p.set == (1 << 1) | (1 << 12) | (1 << 17)
+    assert p.v[1] == 1e-6 and p.v[12] == 2e-12 and p.v[17] == 1.0
+    assert n.bsim4_models[n.bsim4_model(1)].v[1] == 1.0
+    assert A.S21_COMP_BSIM4 == 7

@@ -32,9 +32,11 @@ def seeds(d):
         "ckt_ring3.bin": ring,
         "ckt_hier_mos1.bin": ro3_proto("mos1"),
         "ckt_hier_mos0.bin": ro3_proto("mos0"),
+        "ckt_hier_bsim4.bin": ro3_proto("bsim4_cmos"),
         "ckt_diode.bin": diode,
         "op_ring.bin": P.Op(ckt=ring),
         "tran_ring.bin": P.Tran(ckt=ring, args=tr),
+        "tran_bsim4.bin": P.Tran(ckt=ro3_proto("bsim4_pmos"), args=P.TranOptions(tstep=1e-11, tstop=1e-10)),
         "ac_diode.bin": P.Ac(ckt=diode, args=P.AcOptions(fstart=1, fstop=1000, npts=10)),
     }
     paths = []
