@@ -53,8 +53,9 @@ enum {                                                      /* comps/mod.rs:60-7
     S21_COMP_MOS1 = 4,   /* Mos level 1                               comps/mos.rs:625-969 */
     S21_COMP_MOS0 = 5,   /* Mos level 0 (model = 0 NMOS / 1 PMOS)     comps/mos.rs:1007-1100 */
     S21_COMP_DIODE = 6,  /* Diode                                     comps/diode.rs:215-355 */
-    S21_COMP_BSIM4 = 7   /* BSIM4 (comps/bsim4/bsim4solver.rs:3757-3769), single-instance
-                            analyses only (the large-instance path and the byte ABI)      */
+    S21_COMP_BSIM4 = 7   /* BSIM4 (comps/bsim4/bsim4solver.rs:3757-3769): operating points and
+                            transients, in batches and on the single-instance path; no AC
+                            (the reference's BSIM4 has no load_ac)                        */
 };
 
 /* Optional-double parameter set keyed by proto field number (protos/mos.proto:22-76,
@@ -91,7 +92,7 @@ typedef struct {
     int32_t n_diode_models; const s21_params* diode_models;
     int32_t n_diode_insts;  const s21_params* diode_insts;
     /* Bsim4Model (protos/bsim4.proto:44-46: slot 1 = mos_type, 1 PMOS) and Bsim4InstParams keyed by
-     * field number (protos/bsim4.proto:6-41).  Only the polarity reaches the model
+     * field number (protos/bsim4.proto:6-41); s21_override field numbers on a BSIM4 start at 1.  Only the polarity reaches the model
      * (Bsim4ModelSpecs::from, comps/bsim4/model/mod.rs:21-30); every mode selector is at its default,
      * so a BSIM4 device has no internal nodes (bsim4ports.rs:31-112). */
     int32_t n_bsim4_models; const s21_params* bsim4_models;
@@ -193,6 +194,8 @@ S21_API int s21_batch_op(s21_batch* b);
  *     group.  x's contents are ignored: the solve starts from x = 0 as the reference's does.
  *   - The Newton state lives in a fixed 64 MiB device array of the library (one per GPU, kept until
  *     the process ends, cleared by each call); a batch whose state needs more returns S21_INVALID.
+ *     BSIM4 devices keep their points in the batch's own buffers (allocated by the preparing
+ *     s21_batch_op), which each call clears.
  *   - An instance whose frozen pivot order meets a pivot it cannot keep (zero, or below the smallest
  *     normal number of the factors' precision) reports S21_PIVOT (s21_batch_op resumes such instances
  *     on a new order in a launch of its own, which allocates).
@@ -262,7 +265,8 @@ S21_API int s21_batch_structure(const s21_batch* b, int32_t group, int64_t* nnz_
  * runs a factorisation generated and compiled at run time for the groups' frozen patterns (1 = always,
  * -1 = never, 0 = for a batch of at least 512 instances, or an AC sweep of at least 100,000
  * instance-frequencies).  s21_batch_config reports the last launch: instances per wavefront,
- * wavefronts per workgroup, generated factorisation used. */
+ * wavefronts per workgroup, generated factorisation used.  A batch holding BSIM4 devices always runs the
+ * precompiled record-driven factorisation in a kernel that evaluates them (the generated kernels do not). */
 S21_API int s21_batch_set_ipw(s21_batch* b, int32_t ipw);
 S21_API int s21_batch_set_jit(s21_batch* b, int32_t mode);
 /* Launch-shape and structure knobs the autotuner searches (no counterpart in the reference, whose
@@ -287,7 +291,8 @@ S21_API int s21_batch_set_ac_runs(s21_batch* b, int32_t runs);
 S21_API int32_t s21_batch_last_ac_runs(const s21_batch* b);
 /* Per-phase device cycle counters (s_memtime) of the Newton kernel, for profiling: while on, each
  * instance accumulates [device evaluation, hub gather, residual, factor (+ forward sweep), backward
- * sweep, update, convergence/commit, total] cycles. */
+ * sweep, update, convergence/commit, total] cycles.  Not for batches holding BSIM4 devices (the
+ * profiling kernels do not evaluate them: their launches return S21_INVALID). */
 S21_API int s21_batch_set_profiling(s21_batch* b, int32_t on);
 S21_API int s21_batch_profile(const s21_batch* b, int32_t group, int64_t* out /* [n_inst][8] */);
 /* One assembly pass on the GPU (device evaluation + ordered gather) for instance `inst` at the

@@ -32,6 +32,8 @@ def _field(target, comp_type, name):
         (A.S21_OVR_INST, A.S21_COMP_MOS1): A.MOS1_INST_FIELDS,
         (A.S21_OVR_MODEL, A.S21_COMP_DIODE): A.DIODE_MODEL_FIELDS,
         (A.S21_OVR_INST, A.S21_COMP_DIODE): A.DIODE_INST_FIELDS,
+        (A.S21_OVR_MODEL, A.S21_COMP_BSIM4): A.BSIM4_MODEL_FIELDS,
+        (A.S21_OVR_INST, A.S21_COMP_BSIM4): A.BSIM4_INST_FIELDS,
     }[(target, comp_type)]
     return A._field(table, name)
 

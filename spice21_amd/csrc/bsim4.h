@@ -14,7 +14,9 @@
 // bsim4derive.rs, bsim4inst.rs), so that its results are reproduced to the last bits.
 #pragma once
 
-#if defined(__HIPCC__)
+#if defined(__HIPCC_RTC__)
+// run-time compilation provides the HIP declarations and the math functions itself
+#elif defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #else  // a plain host build (the CPU check compiles these functions with g++)
 #ifndef __host__
@@ -25,10 +27,12 @@
 #endif
 #endif
 
+#ifndef __HIPCC_RTC__
 #include <cmath>
 #include <stdexcept>
+#endif
 
-#include "../../include/spice21_mi355x.h"
+#include "spice21_mi355x.h"
 
 namespace s21 {
 namespace b4 {

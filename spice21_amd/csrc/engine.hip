@@ -64,12 +64,13 @@ struct GroupHost {
     std::vector<std::string> names;
     std::vector<int> kinds;
     std::vector<s21_comp> comps;
-    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts;
+    std::vector<s21_params> mos1_models, mos1_insts, diode_models, diode_insts, bsim4_models, bsim4_insts;
     int n_inst = 0;
     std::vector<Override> overs;
     // the last analysis
     Topology topo;
     std::unique_ptr<DevMem> d_topo, d_par, d_state, d_out;
+    std::unique_ptr<DevMem> d_b4;  // BSIM4 parameters and states
     std::unique_ptr<DevMem> d_acop;  // Mos1 small-signal operating points captured for AC
     std::unique_ptr<DevMem> d_ac;    // AC: per-frequency records, runs' last iterates, runs' outcomes
     GroupGPU gg{};
@@ -99,6 +100,7 @@ struct s21_batch {
     int64_t resumed = 0;  // instance resumptions of the last analysis (launches of recover_pivots)
     int64_t npts = 0;
     int profile = 0;
+    bool has_b4 = false;  // some group holds BSIM4 devices: the kernels that evaluate them
     int last_waves = 1, last_ipw = 1;
     int ipw = 0;  // instances per wavefront; 0 = chosen per launch
     int jit = 0;  // generated factorisation: 0 = for large single-pattern batches, 1 = always, -1 = never
@@ -146,8 +148,18 @@ static void build_comps(GroupHost& g, const s21_tran_options* tr, Topology& t) {
 // kernel's device order.
 struct HostPrep {
     std::vector<double> m1_core, m1_caps, d_par, lin_val, lin_tran, lin_ac;
-    int n_m1 = 0, n_d = 0, n_lin = 0;
+    std::vector<b4::B4Par> b4;  // [n_b4], or [n_inst][n_b4] when an instance overrides a BSIM4 parameter
+    bool b4_per_inst = false;
+    int n_m1 = 0, n_d = 0, n_lin = 0, n_b4 = 0;
 };
+// BSIM4 parameters of one device; the reference panics where derivation throws (bsim4inst.rs:200-223).
+static b4::B4Par derive_b4(const s21_params& mp, const s21_params& ip) {
+    try {
+        return b4::derive_params(mp, ip);
+    } catch (const std::domain_error& e) {
+        fail(S21_ERR, e.what());
+    }
+}
 
 // Inputs of an AC structure: the first angular frequency and each instance's Mos1 operating point
 // captured after the DC solution, [9][n_inst][n_dc] in that solution's device order (dc_comp: the
@@ -161,12 +173,13 @@ struct AcInput {
 // Parameter sets of component `ci` for instance i with the group's overrides applied.
 static void inst_params(const GroupHost& g, const std::vector<const Override*>* ov, const Comp& c, int i,
                         s21_params& mp, s21_params& ip) {
-    const bool mos = c.type == S21_COMP_MOS1;
-    mp = mos ? g.mos1_models[c.model] : g.diode_models[c.model];
-    ip = mos ? g.mos1_insts[c.inst] : g.diode_insts[c.inst];
+    const bool mos = c.type == S21_COMP_MOS1, b4 = c.type == S21_COMP_BSIM4;
+    mp = mos ? g.mos1_models[c.model] : b4 ? g.bsim4_models[c.model] : g.diode_models[c.model];
+    ip = mos ? g.mos1_insts[c.inst] : b4 ? g.bsim4_insts[c.inst] : g.diode_insts[c.inst];
     if (!ov) return;
     for (const Override* x : *ov) {
-        if (x->target == S21_OVR_VALUE) fail(S21_INVALID, mos ? "value override on a Mos1" : "value override on a Diode");
+        if (x->target == S21_OVR_VALUE)
+            fail(S21_INVALID, mos ? "value override on a Mos1" : b4 ? "value override on a BSIM4" : "value override on a Diode");
         s21_params& tp = x->target == S21_OVR_MODEL ? mp : ip;
         tp.v[x->field] = x->vals[i];
         tp.set |= 1ull << x->field;
@@ -198,6 +211,12 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
             inst_params(g, ov_of((int)i), c, 0, mp, ip);
             h0.m1.push_back(derive_mos1(mp, ip, o));
             has_caps.push_back(h0.m1.back().has_caps ? 1 : 0);
+        } else if (c.type == S21_COMP_BSIM4) {
+            if (c.model < 0 || c.model >= (int)g.bsim4_models.size() || c.inst < 0 ||
+                c.inst >= (int)g.bsim4_insts.size())
+                fail(S21_ERR, "Parameters not defined");
+            inst_params(g, ov_of((int)i), c, 0, mp, ip);
+            h0.b4.push_back(derive_b4(mp, ip));
         } else if (c.type == S21_COMP_DIODE) {
             inst_params(g, ov_of((int)i), c, 0, mp, ip);
             h0.dio.push_back(derive_diode(mp, ip, o));
@@ -329,6 +348,26 @@ static void host_prepare(const s21_options& o, GroupHost& g, const s21_tran_opti
                                        P.cz2,  P.dep, P.f1,    P.f3,   P.tt, P.vj,     P.m};
             const size_t off = (size_t)i * n_d + d;
             for (int k = 0; k < D_PAR_F; k++) hp.d_par[k * s + off] = f[k];
+        }
+    }
+    // BSIM4: whole parameter structures (array of structures), per instance only when overridden
+    const int n_b4 = t.cls_start[CLS_BSIM4 + 1] - t.cls_start[CLS_BSIM4];
+    hp.n_b4 = n_b4;
+    hp.b4_per_inst = false;
+    for (int d = 0; d < n_b4; d++) hp.b4_per_inst = hp.b4_per_inst || ov_of(t.dev_comp[t.cls_start[CLS_BSIM4] + d]);
+    hp.b4.resize((size_t)(hp.b4_per_inst ? NI : 1) * n_b4);
+    for (int d = 0; d < n_b4; d++) {
+        const int ci = t.dev_comp[t.cls_start[CLS_BSIM4] + d];
+        const Comp& c = t.comps[ci];
+        const auto* L = ov_of(ci);
+        for (int i = 0; i < (hp.b4_per_inst ? NI : 1); i++) {
+            if (i == 0 || L) {
+                s21_params mp, ip;
+                inst_params(g, L, c, i, mp, ip);
+                hp.b4[(size_t)i * n_b4 + d] = derive_b4(mp, ip);
+            } else {
+                hp.b4[(size_t)i * n_b4 + d] = hp.b4[d];
+            }
         }
     }
     hp.lin_val.assign((size_t)NI * n_lin, 0.0);
@@ -491,6 +530,25 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     if (!par.empty()) HIPCHECK(hipMemcpy(g.d_par->p, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
     g.gg.par = (const double*)g.d_par->p;
 
+    // ---- BSIM4: parameters, then the last load's and the committed points and the limiting fields ----
+    {
+        const size_t nb = (size_t)hp.n_b4;
+        const size_t s_par = ((hp.b4.size() * sizeof(b4::B4Par) + 255) / 256) * 256,
+                     s_op = ((NI * nb * sizeof(b4::B4Op) + 255) / 256) * 256, s_prev = pad(5 * NI * nb);
+        g.d_b4.reset(new DevMem());
+        g.d_b4->alloc(nb ? s_par + 2 * s_op + s_prev : 0);
+        char* p = (char*)g.d_b4->p;
+        if (nb) {
+            HIPCHECK(hipMemcpy(p, hp.b4.data(), hp.b4.size() * sizeof(b4::B4Par), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemsetAsync(p + s_par, 0, 2 * s_op + s_prev, b->stream));
+        }
+        g.gg.b4_par = nb ? (const b4::B4Par*)p : nullptr;
+        g.gg.b4_istr = hp.b4_per_inst ? (int)nb : 0;
+        g.gg.b4_guess = nb ? (b4::B4Op*)(p + s_par) : nullptr;
+        g.gg.b4_op = nb ? (b4::B4Op*)(p + s_par + s_op) : nullptr;
+        g.gg.b4_prev = nb ? (double*)(p + s_par + 2 * s_op) : nullptr;
+    }
+
     // ---- state (zero-initialised: Default op points, comps/mos.rs:524-546) ----
     const size_t s_m1s = pad((size_t)M1_STATE_F * 2 * NI * hp.n_m1), s_ds = pad((size_t)D_STATE_F * NI * hp.n_d),
                  s_cq = pad((size_t)NI * hp.n_lin);
@@ -549,6 +607,7 @@ static void prepare_jit(s21_batch* b, bool ac = false, int64_t n_freq = 0) {
     int64_t total = 0, ops = 0;
     for (const GroupHost& g : b->groups) total += g.n_inst;
     if (b->jit < 0 || (b->jit == 0 && (ac ? total * n_freq < JIT_MIN_AC_POINTS : total < JIT_MIN_INST))) return;
+    if (b->has_b4) return;  // the generated kernels carry no BSIM4 evaluation: record-driven factorisation
     std::vector<const Topology*> pats;
     std::vector<int> group_pattern;
     std::unordered_map<std::string, int> seen;  // pattern -> index, keyed by what its code is written from
@@ -580,6 +639,10 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
         if (g.prepared == key) {
             HIPCHECK(hipMemsetAsync(g.d_state->p, 0, g.d_state->bytes, b->stream));
             HIPCHECK(hipMemsetAsync(g.d_out->p, 0, g.d_out->bytes, b->stream));
+            if (g.gg.b4_guess) {  // the parameters stay; every state after them restarts from zero
+                char* p0 = (char*)g.gg.b4_guess;
+                HIPCHECK(hipMemsetAsync(p0, 0, g.d_b4->bytes - (size_t)(p0 - (char*)g.d_b4->p), b->stream));
+            }
         } else {
             g.prepared.clear();
             prepare_group(b, g, tr);
@@ -594,7 +657,8 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
 
 template <bool MIXED>
 static void launch(s21_batch* b, const GroupGPU* dgroups, int n_groups, int n_blocks, size_t smem, const Args& A) {
-    auto kern = A.prof ? newton_kernel_prof<MIXED> : newton_kernel<MIXED>;
+    if (b->has_b4 && A.prof) fail(S21_INVALID, "the profiling kernels do not evaluate BSIM4 devices");
+    auto kern = b->has_b4 ? newton_kernel_b4<MIXED> : A.prof ? newton_kernel_prof<MIXED> : newton_kernel<MIXED>;
     if (smem > 64 * 1024)
         HIPCHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(WAVE * A.waves), smem, b->stream, dgroups, n_groups, A);
@@ -802,9 +866,8 @@ static void group_from(GroupHost& g, const s21_group& sg) {
         g.kinds.push_back(nl.var_kind ? nl.var_kind[k] : S21_VAR_V);
     }
     g.comps.assign(nl.comps, nl.comps + nl.n_comps);
-    for (const s21_comp& c : g.comps)
-        if (c.type == S21_COMP_BSIM4)
-            fail(S21_INVALID, "BSIM4 devices run on the single-instance path (s21_large_*), not in a batch");
+    if (nl.n_bsim4_models) g.bsim4_models.assign(nl.bsim4_models, nl.bsim4_models + nl.n_bsim4_models);
+    if (nl.n_bsim4_insts) g.bsim4_insts.assign(nl.bsim4_insts, nl.bsim4_insts + nl.n_bsim4_insts);
     if (nl.n_mos1_models) g.mos1_models.assign(nl.mos1_models, nl.mos1_models + nl.n_mos1_models);
     if (nl.n_mos1_insts) g.mos1_insts.assign(nl.mos1_insts, nl.mos1_insts + nl.n_mos1_insts);
     if (nl.n_diode_models) g.diode_models.assign(nl.diode_models, nl.diode_models + nl.n_diode_models);
@@ -813,7 +876,9 @@ static void group_from(GroupHost& g, const s21_group& sg) {
     for (int k = 0; k < sg.n_over; k++) {
         const s21_override& o = sg.over[k];
         if (o.comp < 0 || o.comp >= nl.n_comps || !o.vals) fail(S21_INVALID, "override: bad component");
-        if (o.target != S21_OVR_VALUE && (o.field < 2 || o.field >= S21_PARAM_SLOTS))
+        // BSIM4 parameter sets start at field 1 (Bsim4Model.mos_type, Bsim4InstParams.l)
+        const int f0 = nl.comps[o.comp].type == S21_COMP_BSIM4 ? 1 : 2;
+        if (o.target != S21_OVR_VALUE && (o.field < f0 || o.field >= S21_PARAM_SLOTS))
             fail(S21_INVALID, "override: bad field");
         Override x;
         x.comp = o.comp;
@@ -840,7 +905,11 @@ constexpr int MAX_RESUME = 8;
 struct InstState {
     std::vector<std::vector<double>> m1, d;  // per Mos1 / diode component: its state fields
     std::vector<double> q;                    // per linear component: committed capacitor charge
+    // per BSIM4 component: the limiting fields its last load started from (vds, vgs, vbs, vbd, von), then
+    // its committed point (the B4Op's doubles)
+    std::vector<std::vector<double>> b4;
 };
+constexpr int B4_PREV_F = 5, B4_OP_D = (int)(sizeof(b4::B4Op) / sizeof(double));
 static void dev_maps(const Topology& t, std::vector<int>& m1, std::vector<int>& d, std::vector<int>& lin) {
     m1.assign(t.dev_comp.begin() + t.cls_start[CLS_MOS1], t.dev_comp.begin() + t.cls_start[CLS_MOS1T + 1]);
     d.assign(t.dev_comp.begin() + t.cls_start[CLS_DIODE], t.dev_comp.begin() + t.cls_start[CLS_DIODE + 1]);
@@ -856,6 +925,21 @@ static InstState read_state(const GroupHost& g, int i) {
     S.m1.assign(ncomp, {});
     S.d.assign(ncomp, {});
     S.q.assign(ncomp, 0.0);
+    S.b4.assign(ncomp, {});
+    const Topology& t = g.topo;
+    const size_t nb = (size_t)(t.cls_start[CLS_BSIM4 + 1] - t.cls_start[CLS_BSIM4]);
+    if (nb) {
+        std::vector<double> prev((size_t)B4_PREV_F * NI * nb);
+        std::vector<b4::B4Op> op(nb);
+        HIPCHECK(hipMemcpy(prev.data(), g.gg.b4_prev, prev.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(op.data(), g.gg.b4_op + (size_t)i * nb, nb * sizeof(b4::B4Op), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nb; k++) {
+            std::vector<double>& v = S.b4[t.dev_comp[t.cls_start[CLS_BSIM4] + k]];
+            for (int f = 0; f < B4_PREV_F; f++) v.push_back(prev[f * NI * nb + i * nb + k]);
+            const double* od = (const double*)&op[k];
+            v.insert(v.end(), od, od + B4_OP_D);
+        }
+    }
     std::vector<double> tmp;
     auto rd = [&](const double* base, size_t nf, size_t ndev, std::vector<std::vector<double>>& out,
                   const std::vector<int>& comp) {
@@ -894,6 +978,27 @@ static void write_state(GroupHost& h, const InstState& S) {
     };
     wr(h.gg.m1_state, 2 * M1_STATE_F, nm, S.m1, m1);
     wr(h.gg.d_state, D_STATE_F, nd, S.d, d);
+    const Topology& t = h.topo;
+    const size_t nb = (size_t)(t.cls_start[CLS_BSIM4 + 1] - t.cls_start[CLS_BSIM4]);
+    if (nb) {  // the load resumes from the limiting fields it started from, against the committed point
+        std::vector<double> prev((size_t)B4_PREV_F * nb, 0.0);
+        std::vector<b4::B4Op> guess(nb), op(nb);
+        for (size_t k = 0; k < nb; k++) {
+            const std::vector<double>& v = S.b4[t.dev_comp[t.cls_start[CLS_BSIM4] + k]];
+            if (v.size() != (size_t)(B4_PREV_F + B4_OP_D)) fail(S21_ERR, "internal: BSIM4 state of the wrong size");
+            memset(&guess[k], 0, sizeof(b4::B4Op));
+            guess[k].vds = v[0];
+            guess[k].vgs = v[1];
+            guess[k].vbs = v[2];
+            guess[k].vbd = v[3];
+            guess[k].von = v[4];
+            for (int f = 0; f < B4_PREV_F; f++) prev[f * nb + k] = v[f];
+            memcpy(&op[k], v.data() + B4_PREV_F, sizeof(b4::B4Op));
+        }
+        HIPCHECK(hipMemcpy(h.gg.b4_guess, guess.data(), nb * sizeof(b4::B4Op), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(h.gg.b4_op, op.data(), nb * sizeof(b4::B4Op), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(h.gg.b4_prev, prev.data(), prev.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (nl) {
         tmp.assign(nl, 0.0);
         for (size_t k = 0; k < nl; k++) tmp[k] = S.q[lin[k]];
@@ -956,6 +1061,7 @@ static void recover_pivots(s21_batch* b, const s21_tran_options* tr, int64_t npt
         s21_batch sb;
         sb.opts = b->opts;
         sb.precision = S21_FP64;
+        sb.has_b4 = b->has_b4;
         sb.ipw = 1;
         sb.jit = -1;
         HIPCHECK(hipStreamCreateWithFlags(&sb.stream, hipStreamNonBlocking));
@@ -974,6 +1080,8 @@ static void recover_pivots(s21_batch* b, const s21_tran_options* tr, int64_t npt
             h.mos1_insts = g.mos1_insts;
             h.diode_models = g.diode_models;
             h.diode_insts = g.diode_insts;
+            h.bsim4_models = g.bsim4_models;
+            h.bsim4_insts = g.bsim4_insts;
             h.n_inst = 1;
             for (const Override& o : g.overs) {
                 Override x = o;
@@ -1232,6 +1340,8 @@ int s21_batch_create(int32_t n_groups, const s21_group* groups, const s21_option
         b->precision = precision == S21_MIXED ? S21_MIXED : S21_FP64;
         b->groups.resize(n_groups);
         for (int gi = 0; gi < n_groups; gi++) group_from(b->groups[gi], groups[gi]);
+        for (const GroupHost& g : b->groups)
+            for (const s21_comp& c : g.comps) b->has_b4 = b->has_b4 || c.type == S21_COMP_BSIM4;
         HIPCHECK(hipGetDevice(&b->device));
         HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&b->ev0));
@@ -1604,6 +1714,11 @@ int s21_batch_op_noalloc(s21_batch* b, double* x, int32_t* status, int64_t* iter
         // Default operating points (zeros) and the reference's start x = 0: the array keeps the last
         // call's contents, and the caller's buffers hold whatever they held.
         zero_async(base + st_off[0], off - st_off[0], b->stream);
+        for (const GroupHost& g : b->groups)  // BSIM4 points stay in the batch's own buffer (no allocation)
+            if (g.gg.b4_guess) {
+                char* p0 = (char*)g.gg.b4_guess;
+                zero_async(p0, g.d_b4->bytes - (size_t)(p0 - (char*)g.d_b4->p), b->stream);
+            }
         zero_async(x, xo * sizeof(double), b->stream);
         zero_async(status, io * sizeof(int32_t), b->stream);
         zero_async(iters, io * sizeof(int64_t), b->stream);

@@ -317,6 +317,7 @@ std::string jit_source(const std::vector<const Topology*>& ts, const std::vector
          "    static constexpr int kind = "
       << (complex ? 2 : 1) << ";\n"
          "    static constexpr bool profile = false;\n"
+         "    static constexpr bool bsim4 = false;\n"
          "    template <typename V>\n"
          "    static __device__ bool factor_lane(int g, LDS V* lu, LDS V* c) {\n"
       << dispatch("return ", "lu_gen")

@@ -26,10 +26,12 @@
 #include <hip/hip_runtime.h>
 #endif
 
+#include "bsim4_eval.h"
 #include "layout.h"
 #include "spice21_mi355x.h"
 
 namespace s21 {
+static_assert(b4::B4_NS == BSIM4_NE, "BSIM4 code stride");
 
 // ============================================================================================
 // Device-side descriptors
@@ -81,6 +83,15 @@ struct GroupGPU {
     double* ac_end;         // AC: [n_inst][ac_runs][n] complex: each run's last iterate (first pass)
     int* ac_state;          // AC: [n_inst][ac_runs] outcome of each run's re-solve from the previous
                             // run's end (second pass); entry 0: run to stitch from in order (third pass)
+    // BSIM4 (class-local device j of instance i): parameters b4_par[i * b4_istr + j] (b4_istr 0 when no
+    // instance overrides a BSIM4 parameter); the last load's point and the committed one [n_inst][n_b4];
+    // the limiting fields the last load started from [5][n_inst][n_b4] (vds, vgs, vbs, vbd, von), to
+    // resume an interrupted solve at that load
+    const b4::B4Par* b4_par;
+    int b4_istr;
+    b4::B4Op* b4_guess;
+    b4::B4Op* b4_op;
+    double* b4_prev;
 };
 
 enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2, MODE_AC = 3 };
@@ -357,6 +368,36 @@ __device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS con
     st[13 * s + o] = gs.reversed;
 }
 
+// BSIM4 (bsim4solver.rs:134-145): Bsim4::op from the terminal voltages with limiting against the last
+// load's point, tran_op's companions against the committed point in a transient, the 42 stamps.  The new
+// point is written to global memory as it is computed (its 86 fields never occupy registers together).
+__device__ inline void eval_b4(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
+    const GroupGPU& G = *C.G;
+    const int nb = C.T.n_cls[CLS_BSIM4];
+    const size_t s = (size_t)G.n_inst * nb, o = (size_t)C.inst * nb + j;
+    const b4::B4Par& P = G.b4_par[(size_t)C.inst * G.b4_istr + j];
+    b4::B4Op& N = G.b4_guess[o];
+    b4::B4Op Gl;
+    Gl.vds = N.vds;
+    Gl.vgs = N.vgs;
+    Gl.vbs = N.vbs;
+    Gl.vbd = N.vbd;
+    Gl.von = N.von;
+    GLB double* pv = glb(G.b4_prev);
+    pv[0 * s + o] = Gl.vds;
+    pv[1 * s + o] = Gl.vgs;
+    pv[2 * s + o] = Gl.vbs;
+    pv[3 * s + o] = Gl.vbd;
+    pv[4 * s + o] = Gl.von;
+    b4::op_eval(P, Gl, xv(C.x, (int)dv[0] - 1), xv(C.x, (int)dv[1] - 1), xv(C.x, (int)dv[2] - 1),
+                xv(C.x, (int)dv[3] - 1), A.gmin, N);
+    b4::B4Tran T{};
+    if (tran) b4::tran_eval(P, G.b4_op[o], N, A.tstep, T);
+    Stamps<BSIM4_NE> sv;
+    b4::stamp(P, N, T, sv);
+    apply<BSIM4_NE, false>(C, code, sv.v, false);
+}
+
 __device__ inline void eval_diode(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran,
                                   const Args& A) {
     const GroupGPU& G = *C.G;
@@ -433,6 +474,8 @@ __device__ inline long long cycles() { return (long long)__builtin_readcyclecoun
 // Assembly of A and the RHS at the lane segment's x: colours in order, then the hub gather.  With
 // profiling on (prof, uniform), pg[0] gets the device evaluation's cycles and pg[1] the hub gather's; pg
 // is the instance's counters in global memory, given to one lane of the instance (nullptr elsewhere).
+// B4: the kernel evaluates BSIM4 devices (compiled into the kernels of batches that hold them only).
+template <bool B4 = false>
 __device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A, bool prof = false,
                                 long long* pg = nullptr) {
     const long long t0 = prof ? cycles() : 0;
@@ -457,8 +500,14 @@ __device__ inline void assemble(const Ctx& C, bool act, bool tran, const Args& A
                           xv(C.x, (int)dv[RS] - 1), A.gmin, Put{C.a, C.hub, code});
             else if (cls == CLS_DIODE)
                 eval_diode(C, j, dv, code, tran, A);
-            else
+            else if constexpr (B4) {
+                if (cls == CLS_LIN)
+                    eval_lin(C, j, dv, code, tran, A);
+                else
+                    eval_b4(C, j, dv, code, tran, A);
+            } else {
                 eval_lin(C, j, dv, code, tran, A);
+            }
         }
         wave_sync();
     }
@@ -604,6 +653,7 @@ __device__ inline void backward(const Ctx& C, bool act, LDS const V* lu) {
 }
 
 // Commit the accepted operating point (Component::commit) of every stateful device of the segment.
+template <bool B4 = false>
 __device__ inline void commit(const Ctx& C, bool act) {
     const GroupGPU& G = *C.G;
     const TopoGPU& T = C.T;
@@ -626,6 +676,13 @@ __device__ inline void commit(const Ctx& C, bool act) {
         const size_t s = (size_t)G.n_inst * nm, o = (size_t)C.inst * nm + d;
         for (int f = 0; f < M1_STATE_F; f++) glb(G.m1_state)[f * s + o] = glb(G.m1_state)[(M1_STATE_F + f) * s + o];
     }
+    if constexpr (B4) {
+        const int nb = T.n_cls[CLS_BSIM4];
+        for (int d = act ? C.sl : nb; d < nb; d += C.L) {
+            const size_t o = (size_t)C.inst * nb + d;
+            G.b4_op[o] = G.b4_guess[o];
+        }
+    }
 }
 
 // One Newton solve (Solver::solve, analysis.rs:169-210) of every active segment of the wavefront.
@@ -643,6 +700,8 @@ struct LuLevels {
     // Whether the kernel carries the phase-cycle counters (Profiled<>); without them no profiling code is
     // compiled in.
     static constexpr bool profile = false;
+    // Whether the kernel evaluates BSIM4 devices (WithBsim4<>).
+    static constexpr bool bsim4 = false;
     template <typename V>
     static __device__ bool factor(const Ctx& C, bool act, LDS V* lu) {
         return factor_forward<V>(C, act, lu);
@@ -672,7 +731,7 @@ __device__ SolveOut newton(const Ctx& C, bool act, bool tran, const Args& A, lon
         if (it >= it_left) act = false;  // the segment's share of the iteration limit is used up
         if (!wave_any(act)) break;
         if (act) loads++;
-        assemble(C, act, tran, A, prof, pg);
+        assemble<LU::bsim4>(C, act, tran, A, prof, pg);
         wave_sync();
         if (prof) t0 = cycles();
         const bool bad = seg_any(C, residual(C, act, A.iabstol));
@@ -680,7 +739,7 @@ __device__ SolveOut newton(const Ctx& C, bool act, bool tran, const Args& A, lon
         if (prof) { t1 = cycles(); if (pg) pg[2] += t1 - t0; t0 = t1; }
         const bool conv = act && prev_ok && !bad;
         if (wave_any(conv)) {
-            commit(C, conv);
+            commit<LU::bsim4>(C, conv);
             wave_sync();
         }
         if (conv) {
@@ -906,7 +965,8 @@ __device__ inline void eval_ac_lin(const Ctx& C, const AcCtx& Z_, int j, LDS con
 }
 
 // Complex A|RHS at angular frequency omega (Solver<Complex>::update, analysis.rs:237-251).
-__device__ inline void assemble_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega) {
+__device__ __attribute__((always_inline)) inline void assemble_ac(const Ctx& C, const AcCtx& Z_, bool act,
+                                                                   double omega) {
     const TopoGPU& T = C.T;
     const Idx& I = C.I;
     const int ns = T.nnz + T.n;
@@ -1054,8 +1114,9 @@ __device__ inline void backward_ac(const Ctx& C, const AcCtx& Z_, bool act, LDS 
 // and factors at every iteration; the AC matrix and right-hand side do not depend on x, so every
 // iteration would assemble the same values and factor them into the same LU: the first iteration does
 // both (the factors into a copy of A, which the residual keeps reading) and later iterations reuse them.
+// Inlined by force: a call would bring the function-call convention's private stack into the kernels.
 template <bool MIXED>
-__device__ SolveOut newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega) {
+__device__ __attribute__((always_inline)) SolveOut newton_ac(const Ctx& C, const AcCtx& Z_, bool act, double omega) {
     const TopoGPU& T = C.T;
     int st = act ? RUNNING : S21_OK;
     int loads = 0;
@@ -1222,14 +1283,14 @@ __device__ SolveOut newton_wg(const Ctx& C, bool act, bool tran, const Args& A, 
         if (it >= it_left) act = false;
         if (!__syncthreads_or(act)) break;
         if (act) loads++;
-        assemble(C, act, tran, A, prof, pg);
+        assemble<LU::bsim4>(C, act, tran, A, prof, pg);
         wave_sync();
         if (prof) t0 = cycles();
         const bool bad = seg_any(C, residual(C, act, A.iabstol));
         wave_sync();
         const bool conv = act && prev_ok && !bad;
         if (wave_any(conv)) {
-            commit(C, conv);
+            commit<LU::bsim4>(C, conv);
             wave_sync();
         }
         if (conv) {
@@ -1530,7 +1591,7 @@ __device__ inline void op_tran(Ctx& C, const Args& A, int g, int slot, bool vali
     wave_sync();
 
     if (A.mode == MODE_ASSEMBLE) {
-        assemble(C, valid, A.asm_tran != 0, A);
+        assemble<LU::bsim4>(C, valid, A.asm_tran != 0, A);
         wave_sync();
         for (int s = valid ? C.sl : T.nnz + T.n; s < T.nnz + T.n; s += C.L) G.dbg[s] = C.a[s];
         return;
@@ -1662,7 +1723,7 @@ __device__ inline void newton_body(const GroupGPU* __restrict__ groups, int n_gr
     C.I.code = b16 + T.o_code;
     LDS char* region0 = (LDS char*)((LDS uint4*)smem_u4 + T.blob_vec);
     if (A.mode == MODE_AC) {
-        if constexpr (LU::kind != 1)
+        if constexpr (LU::kind != 1 && !LU::bsim4)  // BSIM4 has no AC stamps (no load_ac in the reference)
             ac_sweep<MIXED, LU>(C, A, g, valid ? (b - G.blk0) * A.waves * A.ipw + slot - inst * ac_units(A) : 0, slot,
                                 valid, region0);
     } else {
@@ -1675,6 +1736,11 @@ template <class LU>
 struct Profiled : LU {
     static constexpr bool profile = true;
 };
+// An LU policy whose kernel evaluates BSIM4 devices (batches that hold them).
+template <class LU>
+struct WithBsim4 : LU {
+    static constexpr bool bsim4 = true;
+};
 
 // Precompiled kernel: the level-scheduled factorisation of the pattern's records.
 template <bool MIXED>
@@ -1686,6 +1752,11 @@ template <bool MIXED>
 __global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel_prof(const GroupGPU* __restrict__ groups,
                                                                        int n_groups, Args A) {
     newton_body<MIXED, Profiled<LuLevels>>(groups, n_groups, A);
+}
+template <bool MIXED>
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void newton_kernel_b4(const GroupGPU* __restrict__ groups,
+                                                                     int n_groups, Args A) {
+    newton_body<MIXED, WithBsim4<LuLevels>>(groups, n_groups, A);
 }
 
 }  // namespace s21

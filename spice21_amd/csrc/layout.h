@@ -6,7 +6,9 @@ namespace s21 {
 
 // Device classes in kernel order.  CLS_MOS1T is a Mos1 without terminal resistances whose source is tied
 // to its bulk: its coinciding entries are merged in registers before they reach the matrix.
-enum { CLS_MOS1 = 0, CLS_MOS1T = 1, CLS_MOS0 = 2, CLS_DIODE = 3, CLS_LIN = 4, N_CLS = 5 };
+enum { CLS_MOS1 = 0, CLS_MOS1T = 1, CLS_MOS0 = 2, CLS_DIODE = 3, CLS_LIN = 4, CLS_BSIM4 = 5, N_CLS = 6 };
+// BSIM4 stamp entries (bsim4_eval.h: 38 matrix stamps, then 4 right-hand-side stamps).
+constexpr int BSIM4_NE = 42;
 __host__ __device__ constexpr bool cls_is_mos1(int cls) { return cls == CLS_MOS1 || cls == CLS_MOS1T; }
 // Entries of a tied Mos1 that merge into an earlier entry with the same slot: (s,s) <- 7, 13, 15;
 // (d,s) <- 14; (s,d) <- 12; (s,g) <- 11; (g,s) <- 10; rhs s <- 25.
@@ -18,7 +20,11 @@ __host__ __device__ constexpr bool mos1t_merged(int e) {
 enum { DEV_CAPS = 1, DEV_PMOS = 2 };
 // Slot codes per device (one per stamp entry) by class.
 __host__ __device__ constexpr int code_stride(int cls) {
-    return cls_is_mos1(cls) ? MOS1_NE : cls == CLS_MOS0 ? MOS0_NE : cls == CLS_DIODE ? DIODE_NE : 6;
+    return cls_is_mos1(cls)    ? MOS1_NE
+           : cls == CLS_MOS0  ? MOS0_NE
+           : cls == CLS_DIODE ? DIODE_NE
+           : cls == CLS_BSIM4 ? BSIM4_NE
+                              : 6;
 }
 
 }  // namespace s21

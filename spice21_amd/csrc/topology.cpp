@@ -838,6 +838,16 @@ void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps, int hub_con
             }
             cls_of[ci] = CLS_DIODE;
             break;
+        case S21_COMP_BSIM4:  // no load_ac in the reference's BSIM4: Component's default (comps/mod.rs:86-88)
+            if (t.ac) fail(S21_ERR, "AC Not Implemented For This Component!");
+            for (int e = 0; e < b4::B4_NS; e++) {
+                const int r = nd[b4::b4_row(e)];
+                const int col = b4::b4_col(e) < 0 ? -2 : nd[b4::b4_col(e)];
+                if (b4::b4_col(e) >= 0 && col < 0) continue;
+                add(e, r, col);
+            }
+            cls_of[ci] = CLS_BSIM4;
+            break;
         default: {
             const int p = nd[0], q = nd[1], iv = nd[2];
             if (c.type == S21_COMP_I && t.ac) fail(S21_ERR, "AC Not Implemented For This Component!");

@@ -133,8 +133,8 @@ class Netlist:
 
     def instance(self, overrides, i):
         """Standalone netlist of instance i of a batch with these overrides: every overridden device gets its
-        own model / instance parameter set, as the reference needs one Mos1Model + Mos1InstParams per
-        varied device."""
+        own model / instance parameter set, as the reference needs one model and instance parameter set per
+        varied device (Mos1, Diode or BSIM4)."""
         import copy
 
         m = copy.deepcopy(self)
@@ -147,6 +147,9 @@ class Netlist:
             if mos:
                 table = A.MOS1_MODEL_FIELDS if target == "model" else A.MOS1_INST_FIELDS
                 lst = m.mos1_models if target == "model" else m.mos1_insts
+            elif typ == A.S21_COMP_BSIM4:
+                table = A.BSIM4_MODEL_FIELDS if target == "model" else A.BSIM4_INST_FIELDS
+                lst = m.bsim4_models if target == "model" else m.bsim4_insts
             else:
                 table = A.DIODE_MODEL_FIELDS if target == "model" else A.DIODE_INST_FIELDS
                 lst = m.diode_models if target == "model" else m.diode_insts
@@ -159,7 +162,7 @@ class Netlist:
             lst.append(p)
             if target == "model" and mos:
                 m._mos1_model_r.append(m._mos1_model_r[model])
-            if target == "model" and not mos:
+            if target == "model" and typ == A.S21_COMP_DIODE:
                 m._diode_has_rs.append(m._diode_has_rs[model])
             new = len(lst) - 1
             if target == "model":

@@ -1,7 +1,8 @@
 """BSIM4 on the GPU: the single-instance path's device kernel (one thread per device: Bsim4::op, tran_op,
 stamp) against the reference's three BSIM4 ring-oscillator waveforms and its dcop known answers, against
 the oracle for non-default instance parameters and a larger ring, through the byte ABI, and the error
-paths (batched engine, AC, trapezoidal integration)."""
+paths (profiling kernels, AC, trapezoidal integration); and the batched kernel's BSIM4 class: the goldens, a
+Monte Carlo batch against the oracle per instance, and agreement with the single-instance path."""
 import json
 import os
 
@@ -163,10 +164,89 @@ def test_bsim4_long_ring_matches_the_oracle(precision):
         assert err <= 1e-6
 
 
+@pytest.mark.parametrize("fixture", sorted(BSIM4_GOLDEN_RUNS))
+def test_bsim4_goldens_through_the_batch(fixture):
+    """The batched kernel's BSIM4 class (one lane per device, stamps through slot codes and hub cells) on
+    the reference's three rings."""
+    kind, tstep, tstop = BSIM4_GOLDEN_RUNS[fixture]
+    gold = json.load(open(os.path.join(GOLDEN, fixture)))
+    nl = ro3(kind)
+    log("bsim4 golden batch", fixture)
+    b = Batch(Group(nl.to_c())).tran(tstep, tstop, [("1", 0.0)])
+    st, it = b.status()
+    log("  status", st, "iterations", it)
+    assert st[0] == A.S21_OK
+    assert np.array_equal(b.axis(), np.array(gold["time"]))
+    names = b.var_names()
+    R = np.stack([np.array(gold[nm]) for nm in names], axis=1)
+    X = b.wave(0, 0)
+    assert np.max(np.abs(X - R)) <= 1e-6
+    assert wave_within(X, R, names, list(nl.kinds) + [0, 1], "bsim4 golden batch")
+
+
+def test_bsim4_monte_carlo_batch_matches_the_oracle():
+    """A 24-instance batch of the CMOS ring whose devices' W, L and DELVTO vary per instance: every
+    instance's operating point and 150-point transient against the oracle on that instance's own netlist
+    (FP64 to 1e-9 of each signal's swing), mixed precision to 1e-6 V, and the overrides visibly matter."""
+    from tests.circuits import materialize
+
+    nl = ro3("bsim4_cmos")
+    cn = nl.to_c()
+    n_inst = 24
+    dev = [ci for ci, c in enumerate(nl.comps) if c[0] == A.S21_COMP_BSIM4]
+    rng = np.random.default_rng(7)
+    over = []
+    for ci in dev:
+        over.append((ci, "inst", "w", 5e-6 * (1.0 + 0.3 * rng.uniform(-1, 1, n_inst))))
+        over.append((ci, "inst", "l", 5e-6 * (1.0 + 0.2 * rng.uniform(-1, 1, n_inst))))
+        over.append((ci, "inst", "delvto", 0.03 * rng.uniform(-1, 1, n_inst)))
+    tstep, tstop = 1e-10, 1.5e-8
+    g = Group(cn, n_inst, over)
+    b = Batch(g).tran(tstep, tstop, [("1", 0.0)])
+    st, _ = b.status()
+    names = b.var_names()
+    worst_ok = True
+    for i in range(n_inst):
+        ci = materialize(nl, over, i).to_c()
+        t, X, names_o, _ = O.tran(ci, tstep, tstop, [("1", 0.0)])
+        assert st[i] == A.S21_OK, (i, st[i])
+        assert list(names_o) == names
+        worst_ok = wave_within(b.wave(0, i), X, names, list(nl.kinds) + [0, 1], f"mc {i}") and worst_ok
+    assert worst_ok
+    assert float(np.max(np.abs(b.wave(0, 0) - b.wave(0, n_inst - 1)))) > 0.05
+    m = Batch(Group(cn, n_inst, over), precision=MIXED).tran(tstep, tstop, [("1", 0.0)])
+    assert (m.status()[0] == A.S21_OK).all()
+    v = np.array(list(nl.kinds) + [0, 1]) == A.S21_VAR_V
+    err = max(float(np.max(np.abs(m.wave(0, i) - b.wave(0, i))[:, v])) for i in range(n_inst))
+    log("  mixed vs FP64 max |dV|", err)
+    assert err <= 1e-6
+    # operating points of the non-default-instance circuit, per instance
+    nv = varied_instances()
+    over2 = [(k, "inst", "delvto", 0.02 * np.linspace(-1, 1, 8))
+             for k, c in enumerate(nv.comps) if c[0] == A.S21_COMP_BSIM4]
+    bo = Batch(Group(nv.to_c(), 8, over2)).op()
+    for i in range(8):
+        xo, _, _ = O.dcop(materialize(nv, over2, i).to_c())
+        rel = np.abs(bo.x(0)[i] - xo) / np.maximum(np.abs(xo), 1e-9)
+        assert bo.status()[0][i] == A.S21_OK and rel.max() < 1e-9, (i, rel.max())
+
+
+def test_bsim4_batch_and_single_instance_paths_agree():
+    nl = bsim4_ring()
+    cn = nl.to_c()
+    b = Batch(Group(cn)).tran(1e-12, 100e-12, [("1", 0.0)])
+    g = Large(cn).tran(1e-12, 100e-12, [("1", 0.0)])
+    assert b.status()[0][0] == A.S21_OK and g.status()[0] == A.S21_OK
+    assert b.var_names() == g.var_names()
+    assert wave_within(b.wave(0, 0), g.wave(), g.var_names(), list(nl.kinds) + [0, 1], "batch vs large")
+
+
 def test_bsim4_error_paths():
     nl = ro3("bsim4_cmos")
-    with pytest.raises(S21Error, match="single-instance path"):
-        Batch([Group(nl.to_c(), 1)])
+    with pytest.raises(S21Error, match="profiling kernels"):
+        bb = Batch([Group(nl.to_c(), 1)])
+        bb.set_profiling(True)
+        bb.op()
     with pytest.raises(S21Error, match="Backward Euler only"):
         Large(nl.to_c()).tran_trap(1e-12, 1e-11, [("1", 0.0)])
     with pytest.raises(S21Error, match="factor reuse is not available"):

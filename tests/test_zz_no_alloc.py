@@ -123,3 +123,24 @@ def test_noalloc_op_rejects_unprepared_and_oversized_batches():
     buf = C.create_string_buffer(256)
     lib().s21_last_error(buf, 256)
     assert b"scratch" in buf.value
+
+
+def test_noalloc_op_with_bsim4_devices_matches_op():
+    """BSIM4 devices keep their points in the batch's own buffers (allocated by the preparing s21_batch_op
+    call): the allocation-free call clears them and reproduces s21_batch_op bit for bit, twice in a row."""
+    from tests.test_gpu_bsim4 import varied_instances
+
+    nl = varied_instances()
+    for n_inst in (12, 6):
+        over = [(k, "inst", "delvto", 0.02 * np.linspace(-1, 1, n_inst))
+                for k, c in enumerate(nl.comps) if c[0] == A.S21_COMP_BSIM4]
+        b = Batch(Group(nl.to_c(), n_inst, over))
+        log("noalloc bsim4", n_inst)
+        b.op()
+        x_ref, (st_ref, it_ref) = b.x(), b.status()
+        assert np.all(st_ref == 0)
+        for _ in range(2):
+            rc, x, st, it = noalloc_op(b, n_inst, b.n_vars())
+            assert rc == A.S21_OK
+            assert np.array_equal(st, st_ref) and np.array_equal(it, it_ref)
+            assert np.array_equal(x, x_ref)
