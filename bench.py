@@ -115,7 +115,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
 
 
-def extra_measurements(which=("tran", "ac", "ring5", "grid")):
+def extra_measurements(which=("tran", "ac", "ring5", "grid", "bsim4")):
     """The metric's transient rate and the other workloads, by HIP-event kernel time (each analysis call
     also rebuilds its host structures, which is not the kernel's throughput)."""
     out = {}
@@ -129,7 +129,39 @@ def extra_measurements(which=("tran", "ac", "ring5", "grid")):
         out.update(_power_grid_single())
     if "grid-op" in which:
         out.update(_power_grid_single(tran=False))
+    if "bsim4" in which:
+        out.update(_bsim4_chain_mc())
     return out
+
+
+def _bsim4_chain_mc(n_inst=4096, stages=16):
+    """Batched BSIM4: Monte-Carlo operating points of 16-stage BSIM4 inverter chains (32 devices each, DELVTO
+    and W per device and instance), both precisions (parity with the oracle is tests/test_gpu_bsim4.py's)."""
+    import numpy as np
+
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
+
+    nl, over = W.bsim4_chain_mc(n_inst, stages)
+    g = Group(nl.to_c(), n_inst, over)
+    rec, xs = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        b = Batch(g, precision=prec)
+        t0 = time.perf_counter()
+        b.op()  # untimed: derives and uploads the structure and every instance's BSIM4 parameters
+        prep_s = time.perf_counter() - t0
+        b.op()
+        ms = b.timing()[0]
+        st, it = b.status()
+        iters = int(it.sum())
+        rec[name] = {"instance_iters_per_s": iters / (ms * 1e-3),
+                     "bsim4_evals_per_s": iters * 2 * stages / (ms * 1e-3), "kernel_ms": ms,
+                     "newton_iters": iters, "failed_instances": int(np.sum(st != 0)),
+                     "first_call_s": prep_s, "launch": list(b.config())}
+        xs[name] = b.x()
+    rec["mixed_vs_fp64_max_node_v"] = float(np.abs(xs["mixed"] - xs["fp64"]).max())
+    return {"bsim4_chain_mc": {"workload": f"{n_inst} x {stages}-stage BSIM4 inverter chain dcop "
+                                           f"({2 * stages} BSIM4 per instance, DELVTO and W per device)", **rec}}
 
 
 def _power_grid_single(tran=True):
@@ -303,7 +335,7 @@ def main():
     ap.add_argument("--ipw", type=int, default=0, help="instances per wavefront (implies --no-autotune)")
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the transient / AC / single-ring measurements")
-    ap.add_argument("--extra-only", choices=["tran", "ac", "ring5", "grid", "grid-op"], default=None,
+    ap.add_argument("--extra-only", choices=["tran", "ac", "ring5", "grid", "grid-op", "bsim4"], default=None,
                     help="only run one of the other workloads and print its record (for profiling runs)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="'mixed' / 'fp64' time one path only (for profiling runs)")

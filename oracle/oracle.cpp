@@ -1725,7 +1725,7 @@ int oracle_dcop_batch(const s21_netlist* net, const s21_options* o, int32_t n_ov
     std::atomic<int32_t> next(0);
     auto work = [&]() {
         std::vector<s21_comp> comps;
-        std::vector<s21_params> mm, mi, dm, di;
+        std::vector<s21_params> mm, mi, dm, di, bm, bi;
         for (;;) {
             const int32_t j = next.fetch_add(1);
             if (j >= n_inst) return;
@@ -1735,6 +1735,8 @@ int oracle_dcop_batch(const s21_netlist* net, const s21_options* o, int32_t n_ov
             mi.assign(net->mos1_insts, net->mos1_insts + net->n_mos1_insts);
             dm.assign(net->diode_models, net->diode_models + net->n_diode_models);
             di.assign(net->diode_insts, net->diode_insts + net->n_diode_insts);
+            bm.assign(net->bsim4_models, net->bsim4_models + net->n_bsim4_models);
+            bi.assign(net->bsim4_insts, net->bsim4_insts + net->n_bsim4_insts);
             for (int32_t k = 0; k < n_over; k++) {
                 const s21_override& ov = over[k];
                 s21_comp& c = comps[ov.comp];
@@ -1742,9 +1744,10 @@ int oracle_dcop_batch(const s21_netlist* net, const s21_options* o, int32_t n_ov
                     c.value = ov.vals[i];
                     continue;
                 }
-                const bool mos = c.type == S21_COMP_MOS1;
-                std::vector<s21_params>& lst =
-                    ov.target == S21_OVR_MODEL ? (mos ? mm : dm) : (mos ? mi : di);
+                const bool model = ov.target == S21_OVR_MODEL;
+                std::vector<s21_params>& lst = c.type == S21_COMP_MOS1    ? (model ? mm : mi)
+                                               : c.type == S21_COMP_BSIM4 ? (model ? bm : bi)
+                                                                          : (model ? dm : di);
                 int32_t& idx = ov.target == S21_OVR_MODEL ? c.model : c.inst;
                 s21_params p = lst[idx];
                 p.v[ov.field] = ov.vals[i];
@@ -1762,6 +1765,10 @@ int oracle_dcop_batch(const s21_netlist* net, const s21_options* o, int32_t n_ov
             n2.n_diode_models = (int32_t)dm.size();
             n2.diode_insts = di.data();
             n2.n_diode_insts = (int32_t)di.size();
+            n2.bsim4_models = bm.data();
+            n2.n_bsim4_models = (int32_t)bm.size();
+            n2.bsim4_insts = bi.data();
+            n2.n_bsim4_insts = (int32_t)bi.size();
             int64_t it = 0;
             const int st = oracle_dcop(&n2, o, x_out + (size_t)j * nv, &it, nullptr, nullptr, 0);
             if (iters_out) iters_out[j] = it;

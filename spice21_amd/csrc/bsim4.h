@@ -114,6 +114,11 @@ struct B4Par {
     B4D d;
     B4S s;
     B4I i;
+    // field K among the structure's doubles (the parameter-view interface of the evaluation functions)
+    template <int K>
+    __host__ __device__ double get() const {
+        return reinterpret_cast<const double*>(this)[K];
+    }
 };
 
 // Instance parameters a Bsim4InstParams message can set (protos/bsim4.proto:6-41) that reach the model at

@@ -6,6 +6,18 @@
 
 #include "bsim4.h"
 
+// A parameter read in the functions below: field f of part m (model values), d (derived), s (size-dependent)
+// or i (instance) of the device's B4Par, through the parameter view P -- a B4Par itself, or the batched
+// kernel's view that gathers each field from where it is stored -- as P.get<K>(), K the field's index among
+// the structure's doubles.
+#define B4_FIELD(part, type, f)                                                                          \
+    P.template get<(int)((__builtin_offsetof(::s21::b4::B4Par, part) + __builtin_offsetof(::s21::b4::type, f)) / \
+                         sizeof(double))>()
+#define B4_M(f) B4_FIELD(m, B4M, f)
+#define B4_D(f) B4_FIELD(d, B4D, f)
+#define B4_S(f) B4_FIELD(s, B4S, f)
+#define B4_I(f) B4_FIELD(i, B4I, f)
+
 namespace s21 {
 namespace b4 {
 
@@ -126,13 +138,10 @@ __host__ __device__ inline void junction(double Isat, double Nvtm, double vjmFwd
 }
 
 // The DC part of the operating point and the intrinsic charges (bsim4solver.rs:145-3754, default modes).
-__host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd_, double vg_, double vs_, double vb_,
+template <class PV>
+__host__ __device__ inline void op_eval(const PV& P, const B4Op& G, double vd_, double vg_, double vs_, double vb_,
                                         double gmin, B4Op& N) {
-    const B4M& M = P.m;
-    const B4D& D = P.d;
-    const B4S& S = P.s;
-    const B4I& I = P.i;
-    const double p = M.p;
+    const double p = B4_M(p);
     double T0, T1, T2, T3, T4, T5, T6, T7, T8, T9, T10, T11, T12, T13, T14, tmp, tmp1, tmp2, tmp3, tmp4;
     double dT0_dVg, dT0_dVd, dT0_dVb, dT1_dVg, dT1_dVd, dT1_dVb, dT2_dVg, dT2_dVd, dT2_dVb;
     double dT3_dVg, dT3_dVd, dT3_dVb, dT4_dVd, dT5_dVb, dT1b, dT2b, dT3b, dT4b, dT5b, dT6b;
@@ -168,10 +177,10 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         vgs = vgd + vds;
     }
     if (vds >= 0.0) {
-        vbs = pnjlim(vbs, G.vbs, VT_REF, D.vcrit);
+        vbs = pnjlim(vbs, G.vbs, VT_REF, B4_D(vcrit));
         vbd = vbs - vds;
     } else {
-        vbd = pnjlim(vbd, G.vbd, VT_REF, D.vcrit);
+        vbd = pnjlim(vbd, G.vbd, VT_REF, B4_D(vcrit));
         vbs = vbd + vds;
     }
     vbd = vbs - vds;
@@ -183,19 +192,19 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     const double vbd_jct = vdbd;
 
     // source/drain junction diodes (diomod 1) and trap-assisted tunnelling (:528-771)
-    junction(I.SourceSatCurrent, D.Nvtms, I.vjsmFwd, I.IVjsmFwd, vbs_jct, gmin, N.gbs, N.cbs);
-    junction(I.DrainSatCurrent, D.Nvtmd, I.vjdmFwd, I.IVjdmFwd, vbd_jct, gmin, N.gbd, N.cbd);
+    junction(B4_I(SourceSatCurrent), B4_D(Nvtms), B4_I(vjsmFwd), B4_I(IVjsmFwd), vbs_jct, gmin, N.gbs, N.cbs);
+    junction(B4_I(DrainSatCurrent), B4_D(Nvtmd), B4_I(vjdmFwd), B4_I(IVjdmFwd), vbd_jct, gmin, N.gbd, N.cbd);
     double Ts1, Ts2, Ts3, Ts4, Ts5, Ts6;
-    tat(M.vtss, D.Nvtmrss, vbs_jct, Ts1, dT1b);
-    tat(M.vtsd, D.Nvtmrsd, vbd_jct, Ts2, dT2b);
-    tat(M.vtssws, D.Nvtmrssws, vbs_jct, Ts3, dT3b);
-    tat(M.vtsswd, D.Nvtmrsswd, vbd_jct, Ts4, dT4b);
-    tat(M.vtsswgs, D.Nvtmrsswgs, vbs_jct, Ts5, dT5b);
-    tat(M.vtsswgd, D.Nvtmrsswgd, vbd_jct, Ts6, dT6b);
-    N.gbs += I.SjctTempRevSatCur * dT1b + I.SswTempRevSatCur * dT3b + I.SswgTempRevSatCur * dT5b;
-    N.cbs -= I.SjctTempRevSatCur * (Ts1 - 1.0) + I.SswTempRevSatCur * (Ts3 - 1.0) + I.SswgTempRevSatCur * (Ts5 - 1.0);
-    N.gbd += I.DjctTempRevSatCur * dT2b + I.DswTempRevSatCur * dT4b + I.DswgTempRevSatCur * dT6b;
-    N.cbd -= I.DjctTempRevSatCur * (Ts2 - 1.0) + I.DswTempRevSatCur * (Ts4 - 1.0) + I.DswgTempRevSatCur * (Ts6 - 1.0);
+    tat(B4_M(vtss), B4_D(Nvtmrss), vbs_jct, Ts1, dT1b);
+    tat(B4_M(vtsd), B4_D(Nvtmrsd), vbd_jct, Ts2, dT2b);
+    tat(B4_M(vtssws), B4_D(Nvtmrssws), vbs_jct, Ts3, dT3b);
+    tat(B4_M(vtsswd), B4_D(Nvtmrsswd), vbd_jct, Ts4, dT4b);
+    tat(B4_M(vtsswgs), B4_D(Nvtmrsswgs), vbs_jct, Ts5, dT5b);
+    tat(B4_M(vtsswgd), B4_D(Nvtmrsswgd), vbd_jct, Ts6, dT6b);
+    N.gbs += B4_I(SjctTempRevSatCur) * dT1b + B4_I(SswTempRevSatCur) * dT3b + B4_I(SswgTempRevSatCur) * dT5b;
+    N.cbs -= B4_I(SjctTempRevSatCur) * (Ts1 - 1.0) + B4_I(SswTempRevSatCur) * (Ts3 - 1.0) + B4_I(SswgTempRevSatCur) * (Ts5 - 1.0);
+    N.gbd += B4_I(DjctTempRevSatCur) * dT2b + B4_I(DswTempRevSatCur) * dT4b + B4_I(DswgTempRevSatCur) * dT6b;
+    N.cbd -= B4_I(DjctTempRevSatCur) * (Ts2 - 1.0) + B4_I(DswTempRevSatCur) * (Ts4 - 1.0) + B4_I(DswgTempRevSatCur) * (Ts6 - 1.0);
 
     double Vds, Vgs, Vbs, Vdb;
     if (vds >= 0.0) {
@@ -213,62 +222,62 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     }
     (void)Vgs;
     (void)Vdb;
-    const double toxe = M.toxe;
-    const double epssub = D.epssub;
+    const double toxe = B4_M(toxe);
+    const double epssub = B4_D(epssub);
 
     // Vbseff (:790-810)
     double Vbseff, dVbseff_dVb;
-    T0 = Vbs - I.vbsc - 0.001;
-    T1 = sqrt(T0 * T0 - 0.004 * I.vbsc);
+    T0 = Vbs - B4_I(vbsc) - 0.001;
+    T1 = sqrt(T0 * T0 - 0.004 * B4_I(vbsc));
     if (T0 >= 0.0) {
-        Vbseff = I.vbsc + 0.5 * (T0 + T1);
+        Vbseff = B4_I(vbsc) + 0.5 * (T0 + T1);
         dVbseff_dVb = 0.5 * (1.0 + T0 / T1);
     } else {
         T2 = -0.002 / (T1 - T0);
-        Vbseff = I.vbsc * (1.0 + T2);
-        dVbseff_dVb = T2 * I.vbsc / T1;
+        Vbseff = B4_I(vbsc) * (1.0 + T2);
+        dVbseff_dVb = T2 * B4_I(vbsc) / T1;
     }
-    T9 = 0.95 * S.phi;
+    T9 = 0.95 * B4_S(phi);
     T0 = T9 - Vbseff - 0.001;
     T1 = sqrt(T0 * T0 + 0.004 * T9);
     Vbseff = T9 - 0.5 * (T0 + T1);
     dVbseff_dVb *= 0.5 * (1.0 + T0 / T1);
-    const double Phis = S.phi - Vbseff;
+    const double Phis = B4_S(phi) - Vbseff;
     const double dPhis_dVb = -1.0;
     const double sqrtPhis = sqrt(Phis);
     const double dsqrtPhis_dVb = -0.5 / sqrtPhis;
-    const double Xdep = S.Xdep0 * sqrtPhis / S.sqrtPhi;
-    const double dXdep_dVb = (S.Xdep0 / S.sqrtPhi) * dsqrtPhis_dVb;
-    const double Leff = S.leff;
-    const double Vtm = D.vtm;
+    const double Xdep = B4_S(Xdep0) * sqrtPhis / B4_S(sqrtPhi);
+    const double dXdep_dVb = (B4_S(Xdep0) / B4_S(sqrtPhi)) * dsqrtPhis_dVb;
+    const double Leff = B4_S(leff);
+    const double Vtm = B4_D(vtm);
 
     // Vth (:826-905)
     T3 = sqrt(Xdep);
-    const double V0 = S.vbi - S.phi;
-    T0 = S.dvt2 * Vbseff;
+    const double V0 = B4_S(vbi) - B4_S(phi);
+    T0 = B4_S(dvt2) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
-        T2 = S.dvt2;
+        T2 = B4_S(dvt2);
     } else {
         T4 = 1.0 / (3.0 + 8.0 * T0);
         T1 = (1.0 + 3.0 * T0) * T4;
-        T2 = S.dvt2 * T4 * T4;
+        T2 = B4_S(dvt2) * T4 * T4;
     }
-    const double lt1 = D.factor1 * T3 * T1;
-    const double dlt1_dVb = D.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
-    T0 = S.dvt2w * Vbseff;
+    const double lt1 = B4_D(factor1) * T3 * T1;
+    const double dlt1_dVb = B4_D(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    T0 = B4_S(dvt2w) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
-        T2 = S.dvt2w;
+        T2 = B4_S(dvt2w);
     } else {
         T4 = 1.0 / (3.0 + 8.0 * T0);
         T1 = (1.0 + 3.0 * T0) * T4;
-        T2 = S.dvt2w * T4 * T4;
+        T2 = B4_S(dvt2w) * T4 * T4;
     }
-    const double ltw = D.factor1 * T3 * T1;
-    const double dltw_dVb = D.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    const double ltw = B4_D(factor1) * T3 * T1;
+    const double dltw_dVb = B4_D(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
     double Theta0, dTheta0_dVb;
-    T0 = S.dvt1 * Leff / lt1;
+    T0 = B4_S(dvt1) * Leff / lt1;
     if (T0 < EXP_THRESHOLD) {
         T1 = exp(T0);
         T2 = T1 - 1.0;
@@ -281,10 +290,10 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         Theta0 = 1.0 / (MAX_EXP - 2.0);
         dTheta0_dVb = 0.0;
     }
-    N.thetavth = S.dvt0 * Theta0;
+    N.thetavth = B4_S(dvt0) * Theta0;
     const double Delt_vth = N.thetavth * V0;
-    const double dDelt_vth_dVb = S.dvt0 * dTheta0_dVb * V0;
-    T0 = S.dvt1w * S.weff * Leff / ltw;
+    const double dDelt_vth_dVb = B4_S(dvt0) * dTheta0_dVb * V0;
+    T0 = B4_S(dvt1w) * B4_S(weff) * Leff / ltw;
     if (T0 < EXP_THRESHOLD) {
         T1 = exp(T0);
         T2 = T1 - 1.0;
@@ -297,13 +306,13 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         T5 = 1.0 / (MAX_EXP - 2.0);
         dT5_dVb = 0.0;
     }
-    T0 = S.dvt0w * T5;
+    T0 = B4_S(dvt0w) * T5;
     T2 = T0 * V0;
-    dT2_dVb = S.dvt0w * dT5_dVb * V0;
-    T0 = sqrt(1.0 + S.lpe0 / Leff);
-    T1 = S.k1ox * (T0 - 1.0) * S.sqrtPhi + (S.kt1 + S.kt1l / Leff + S.kt2 * Vbseff) * (D.TempRatio - 1.0);
-    const double Vth_NarrowW = toxe * S.phi / (S.weff + S.w0);
-    T3 = I.eta0 + S.etab * Vbseff;
+    dT2_dVb = B4_S(dvt0w) * dT5_dVb * V0;
+    T0 = sqrt(1.0 + B4_S(lpe0) / Leff);
+    T1 = B4_S(k1ox) * (T0 - 1.0) * B4_S(sqrtPhi) + (B4_S(kt1) + B4_S(kt1l) / Leff + B4_S(kt2) * Vbseff) * (B4_D(TempRatio) - 1.0);
+    const double Vth_NarrowW = toxe * B4_S(phi) / (B4_S(weff) + B4_S(w0));
+    T3 = B4_I(eta0) + B4_S(etab) * Vbseff;
     if (T3 < 1.0e-4) {
         T9 = 1.0 / (3.0 - 2.0e4 * T3);
         T3 = (2.0e-4 - T3) * T9;
@@ -311,46 +320,46 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     } else {
         T4 = 1.0;
     }
-    const double dDIBL_Sft_dVd = T3 * S.theta0vb0;
+    const double dDIBL_Sft_dVd = T3 * B4_S(theta0vb0);
     const double DIBL_Sft = dDIBL_Sft_dVd * Vds;
-    const double Lpe_Vb = sqrt(1.0 + S.lpeb / Leff);
-    double Vth = p * I.vth0 + (S.k1ox * sqrtPhis - S.k1 * S.sqrtPhi) * Lpe_Vb - I.k2ox * Vbseff - Delt_vth - T2 +
-                 (S.k3 + S.k3b * Vbseff) * Vth_NarrowW + T1 - DIBL_Sft;
-    double dVth_dVb = Lpe_Vb * S.k1ox * dsqrtPhis_dVb - I.k2ox - dDelt_vth_dVb - dT2_dVb + S.k3b * Vth_NarrowW -
-                      S.etab * Vds * S.theta0vb0 * T4 + S.kt2 * (D.TempRatio - 1.0);
+    const double Lpe_Vb = sqrt(1.0 + B4_S(lpeb) / Leff);
+    double Vth = p * B4_I(vth0) + (B4_S(k1ox) * sqrtPhis - B4_S(k1) * B4_S(sqrtPhi)) * Lpe_Vb - B4_I(k2ox) * Vbseff - Delt_vth - T2 +
+                 (B4_S(k3) + B4_S(k3b) * Vbseff) * Vth_NarrowW + T1 - DIBL_Sft;
+    double dVth_dVb = Lpe_Vb * B4_S(k1ox) * dsqrtPhis_dVb - B4_I(k2ox) - dDelt_vth_dVb - dT2_dVb + B4_S(k3b) * Vth_NarrowW -
+                      B4_S(etab) * Vds * B4_S(theta0vb0) * T4 + B4_S(kt2) * (B4_D(TempRatio) - 1.0);
     double dVth_dVd = -dDIBL_Sft_dVd;
 
     // n (:907-926)
     double n, dn_dVb, dn_dVd;
     tmp1 = epssub / Xdep;
-    N.nstar = D.vtm / Q * (D.coxe + tmp1 + S.cit);
-    tmp2 = S.nfactor * tmp1;
-    tmp3 = S.cdsc + S.cdscb * Vbseff + S.cdscd * Vds;
-    tmp4 = (tmp2 + tmp3 * Theta0 + S.cit) / D.coxe;
+    N.nstar = B4_D(vtm) / Q * (B4_D(coxe) + tmp1 + B4_S(cit));
+    tmp2 = B4_S(nfactor) * tmp1;
+    tmp3 = B4_S(cdsc) + B4_S(cdscb) * Vbseff + B4_S(cdscd) * Vds;
+    tmp4 = (tmp2 + tmp3 * Theta0 + B4_S(cit)) / B4_D(coxe);
     if (tmp4 >= -0.5) {
         n = 1.0 + tmp4;
-        dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + S.cdscb * Theta0) / D.coxe;
-        dn_dVd = S.cdscd * Theta0 / D.coxe;
+        dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + B4_S(cdscb) * Theta0) / B4_D(coxe);
+        dn_dVd = B4_S(cdscd) * Theta0 / B4_D(coxe);
     } else {
         T0 = 1.0 / (3.0 + 8.0 * tmp4);
         n = (1.0 + 3.0 * tmp4) * T0;
         T0 *= T0;
-        dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + S.cdscb * Theta0) / D.coxe * T0;
-        dn_dVd = S.cdscd * Theta0 / D.coxe * T0;
+        dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + B4_S(cdscb) * Theta0) / B4_D(coxe) * T0;
+        dn_dVd = B4_S(cdscd) * Theta0 / B4_D(coxe) * T0;
     }
     // pocket implant (:928-963)
-    if (S.dvtp0 > 0.0) {
-        T0 = -S.dvtp1 * Vds;
+    if (B4_S(dvtp0) > 0.0) {
+        T0 = -B4_S(dvtp1) * Vds;
         double dT2_dVd_;
         if (T0 < -EXP_THRESHOLD) {
             T2 = MIN_EXP;
             dT2_dVd_ = 0.0;
         } else {
             T2 = exp(T0);
-            dT2_dVd_ = -S.dvtp1 * T2;
+            dT2_dVd_ = -B4_S(dvtp1) * T2;
         }
-        T3 = Leff + S.dvtp0 * (1.0 + T2);
-        const double dT3_dVd_ = S.dvtp0 * dT2_dVd_;
+        T3 = Leff + B4_S(dvtp0) * (1.0 + T2);
+        const double dT3_dVd_ = B4_S(dvtp0) * dT2_dVd_;
         T4 = Vtm * log(Leff / T3);
         dT4_dVd = -Vtm * dT3_dVd_ / T3;
         const double dDITS_Sft_dVd = dn_dVd * T4 + n * dT4_dVd;
@@ -360,23 +369,23 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dVth_dVb -= dDITS_Sft_dVb;
     }
     // DITS_SFT2 (:965-974)
-    if (!((S.dvtp4 == 0.0) || (S.dvtp2factor == 0.0))) {
-        T1 = 2.0 * S.dvtp4 * Vds;
+    if (!((B4_S(dvtp4) == 0.0) || (B4_S(dvtp2factor) == 0.0))) {
+        T1 = 2.0 * B4_S(dvtp4) * Vds;
         T0 = dexpb(T1);
         T10 = dexpc(T1);
-        const double DITS_Sft2 = S.dvtp2factor * (T0 - 1.0) / (T0 + 1.0);
-        const double dDITS_Sft2_dVd = S.dvtp2factor * S.dvtp4 * 4.0 * T10 / ((T0 + 1.0) * (T0 + 1.0));
+        const double DITS_Sft2 = B4_S(dvtp2factor) * (T0 - 1.0) / (T0 + 1.0);
+        const double dDITS_Sft2_dVd = B4_S(dvtp2factor) * B4_S(dvtp4) * 4.0 * T10 / ((T0 + 1.0) * (T0 + 1.0));
         Vth -= DITS_Sft2;
         dVth_dVd -= dDITS_Sft2_dVd;
     }
     N.von = Vth;
 
     // poly depletion (:978-1004)
-    T0 = I.vfb + S.phi;
+    T0 = B4_I(vfb) + B4_S(phi);
     T1 = EPSSI;
     double vgs_eff, dvgs_eff_dvg, vgd_eff, dvgd_eff_dvg;
-    poly_depletion(T0, S.ngate, T1, D.coxe, vgs, vgs_eff, dvgs_eff_dvg);
-    poly_depletion(T0, S.ngate, T1, D.coxe, vgd, vgd_eff, dvgd_eff_dvg);
+    poly_depletion(T0, B4_S(ngate), T1, B4_D(coxe), vgs, vgs_eff, dvgs_eff_dvg);
+    poly_depletion(T0, B4_S(ngate), T1, B4_D(coxe), vgd, vgd_eff, dvgd_eff_dvg);
     double Vgs_eff, dVgs_eff_dVg;
     if (N.mode > 0) {
         Vgs_eff = vgs_eff;
@@ -394,13 +403,13 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     // Vgsteff (:1008-1062)
     double dT10_dVg, dT10_dVd, dT10_dVb, dT9_dVg, dT9_dVd, dT9_dVb;
     T0 = n * Vtm;
-    T1 = S.mstar * Vgst;
+    T1 = B4_S(mstar) * Vgst;
     T2 = T1 / T0;
     if (T2 > EXP_THRESHOLD) {
         T10 = T1;
-        dT10_dVg = S.mstar * dVgs_eff_dVg;
-        dT10_dVd = -dVth_dVd * S.mstar;
-        dT10_dVb = -dVth_dVb * S.mstar;
+        dT10_dVg = B4_S(mstar) * dVgs_eff_dVg;
+        dT10_dVd = -dVth_dVd * B4_S(mstar);
+        dT10_dVb = -dVth_dVb * B4_S(mstar);
     } else if (T2 < -EXP_THRESHOLD) {
         T10 = Vtm * log(1.0 + MIN_EXP);
         dT10_dVg = 0.0;
@@ -411,32 +420,32 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         const double ExpVgst = exp(T2);
         T3 = Vtm * log(1.0 + ExpVgst);
         T10 = n * T3;
-        dT10_dVg = S.mstar * ExpVgst / (1.0 + ExpVgst);
+        dT10_dVg = B4_S(mstar) * ExpVgst / (1.0 + ExpVgst);
         dT10_dVb = T3 * dn_dVb - dT10_dVg * (dVth_dVb + Vgst * dn_dVb / n);
         dT10_dVd = T3 * dn_dVd - dT10_dVg * (dVth_dVd + Vgst * dn_dVd / n);
         dT10_dVg *= dVgs_eff_dVg;
     }
-    T1 = S.voffcbn - (1.0 - S.mstar) * Vgst;
+    T1 = B4_S(voffcbn) - (1.0 - B4_S(mstar)) * Vgst;
     T2 = T1 / T0;
     if (T2 < -EXP_THRESHOLD) {
-        T3 = D.coxe * MIN_EXP / S.cdep0;
-        T9 = S.mstar + T3 * n;
+        T3 = B4_D(coxe) * MIN_EXP / B4_S(cdep0);
+        T9 = B4_S(mstar) + T3 * n;
         dT9_dVg = 0.0;
         dT9_dVd = dn_dVd * T3;
         dT9_dVb = dn_dVb * T3;
     } else if (T2 > EXP_THRESHOLD) {
-        T3 = D.coxe * MAX_EXP / S.cdep0;
-        T9 = S.mstar + T3 * n;
+        T3 = B4_D(coxe) * MAX_EXP / B4_S(cdep0);
+        T9 = B4_S(mstar) + T3 * n;
         dT9_dVg = 0.0;
         dT9_dVd = dn_dVd * T3;
         dT9_dVb = dn_dVb * T3;
     } else {
         const double ExpVgst = exp(T2);
-        T3 = D.coxe / S.cdep0;
+        T3 = B4_D(coxe) / B4_S(cdep0);
         T4 = T3 * ExpVgst;
         T5 = T1 * T4 / T0;
-        T9 = S.mstar + n * T4;
-        dT9_dVg = T3 * (S.mstar - 1.0) * ExpVgst / Vtm;
+        T9 = B4_S(mstar) + n * T4;
+        dT9_dVg = T3 * (B4_S(mstar) - 1.0) * ExpVgst / Vtm;
         dT9_dVb = T4 * dn_dVb - dT9_dVg * dVth_dVb - T5 * dn_dVb;
         dT9_dVd = T4 * dn_dVd - dT9_dVg * dVth_dVd - T5 * dn_dVd;
         dT9_dVg *= dVgs_eff_dVg;
@@ -449,10 +458,10 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     const double dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
 
     // effective width (:1064-1078); Rds(V) is computed only for rdsmod > 1 in the reference: zero here
-    T9 = sqrtPhis - S.sqrtPhi;
-    double Weff = S.weff - 2.0 * (S.dwg * Vgsteff + S.dwb * T9);
-    double dWeff_dVg = -2.0 * S.dwg;
-    double dWeff_dVb = -2.0 * S.dwb * dsqrtPhis_dVb;
+    T9 = sqrtPhis - B4_S(sqrtPhi);
+    double Weff = B4_S(weff) - 2.0 * (B4_S(dwg) * Vgsteff + B4_S(dwb) * T9);
+    double dWeff_dVg = -2.0 * B4_S(dwg);
+    double dWeff_dVb = -2.0 * B4_S(dwb) * dsqrtPhis_dVb;
     if (Weff < 2.0e-8) {
         T0 = 1.0 / (6.0e-8 - 2.0 * Weff);
         Weff = 2.0e-8 * (4.0e-8 - Weff) * T0;
@@ -463,22 +472,22 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     const double Rds = 0.0, dRds_dVg = 0.0, dRds_dVb = 0.0;
 
     // Abulk (:1105-1163)
-    T9 = 0.5 * S.k1ox * Lpe_Vb / sqrtPhis;
-    T1 = T9 + I.k2ox - S.k3b * Vth_NarrowW;
+    T9 = 0.5 * B4_S(k1ox) * Lpe_Vb / sqrtPhis;
+    T1 = T9 + B4_I(k2ox) - B4_S(k3b) * Vth_NarrowW;
     dT1_dVb = -T9 / sqrtPhis * dsqrtPhis_dVb;
-    T9 = sqrt(S.xj * Xdep);
+    T9 = sqrt(B4_S(xj) * Xdep);
     tmp1 = Leff + 2.0 * T9;
     T5 = Leff / tmp1;
-    tmp2 = S.a0 * T5;
-    tmp3 = S.weff + S.b1;
-    tmp4 = S.b0 / tmp3;
+    tmp2 = B4_S(a0) * T5;
+    tmp3 = B4_S(weff) + B4_S(b1);
+    tmp4 = B4_S(b0) / tmp3;
     T2 = tmp2 + tmp4;
     dT2_dVb = -T9 / tmp1 / Xdep * dXdep_dVb;
     T6 = T5 * T5;
     T7 = T5 * T6;
     double Abulk0 = 1.0 + T1 * T2;
     double dAbulk0_dVb = T1 * tmp2 * dT2_dVb + T2 * dT1_dVb;
-    T8 = S.ags * S.a0 * T7;
+    T8 = B4_S(ags) * B4_S(a0) * T7;
     double dAbulk_dVg = -T1 * T8;
     double Abulk = Abulk0 + dAbulk_dVg * Vgsteff;
     double dAbulk_dVb = dAbulk0_dVb - T8 * Vgsteff * (dT1_dVb + 3.0 * T1 * dT2_dVb);
@@ -495,14 +504,14 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dAbulk_dVg *= T10;
     }
     N.Abulk = Abulk;
-    T2 = S.keta * Vbseff;
+    T2 = B4_S(keta) * Vbseff;
     if (T2 >= -0.9) {
         T0 = 1.0 / (1.0 + T2);
-        dT0_dVb = -S.keta * T0 * T0;
+        dT0_dVb = -B4_S(keta) * T0 * T0;
     } else {
         T1 = 1.0 / (0.8 + T2);
         T0 = (17.0 + 20.0 * T2) * T1;
-        dT0_dVb = -S.keta * T1 * T1;
+        dT0_dVb = -B4_S(keta) * T1 * T1;
     }
     dAbulk_dVg *= T0;
     dAbulk_dVb = dAbulk_dVb * T0 + Abulk * dT0_dVb;
@@ -514,20 +523,20 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     double Denomi, dDenomi_dVg, dDenomi_dVd, dDenomi_dVb;
     T14 = 0.0;
     T0 = Vgsteff + Vth + Vth - T14;
-    T2 = S.ua + S.uc * Vbseff;
+    T2 = B4_S(ua) + B4_S(uc) * Vbseff;
     T3 = T0 / toxe;
     T12 = sqrt(Vth * Vth + 0.0001);
     T9 = 1.0 / (Vgsteff + 2.0 * T12);
     T10 = T9 * toxe;
-    T8 = S.ud * T10 * T10 * Vth;
+    T8 = B4_S(ud) * T10 * T10 * Vth;
     T6 = T8 * Vth;
-    T5 = T3 * (T2 + S.ub * T3) + T6;
+    T5 = T3 * (T2 + B4_S(ub) * T3) + T6;
     T7 = -2.0 * T6 * T9;
     T11 = T7 * Vth / T12;
-    dDenomi_dVg = (T2 + 2.0 * S.ub * T3) / toxe;
+    dDenomi_dVg = (T2 + 2.0 * B4_S(ub) * T3) / toxe;
     T13 = 2.0 * (dDenomi_dVg + T11 + T8);
     dDenomi_dVd = T13 * dVth_dVd;
-    dDenomi_dVb = T13 * dVth_dVb + S.uc * T3;
+    dDenomi_dVb = T13 * dVth_dVb + B4_S(uc) * T3;
     dDenomi_dVg += T7;
     if (T5 >= -0.8) {
         Denomi = 1.0 + T5;
@@ -539,36 +548,36 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dDenomi_dVd *= T9;
         dDenomi_dVb *= T9;
     }
-    const double ueff = I.u0temp / Denomi;
+    const double ueff = B4_I(u0temp) / Denomi;
     T9 = -ueff / Denomi;
     const double dueff_dVg = T9 * dDenomi_dVg;
     const double dueff_dVd = T9 * dDenomi_dVd;
     const double dueff_dVb = T9 * dDenomi_dVb;
 
     // Vdsat (:1317-1396)
-    const double WVCox = Weff * I.vsattemp * D.coxe;
+    const double WVCox = Weff * B4_I(vsattemp) * B4_D(coxe);
     const double WVCoxRds = WVCox * Rds;
-    double Esat = 2.0 * I.vsattemp / ueff;
+    double Esat = 2.0 * B4_I(vsattemp) / ueff;
     double EsatL = Esat * Leff;
     T0 = -EsatL / ueff;
     double dEsatL_dVg = T0 * dueff_dVg;
     double dEsatL_dVd = T0 * dueff_dVd;
     double dEsatL_dVb = T0 * dueff_dVb;
     double Lambda, dLambda_dVg;
-    if (S.a1 == 0.0) {
-        Lambda = S.a2;
+    if (B4_S(a1) == 0.0) {
+        Lambda = B4_S(a2);
         dLambda_dVg = 0.0;
-    } else if (S.a1 > 0.0) {
-        T0 = 1.0 - S.a2;
-        T1 = T0 - S.a1 * Vgsteff - 0.0001;
+    } else if (B4_S(a1) > 0.0) {
+        T0 = 1.0 - B4_S(a2);
+        T1 = T0 - B4_S(a1) * Vgsteff - 0.0001;
         T2 = sqrt(T1 * T1 + 0.0004 * T0);
-        Lambda = S.a2 + T0 - 0.5 * (T1 + T2);
-        dLambda_dVg = 0.5 * S.a1 * (1.0 + T1 / T2);
+        Lambda = B4_S(a2) + T0 - 0.5 * (T1 + T2);
+        dLambda_dVg = 0.5 * B4_S(a1) * (1.0 + T1 / T2);
     } else {
-        T1 = S.a2 + S.a1 * Vgsteff - 0.0001;
-        T2 = sqrt(T1 * T1 + 0.0004 * S.a2);
+        T1 = B4_S(a2) + B4_S(a1) * Vgsteff - 0.0001;
+        T2 = sqrt(T1 * T1 + 0.0004 * B4_S(a2));
         Lambda = 0.5 * (T1 + T2);
-        dLambda_dVg = 0.5 * S.a1 * (1.0 + T1 / T2);
+        dLambda_dVg = 0.5 * B4_S(a1) * (1.0 + T1 / T2);
     }
     const double Vgst2Vtm = Vgsteff + 2.0 * Vtm;
     if (Rds > 0.0) {
@@ -621,13 +630,13 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
 
     // Vdseff (:1398-1438)
     double Vdseff, dVdseff_dVg, dVdseff_dVd, dVdseff_dVb;
-    T1 = Vdsat - Vds - S.delta;
+    T1 = Vdsat - Vds - B4_S(delta);
     dT1_dVg = dVdsat_dVg;
     dT1_dVd = dVdsat_dVd - 1.0;
     dT1_dVb = dVdsat_dVb;
-    T2 = sqrt(T1 * T1 + 4.0 * S.delta * Vdsat);
+    T2 = sqrt(T1 * T1 + 4.0 * B4_S(delta) * Vdsat);
     T0 = T1 / T2;
-    T9 = 2.0 * S.delta;
+    T9 = 2.0 * B4_S(delta);
     T3 = T9 / T2;
     dT2_dVg = T0 * dT1_dVg + T3 * dVdsat_dVg;
     dT2_dVd = T0 * dT1_dVd + T3 * dVdsat_dVd;
@@ -676,16 +685,16 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     const double dVasat_dVd = dT0_dVd / T1;
 
     // Idl (:1512-1565)
-    tmp1 = I.vtfbphi2;
-    tmp2 = 2.0e8 * I.toxp;
+    tmp1 = B4_I(vtfbphi2);
+    tmp2 = 2.0e8 * B4_I(toxp);
     dT0_dVg = 1.0 / tmp2;
     T0 = (Vgsteff + tmp1) * dT0_dVg;
-    tmp3 = exp(M.bdos * 0.7 * log(T0));
+    tmp3 = exp(B4_M(bdos) * 0.7 * log(T0));
     T1 = 1.0 + tmp3;
-    T2 = M.bdos * 0.7 * tmp3 / T0;
-    const double Tcen = M.ados * 1.9e-9 / T1;
+    T2 = B4_M(bdos) * 0.7 * tmp3 / T0;
+    const double Tcen = B4_M(ados) * 1.9e-9 / T1;
     const double dTcen_dVg = -Tcen * T2 * dT0_dVg / T1;
-    const double Coxeff = epssub * I.coxp / (epssub + I.coxp * Tcen);
+    const double Coxeff = epssub * B4_I(coxp) / (epssub + B4_I(coxp) * Tcen);
     const double dCoxeff_dVg = -Coxeff * Coxeff * dTcen_dVg / epssub;
     const double CoxeffWovL = Coxeff * Weff / Leff;
     const double beta = ueff * CoxeffWovL;
@@ -721,17 +730,17 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
 
     // pocket-implant degradation factor (:1567-1576)
     double FP, dFP_dVg;
-    if (S.fprout <= 0.0) {
+    if (B4_S(fprout) <= 0.0) {
         FP = 1.0;
         dFP_dVg = 0.0;
     } else {
-        T9 = S.fprout * sqrt(Leff) / Vgst2Vtm;
+        T9 = B4_S(fprout) * sqrt(Leff) / Vgst2Vtm;
         FP = 1.0 / (1.0 + T9);
         dFP_dVg = FP * FP * T9 / Vgst2Vtm;
     }
     // VACLM (:1578-1645)
     double PvagTerm, dPvagTerm_dVg, dPvagTerm_dVd, dPvagTerm_dVb;
-    T8 = S.pvag / EsatL;
+    T8 = B4_S(pvag) / EsatL;
     T9 = T8 * Vgsteff;
     if (T9 > -0.9) {
         PvagTerm = 1.0 + T9;
@@ -748,7 +757,7 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dPvagTerm_dVd = -T9 * dEsatL_dVd;
     }
     double Cclm, dCclm_dVg, dCclm_dVd, dCclm_dVb, VACLM, dVACLM_dVg, dVACLM_dVd, dVACLM_dVb;
-    if ((S.pclm > MIN_EXP) && (diffVds > 1.0e-10)) {
+    if ((B4_S(pclm) > MIN_EXP) && (diffVds > 1.0e-10)) {
         T0 = 1.0 + Rds * Idl;
         dT0_dVg = dRds_dVg * Idl + Rds * dIdl_dVg;
         dT0_dVd = Rds * dIdl_dVd;
@@ -758,7 +767,7 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dT1_dVg = (dVdsat_dVg - T2 * dEsatL_dVg / Leff) / Esat;
         dT1_dVd = (dVdsat_dVd - T2 * dEsatL_dVd / Leff) / Esat;
         dT1_dVb = (dVdsat_dVb - T2 * dEsatL_dVb / Leff) / Esat;
-        Cclm = FP * PvagTerm * T0 * T1 / (S.pclm * S.litl);
+        Cclm = FP * PvagTerm * T0 * T1 / (B4_S(pclm) * B4_S(litl));
         dCclm_dVg = Cclm * (dFP_dVg / FP + dPvagTerm_dVg / PvagTerm + dT0_dVg / T0 + dT1_dVg / T1);
         dCclm_dVb = Cclm * (dPvagTerm_dVb / PvagTerm + dT0_dVb / T0 + dT1_dVb / T1);
         dCclm_dVd = Cclm * (dPvagTerm_dVd / PvagTerm + dT0_dVd / T0 + dT1_dVd / T1);
@@ -774,7 +783,7 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     }
     // VADIBL (:1647-1694)
     double VADIBL, dVADIBL_dVg, dVADIBL_dVd, dVADIBL_dVb;
-    if (S.thetaRout > MIN_EXP) {
+    if (B4_S(thetaRout) > MIN_EXP) {
         T8 = Abulk * Vdsat;
         T0 = Vgst2Vtm * T8;
         dT0_dVg = Vgst2Vtm * Abulk * dVdsat_dVg + T8 + Vgst2Vtm * Vdsat * dAbulk_dVg;
@@ -785,23 +794,23 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dT1_dVb = Abulk * dVdsat_dVb + dAbulk_dVb * Vdsat;
         dT1_dVd = Abulk * dVdsat_dVd;
         T9 = T1 * T1;
-        T2 = S.thetaRout;
+        T2 = B4_S(thetaRout);
         VADIBL = (Vgst2Vtm - T0 / T1) / T2;
         dVADIBL_dVg = (1.0 - dT0_dVg / T1 + T0 * dT1_dVg / T9) / T2;
         dVADIBL_dVb = (-dT0_dVb / T1 + T0 * dT1_dVb / T9) / T2;
         dVADIBL_dVd = (-dT0_dVd / T1 + T0 * dT1_dVd / T9) / T2;
-        T7 = S.pdiblb * Vbseff;
+        T7 = B4_S(pdiblb) * Vbseff;
         if (T7 >= -0.9) {
             T3 = 1.0 / (1.0 + T7);
             VADIBL *= T3;
             dVADIBL_dVg *= T3;
-            dVADIBL_dVb = (dVADIBL_dVb - VADIBL * S.pdiblb) * T3;
+            dVADIBL_dVb = (dVADIBL_dVb - VADIBL * B4_S(pdiblb)) * T3;
             dVADIBL_dVd *= T3;
         } else {
             T4 = 1.0 / (0.8 + T7);
             T3 = (17.0 + 20.0 * T7) * T4;
             dVADIBL_dVg *= T3;
-            dVADIBL_dVb = dVADIBL_dVb * T3 - VADIBL * S.pdiblb * T4 * T4;
+            dVADIBL_dVb = dVADIBL_dVb * T3 - VADIBL * B4_S(pdiblb) * T4 * T4;
             dVADIBL_dVd *= T3;
             VADIBL *= T3;
         }
@@ -820,19 +829,19 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     const double dVa_dVd = dVasat_dVd + dVACLM_dVd;
     // VADITS (:1702-1723)
     double VADITS, dVADITS_dVg, dVADITS_dVd;
-    T0 = S.pditsd * Vds;
+    T0 = B4_S(pditsd) * Vds;
     if (T0 > EXP_THRESHOLD) {
         T1 = MAX_EXP;
         dT1_dVd = 0.0;
     } else {
         T1 = exp(T0);
-        dT1_dVd = T1 * S.pditsd;
+        dT1_dVd = T1 * B4_S(pditsd);
     }
-    if (S.pdits > MIN_EXP) {
-        T2 = 1.0 + M.pditsl * Leff;
-        VADITS = (1.0 + T2 * T1) / S.pdits;
+    if (B4_S(pdits) > MIN_EXP) {
+        T2 = 1.0 + B4_M(pditsl) * Leff;
+        VADITS = (1.0 + T2 * T1) / B4_S(pdits);
         dVADITS_dVg = VADITS * dFP_dVg;
-        dVADITS_dVd = FP * T2 * dT1_dVd / S.pdits;
+        dVADITS_dVd = FP * T2 * dT1_dVd / B4_S(pdits);
         VADITS *= FP;
     } else {
         VADITS = MAX_EXP;
@@ -841,16 +850,16 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     }
     // VASCBE (:1725-1744)
     double VASCBE = MAX_EXP, dVASCBE_dVg = 0.0, dVASCBE_dVd = 0.0, dVASCBE_dVb = 0.0;
-    if (S.pscbe2 > 0.0 && S.pscbe1 >= 0.0) {
-        if (diffVds > S.pscbe1 * S.litl / EXP_THRESHOLD) {
-            T0 = S.pscbe1 * S.litl / diffVds;
-            VASCBE = Leff * exp(T0) / S.pscbe2;
+    if (B4_S(pscbe2) > 0.0 && B4_S(pscbe1) >= 0.0) {
+        if (diffVds > B4_S(pscbe1) * B4_S(litl) / EXP_THRESHOLD) {
+            T0 = B4_S(pscbe1) * B4_S(litl) / diffVds;
+            VASCBE = Leff * exp(T0) / B4_S(pscbe2);
             T1 = T0 * VASCBE / diffVds;
             dVASCBE_dVg = T1 * dVdseff_dVg;
             dVASCBE_dVd = -T1 * (1.0 - dVdseff_dVd);
             dVASCBE_dVb = T1 * dVdseff_dVb;
         } else {
-            VASCBE = MAX_EXP * Leff / S.pscbe2;
+            VASCBE = MAX_EXP * Leff / B4_S(pscbe2);
             dVASCBE_dVg = dVASCBE_dVd = dVASCBE_dVb = 0.0;
         }
     }
@@ -883,13 +892,13 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     Idsa *= T9;
     // substrate current (:1782-1820)
     double Isub, Gbd, Gbb, Gbg;
-    tmp = S.alpha0 + S.alpha1 * Leff;
-    if ((tmp <= 0.0) || (S.beta0 <= 0.0)) {
+    tmp = B4_S(alpha0) + B4_S(alpha1) * Leff;
+    if ((tmp <= 0.0) || (B4_S(beta0) <= 0.0)) {
         Isub = Gbd = Gbb = Gbg = 0.0;
     } else {
         T2 = tmp / Leff;
-        if (diffVds > S.beta0 / EXP_THRESHOLD) {
-            T0 = -S.beta0 / diffVds;
+        if (diffVds > B4_S(beta0) / EXP_THRESHOLD) {
+            T0 = -B4_S(beta0) / diffVds;
             T1 = T2 * diffVds * exp(T0);
             T3 = T1 / diffVds * (T0 - 1.0);
             dT1_dVg = T3 * dVdseff_dVg;
@@ -945,57 +954,57 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         dvgs_eff_dvg = N.dvgs_eff_dvg;
         vgd_eff = N.vgd_eff;
         dvgd_eff_dvg = N.dvgd_eff_dvg;
-        T1 = (vds - vgs_eff - S.egidl) / T0g;
-        if ((S.agidl <= 0.0) || (S.bgidl <= 0.0) || (T1 <= 0.0) || (S.cgidl <= 0.0) || (vbd > 0.0)) {
+        T1 = (vds - vgs_eff - B4_S(egidl)) / T0g;
+        if ((B4_S(agidl) <= 0.0) || (B4_S(bgidl) <= 0.0) || (T1 <= 0.0) || (B4_S(cgidl) <= 0.0) || (vbd > 0.0)) {
             Igidl = Ggidld = Ggidlg = Ggidlb = 0.0;
         } else {
             dT1_dVd = 1.0 / T0g;
             dT1_dVg = -dvgs_eff_dvg * dT1_dVd;
-            T2 = S.bgidl / T1;
+            T2 = B4_S(bgidl) / T1;
             if (T2 < 100.0) {
-                Igidl = S.agidl * S.weffCJ * T1 * exp(-T2);
+                Igidl = B4_S(agidl) * B4_S(weffCJ) * T1 * exp(-T2);
                 T3 = Igidl * (1.0 + T2) / T1;
                 Ggidld = T3 * dT1_dVd;
                 Ggidlg = T3 * dT1_dVg;
             } else {
-                Igidl = S.agidl * S.weffCJ * 3.720075976e-44;
+                Igidl = B4_S(agidl) * B4_S(weffCJ) * 3.720075976e-44;
                 Ggidld = Igidl * dT1_dVd;
                 Ggidlg = Igidl * dT1_dVg;
                 Igidl *= T1;
             }
             T4 = vbd * vbd;
             T5 = -vbd * T4;
-            T6 = S.cgidl + T5;
+            T6 = B4_S(cgidl) + T5;
             T7 = T5 / T6;
-            T8 = 3.0 * S.cgidl * T4 / T6 / T6;
+            T8 = 3.0 * B4_S(cgidl) * T4 / T6 / T6;
             Ggidld = Ggidld * T7 + Igidl * T8;
             Ggidlg = Ggidlg * T7;
             Ggidlb = -Igidl * T8;
             Igidl *= T7;
         }
-        T1 = (-vds - vgd_eff - S.egisl) / T0g;
-        if ((S.agisl <= 0.0) || (S.bgisl <= 0.0) || (T1 <= 0.0) || (S.cgisl <= 0.0) || (vbs > 0.0)) {
+        T1 = (-vds - vgd_eff - B4_S(egisl)) / T0g;
+        if ((B4_S(agisl) <= 0.0) || (B4_S(bgisl) <= 0.0) || (T1 <= 0.0) || (B4_S(cgisl) <= 0.0) || (vbs > 0.0)) {
             Igisl = Ggisls = Ggislg = Ggislb = 0.0;
         } else {
             dT1_dVd = 1.0 / T0g;
             dT1_dVg = -dvgd_eff_dvg * dT1_dVd;
-            T2 = S.bgisl / T1;
+            T2 = B4_S(bgisl) / T1;
             if (T2 < 100.0) {
-                Igisl = S.agisl * S.weffCJ * T1 * exp(-T2);
+                Igisl = B4_S(agisl) * B4_S(weffCJ) * T1 * exp(-T2);
                 T3 = Igisl * (1.0 + T2) / T1;
                 Ggisls = T3 * dT1_dVd;
                 Ggislg = T3 * dT1_dVg;
             } else {
-                Igisl = S.agisl * S.weffCJ * 3.720075976e-44;
+                Igisl = B4_S(agisl) * B4_S(weffCJ) * 3.720075976e-44;
                 Ggisls = Igisl * dT1_dVd;
                 Ggislg = Igisl * dT1_dVg;
                 Igisl *= T1;
             }
             T4 = vbs * vbs;
             T5 = -vbs * T4;
-            T6 = S.cgisl + T5;
+            T6 = B4_S(cgisl) + T5;
             T7 = T5 / T6;
-            T8 = 3.0 * S.cgisl * T4 / T6 / T6;
+            T8 = 3.0 * B4_S(cgisl) * T4 / T6 / T6;
             Ggisls = Ggisls * T7 + Igisl * T8;
             Ggislg = Ggislg * T7;
             Ggislb = -Igisl * T8;
@@ -1012,31 +1021,31 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     N.ggislb = Ggislb;
     // gate tunnelling: igcmod and igbmod are 0
     double cd = cdrain;
-    if (I.nf != 1.0) {  // :2591-2635
-        cd *= I.nf;
-        N.gds *= I.nf;
-        N.gm *= I.nf;
-        N.gmbs *= I.nf;
-        N.IdovVds *= I.nf;
-        N.gbbs *= I.nf;
-        N.gbgs *= I.nf;
-        N.gbds *= I.nf;
-        N.csub *= I.nf;
-        N.Igidl *= I.nf;
-        N.ggidld *= I.nf;
-        N.ggidlg *= I.nf;
-        N.ggidlb *= I.nf;
-        N.Igisl *= I.nf;
-        N.ggisls *= I.nf;
-        N.ggislg *= I.nf;
-        N.ggislb *= I.nf;
+    if (B4_I(nf) != 1.0) {  // :2591-2635
+        cd *= B4_I(nf);
+        N.gds *= B4_I(nf);
+        N.gm *= B4_I(nf);
+        N.gmbs *= B4_I(nf);
+        N.IdovVds *= B4_I(nf);
+        N.gbbs *= B4_I(nf);
+        N.gbgs *= B4_I(nf);
+        N.gbds *= B4_I(nf);
+        N.csub *= B4_I(nf);
+        N.Igidl *= B4_I(nf);
+        N.ggidld *= B4_I(nf);
+        N.ggidlg *= B4_I(nf);
+        N.ggidlb *= B4_I(nf);
+        N.Igisl *= B4_I(nf);
+        N.ggisls *= B4_I(nf);
+        N.ggislg *= B4_I(nf);
+        N.ggislb *= B4_I(nf);
     }
     N.ggidls = -(N.ggidld + N.ggidlg + N.ggidlb);
     N.ggisld = -(N.ggisls + N.ggislg + N.ggislb);
     N.cd = cd;
 
     // noise-analysis inversion charge, tnoimod 0 (:2643-2667): reassigns Abulk, Vdsat, Vdseff
-    Abulk = Abulk0 * S.abulkCVfactor;
+    Abulk = Abulk0 * B4_S(abulkCVfactor);
     Vdsat = Vgsteff / Abulk;
     T0 = Vdsat - Vds - DELTA_4;
     T1 = sqrt(T0 * T0 + 4.0 * DELTA_4 * Vdsat);
@@ -1052,7 +1061,7 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
     T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1.0e-20);
     T2 = Vdseff / T1;
     T3 = T0 * T2;
-    N.qinv = Coxeff * S.weffCV * I.nf * S.leffCV * (Vgsteff - 0.5 * T0 + Abulk * T3);
+    N.qinv = Coxeff * B4_S(weffCV) * B4_I(nf) * B4_S(leffCV) * (Vgsteff - 0.5 * T0 + Abulk * T3);
 
     // C-V, capmod 2 with cvchargemod 0 and the 40/60 partition of xpart 0 (:2672-3476)
     double qgate, qbulk, qdrn, qsrc;
@@ -1062,17 +1071,17 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
             VbseffCV = Vbseff;
             dVbseffCV_dVb = 1.0;
         } else {
-            VbseffCV = S.phi - Phis;
+            VbseffCV = B4_S(phi) - Phis;
             dVbseffCV_dVb = -dPhis_dVb;
         }
-        const double CoxWL = D.coxe * S.weffCV * S.leffCV * I.nf;
+        const double CoxWL = B4_D(coxe) * B4_S(weffCV) * B4_S(leffCV) * B4_I(nf);
         double VgsteffC, dVgsteffC_dVg, dVgsteffC_dVd, dVgsteffC_dVb;  // VgsteffCV (:2974-3005)
         {
-            const double noff = n * S.noff;
-            const double dnoff_dVd = S.noff * dn_dVd;
-            const double dnoff_dVb = S.noff * dn_dVb;
+            const double noff = n * B4_S(noff);
+            const double dnoff_dVd = B4_S(noff) * dn_dVd;
+            const double dnoff_dVb = B4_S(noff) * dn_dVb;
             T0 = Vtm * noff;
-            const double voffcv = S.voffcv;
+            const double voffcv = B4_S(voffcv);
             const double VgstNVt = (Vgst - voffcv) / T0;
             if (VgstNVt > EXP_THRESHOLD) {
                 VgsteffC = Vgst - voffcv;
@@ -1095,40 +1104,40 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
             }
         }
         // charge-thickness model (:3215-3474)
-        double V3 = I.vfbzb - Vgs_eff + VbseffCV - DELTA_3;
-        if (I.vfbzb <= 0.0)
-            T0 = sqrt(V3 * V3 - 4.0 * DELTA_3 * I.vfbzb);
+        double V3 = B4_I(vfbzb) - Vgs_eff + VbseffCV - DELTA_3;
+        if (B4_I(vfbzb) <= 0.0)
+            T0 = sqrt(V3 * V3 - 4.0 * DELTA_3 * B4_I(vfbzb));
         else
-            T0 = sqrt(V3 * V3 + 4.0 * DELTA_3 * I.vfbzb);
+            T0 = sqrt(V3 * V3 + 4.0 * DELTA_3 * B4_I(vfbzb));
         T1 = 0.5 * (1.0 + V3 / T0);
-        const double Vfbeff = I.vfbzb - 0.5 * (V3 + T0);
+        const double Vfbeff = B4_I(vfbzb) - 0.5 * (V3 + T0);
         const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
         const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
-        const double Cox = I.coxp;
-        double Tox = 1.0e8 * I.toxp;
-        T0 = (Vgs_eff - VbseffCV - I.vfbzb) / Tox;
+        const double Cox = B4_I(coxp);
+        double Tox = 1.0e8 * B4_I(toxp);
+        T0 = (Vgs_eff - VbseffCV - B4_I(vfbzb)) / Tox;
         dT0_dVg = dVgs_eff_dVg / Tox;
         dT0_dVb = -dVbseffCV_dVb / Tox;
-        tmp = T0 * S.acde;
+        tmp = T0 * B4_S(acde);
         double Tcen1, dTcen_dVb, dTcen_dVg;
         if ((-EXP_THRESHOLD < tmp) && (tmp < EXP_THRESHOLD)) {
-            Tcen1 = S.ldeb * exp(tmp);
-            dTcen_dVg = S.acde * Tcen1;
+            Tcen1 = B4_S(ldeb) * exp(tmp);
+            dTcen_dVg = B4_S(acde) * Tcen1;
             dTcen_dVb = dTcen_dVg * dT0_dVb;
             dTcen_dVg *= dT0_dVg;
         } else if (tmp <= -EXP_THRESHOLD) {
-            Tcen1 = S.ldeb * MIN_EXP;
+            Tcen1 = B4_S(ldeb) * MIN_EXP;
             dTcen_dVg = 0.0;
             dTcen_dVb = 0.0;
         } else {
-            Tcen1 = S.ldeb * MAX_EXP;
+            Tcen1 = B4_S(ldeb) * MAX_EXP;
             dTcen_dVg = 0.0;
             dTcen_dVb = 0.0;
         }
-        const double LINK = 1.0e-3 * I.toxp;
-        V3 = S.ldeb - Tcen1 - LINK;
-        const double V4 = sqrt(V3 * V3 + 4.0 * LINK * S.ldeb);
-        const double Tcen2 = S.ldeb - 0.5 * (V3 + V4);
+        const double LINK = 1.0e-3 * B4_I(toxp);
+        V3 = B4_S(ldeb) - Tcen1 - LINK;
+        const double V4 = sqrt(V3 * V3 + 4.0 * LINK * B4_S(ldeb));
+        const double Tcen2 = B4_S(ldeb) - 0.5 * (V3 + V4);
         T1 = 0.5 * (1.0 + V3 / V4);
         dTcen_dVg *= T1;
         dTcen_dVb *= T1;
@@ -1139,35 +1148,35 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         double dCoxeff_dVg_ = T2 * T2 * T3;
         double dCoxeff_dVbC = dCoxeff_dVg_ * dTcen_dVb;
         double dCoxeff_dVgC = dCoxeff_dVg_ * dTcen_dVg;
-        double CoxWLcen = CoxWL * CoxeffC / D.coxe;
-        const double Qac0 = CoxWLcen * (Vfbeff - I.vfbzb);
+        double CoxWLcen = CoxWL * CoxeffC / B4_D(coxe);
+        const double Qac0 = CoxWLcen * (Vfbeff - B4_I(vfbzb));
         double QovCox = Qac0 / CoxeffC;
         const double dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVgC;
         const double dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVbC;
-        T0 = 0.5 * S.k1ox;
+        T0 = 0.5 * B4_S(k1ox);
         T3 = Vgs_eff - Vfbeff - VbseffCV - VgsteffC;
-        if (S.k1ox == 0.0) {
+        if (B4_S(k1ox) == 0.0) {
             T1 = 0.0;
             T2 = 0.0;
         } else if (T3 < 0.0) {
-            T1 = T0 + T3 / S.k1ox;
+            T1 = T0 + T3 / B4_S(k1ox);
             T2 = CoxWLcen;
         } else {
             T1 = sqrt(T0 * T0 + T3);
             T2 = CoxWLcen * T0 / T1;
         }
-        const double Qsub0 = CoxWLcen * S.k1ox * (T1 - T0);
+        const double Qsub0 = CoxWLcen * B4_S(k1ox) * (T1 - T0);
         QovCox = Qsub0 / CoxeffC;
         const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteffC_dVg) + QovCox * dCoxeff_dVgC;
         const double dQsub0_dVd = -T2 * dVgsteffC_dVd;
         const double dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteffC_dVb) + QovCox * dCoxeff_dVbC;
         double DenomiC;
-        if (S.k1ox <= 0.0) {
-            DenomiC = 0.25 * S.moin * Vtm;
-            T0 = 0.5 * S.sqrtPhi;
+        if (B4_S(k1ox) <= 0.0) {
+            DenomiC = 0.25 * B4_S(moin) * Vtm;
+            T0 = 0.5 * B4_S(sqrtPhi);
         } else {
-            DenomiC = S.moin * Vtm * S.k1ox * S.k1ox;
-            T0 = S.k1ox * S.sqrtPhi;
+            DenomiC = B4_S(moin) * Vtm * B4_S(k1ox) * B4_S(k1ox);
+            T0 = B4_S(k1ox) * B4_S(sqrtPhi);
         }
         T1 = 2.0 * T0 + VgsteffC;
         const double DeltaPhi = Vtm * log(1.0 + T1 * VgsteffC / DenomiC);
@@ -1178,11 +1187,11 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         const double VgDP = 0.5 * (T0 + T1);
         const double dVgDP_dVg = 0.5 * (dT0_dVg + (T0 * dT0_dVg + 0.002) / T1);
         Tox += Tox;
-        T0 = (VgsteffC + I.vtfbphi2) / Tox;
-        tmp = exp(M.bdos * 0.7 * log(T0));
+        T0 = (VgsteffC + B4_I(vtfbphi2)) / Tox;
+        tmp = exp(B4_M(bdos) * 0.7 * log(T0));
         T1 = 1.0 + tmp;
-        T2 = M.bdos * 0.7 * tmp / (T0 * Tox);
-        const double Tcen3 = M.ados * 1.9e-9 / T1;
+        T2 = B4_M(bdos) * 0.7 * tmp / (T0 * Tox);
+        const double Tcen3 = B4_M(ados) * 1.9e-9 / T1;
         double dTcen3_dVg = -Tcen3 * T2 / T1;
         const double dTcen3_dVd = dTcen3_dVg * dVgsteffC_dVd;
         const double dTcen3_dVb = dTcen3_dVg * dVgsteffC_dVb;
@@ -1195,9 +1204,9 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         const double dCoxeff_dVd = dCoxeff_dVg_ * dTcen3_dVd;
         dCoxeff_dVbC = dCoxeff_dVg_ * dTcen3_dVb;
         dCoxeff_dVgC = dCoxeff_dVg_ * dTcen3_dVg;
-        CoxWLcen = CoxWL * CoxeffC / D.coxe;
-        const double AbulkCV = Abulk0 * S.abulkCVfactor;
-        const double dAbulkCV_dVb = S.abulkCVfactor * dAbulk0_dVb;
+        CoxWLcen = CoxWL * CoxeffC / B4_D(coxe);
+        const double AbulkCV = Abulk0 * B4_S(abulkCVfactor);
+        const double dAbulkCV_dVb = B4_S(abulkCVfactor) * dAbulk0_dVb;
         const double VdsatCV = VgDP / AbulkCV;
         T0 = VdsatCV - Vds - DELTA_4;
         dT0_dVg = dVgDP_dVg / AbulkCV;
@@ -1300,12 +1309,12 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
 
     // junction capacitances (:3499-3614)
     {
-        const double czbd = D.DunitAreaTempJctCap * I.Adeff;
-        const double czbs = D.SunitAreaTempJctCap * I.Aseff;
-        const double czbdsw = D.DunitLengthSidewallTempJctCap * I.Pdeff;
-        const double czbdswg = D.DunitLengthGateSidewallTempJctCap * S.weffCJ * I.nf;
-        const double czbssw = D.SunitLengthSidewallTempJctCap * I.Pseff;
-        const double czbsswg = D.SunitLengthGateSidewallTempJctCap * S.weffCJ * I.nf;
+        const double czbd = B4_D(DunitAreaTempJctCap) * B4_I(Adeff);
+        const double czbs = B4_D(SunitAreaTempJctCap) * B4_I(Aseff);
+        const double czbdsw = B4_D(DunitLengthSidewallTempJctCap) * B4_I(Pdeff);
+        const double czbdswg = B4_D(DunitLengthGateSidewallTempJctCap) * B4_S(weffCJ) * B4_I(nf);
+        const double czbssw = B4_D(SunitLengthSidewallTempJctCap) * B4_I(Pseff);
+        const double czbsswg = B4_D(SunitLengthGateSidewallTempJctCap) * B4_S(weffCJ) * B4_I(nf);
         auto side = [](double v, double cz, double Phi, double MJ, double& q, double& c) {
             const double arg = 1.0 - v / Phi;
             const double sarg = (MJ == 0.5) ? 1.0 / sqrt(arg) : exp(-MJ * log(arg));
@@ -1318,12 +1327,12 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         } else if (vbs_jct < 0.0) {
             N.qbs = 0.0;
             N.capbs = 0.0;
-            if (czbs > 0.0) side(vbs_jct, czbs, D.PhiBS, M.mjs, N.qbs, N.capbs);
-            if (czbssw > 0.0) side(vbs_jct, czbssw, D.PhiBSWS, M.mjsws, N.qbs, N.capbs);
-            if (czbsswg > 0.0) side(vbs_jct, czbsswg, D.PhiBSWGS, M.mjswgs, N.qbs, N.capbs);
+            if (czbs > 0.0) side(vbs_jct, czbs, B4_D(PhiBS), B4_M(mjs), N.qbs, N.capbs);
+            if (czbssw > 0.0) side(vbs_jct, czbssw, B4_D(PhiBSWS), B4_M(mjsws), N.qbs, N.capbs);
+            if (czbsswg > 0.0) side(vbs_jct, czbsswg, B4_D(PhiBSWGS), B4_M(mjswgs), N.qbs, N.capbs);
         } else {
             T0 = czbs + czbssw + czbsswg;
-            T1 = vbs_jct * (czbs * M.mjs / D.PhiBS + czbssw * M.mjsws / D.PhiBSWS + czbsswg * M.mjswgs / D.PhiBSWGS);
+            T1 = vbs_jct * (czbs * B4_M(mjs) / B4_D(PhiBS) + czbssw * B4_M(mjsws) / B4_D(PhiBSWS) + czbsswg * B4_M(mjswgs) / B4_D(PhiBSWGS));
             N.qbs = vbs_jct * (T0 + 0.5 * T1);
             N.capbs = T0 + T1;
         }
@@ -1333,12 +1342,12 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         } else if (vbd_jct < 0.0) {
             N.qbd = 0.0;
             N.capbd = 0.0;
-            if (czbd > 0.0) side(vbd_jct, czbd, D.PhiBD, M.mjd, N.qbd, N.capbd);
-            if (czbdsw > 0.0) side(vbd_jct, czbdsw, D.PhiBSWD, M.mjswd, N.qbd, N.capbd);
-            if (czbdswg > 0.0) side(vbd_jct, czbdswg, D.PhiBSWGD, M.mjswgd, N.qbd, N.capbd);
+            if (czbd > 0.0) side(vbd_jct, czbd, B4_D(PhiBD), B4_M(mjd), N.qbd, N.capbd);
+            if (czbdsw > 0.0) side(vbd_jct, czbdsw, B4_D(PhiBSWD), B4_M(mjswd), N.qbd, N.capbd);
+            if (czbdswg > 0.0) side(vbd_jct, czbdswg, B4_D(PhiBSWGD), B4_M(mjswgd), N.qbd, N.capbd);
         } else {
             T0 = czbd + czbdsw + czbdswg;
-            T1 = vbd_jct * (czbd * M.mjd / D.PhiBD + czbdsw * M.mjswd / D.PhiBSWD + czbdswg * M.mjswgd / D.PhiBSWGD);
+            T1 = vbd_jct * (czbd * B4_M(mjd) / B4_D(PhiBD) + czbdsw * B4_M(mjswd) / B4_D(PhiBSWD) + czbdswg * B4_M(mjswgd) / B4_D(PhiBSWGD));
             N.qbd = vbd_jct * (T0 + 0.5 * T1);
             N.capbd = T0 + T1;
         }
@@ -1364,23 +1373,23 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         T0 = vgdx + DELTA_1;
         T1 = sqrt(T0 * T0 + 4.0 * DELTA_1);
         T2 = 0.5 * (T0 - T1);
-        T3 = S.weffCV * S.cgdl;
-        T4 = sqrt(1.0 - 4.0 * T2 / S.ckappad);
-        cgdo = S.cgdo + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        qgdo = (S.cgdo + T3) * vgdx - T3 * (T2 + 0.5 * S.ckappad * (T4 - 1.0));
+        T3 = B4_S(weffCV) * B4_S(cgdl);
+        T4 = sqrt(1.0 - 4.0 * T2 / B4_S(ckappad));
+        cgdo = B4_S(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        qgdo = (B4_S(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * B4_S(ckappad) * (T4 - 1.0));
         T0 = vgsx + DELTA_1;
         T1 = sqrt(T0 * T0 + 4.0 * DELTA_1);
         T2 = 0.5 * (T0 - T1);
-        T3 = S.weffCV * S.cgsl;
-        T4 = sqrt(1.0 - 4.0 * T2 / S.ckappas);
-        cgso = S.cgso + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        qgso = (S.cgso + T3) * vgsx - T3 * (T2 + 0.5 * S.ckappas * (T4 - 1.0));
+        T3 = B4_S(weffCV) * B4_S(cgsl);
+        T4 = sqrt(1.0 - 4.0 * T2 / B4_S(ckappas));
+        cgso = B4_S(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        qgso = (B4_S(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * B4_S(ckappas) * (T4 - 1.0));
     }
-    if (I.nf != 1.0) {
-        cgdo *= I.nf;
-        cgso *= I.nf;
-        qgdo *= I.nf;
-        qgso *= I.nf;
+    if (B4_I(nf) != 1.0) {
+        cgdo *= B4_I(nf);
+        cgso *= B4_I(nf);
+        qgdo *= B4_I(nf);
+        qgso *= B4_I(nf);
     }
     N.cgdo = cgdo;
     N.qgdo = qgdo;
@@ -1395,7 +1404,7 @@ __host__ __device__ inline void op_eval(const B4Par& P, const B4Op& G, double vd
         qdrn -= qgdo;
     } else {
         qsrc = qdrn - qgso;
-        const double qgb = S.cgbo * vgb;
+        const double qgb = B4_S(cgbo) * vgb;
         qgate += qgdo + qgso + qgb;
         qbulk -= qgb;
         qdrn = -(qgate + qbulk + qsrc);
@@ -1415,19 +1424,19 @@ struct B4Tran {
 };
 // `O` is the committed operating point (its charges and their currents); the charge currents of `N` are
 // filled in (cqb, cqg, cqd: Backward Euler, i = dq / dt, analysis.rs:420-428).
-__host__ __device__ inline void tran_eval(const B4Par& P, const B4Op& O, B4Op& N, double dt, B4Tran& T) {
-    const B4S& S = P.s;
+template <class PV>
+__host__ __device__ inline void tran_eval(const PV& P, const B4Op& O, B4Op& N, double dt, B4Tran& T) {
     const double ag0 = 1.0 / dt;
     const double cgdo = N.cgdo, cgso = N.cgso;
     T = B4Tran{};
     if (N.mode > 0) {
-        T.gcggb = (N.cggb + cgdo + cgso + S.cgbo) * ag0;
+        T.gcggb = (N.cggb + cgdo + cgso + B4_S(cgbo)) * ag0;
         T.gcgdb = (N.cgdb - cgdo) * ag0;
         T.gcgsb = (N.cgsb - cgso) * ag0;
         T.gcgbb = -(T.gcggb + T.gcgdb + T.gcgsb);
         T.gcdgb = (N.cdgb - cgdo) * ag0;
         T.gcsgb = -(N.cggb + N.cbgb + N.cdgb + cgso) * ag0;
-        T.gcbgb = (N.cbgb - S.cgbo) * ag0;
+        T.gcbgb = (N.cbgb - B4_S(cgbo)) * ag0;
         T.gcddb = (N.cddb + N.capbd + cgdo) * ag0;
         T.gcdsb = N.cdsb * ag0;
         T.gcsdb = -(N.cgdb + N.cbdb + N.cddb) * ag0;
@@ -1440,13 +1449,13 @@ __host__ __device__ inline void tran_eval(const B4Par& P, const B4Op& O, B4Op& N
         T.sxpart = 0.6;
         T.dxpart = 0.4;
     } else {
-        T.gcggb = (N.cggb + cgdo + cgso + S.cgbo) * ag0;
+        T.gcggb = (N.cggb + cgdo + cgso + B4_S(cgbo)) * ag0;
         T.gcgdb = (N.cgsb - cgdo) * ag0;
         T.gcgsb = (N.cgdb - cgso) * ag0;
         T.gcgbb = -(T.gcggb + T.gcgdb + T.gcgsb);
         T.gcdgb = -(N.cggb + N.cbgb + N.cdgb + cgdo) * ag0;
         T.gcsgb = (N.cdgb - cgso) * ag0;
-        T.gcbgb = (N.cbgb - S.cgbo) * ag0;
+        T.gcbgb = (N.cbgb - B4_S(cgbo)) * ag0;
         T.gcddb = (N.capbd + cgdo - (N.cgsb + N.cbsb + N.cdsb)) * ag0;
         T.gcdsb = -(N.cgdb + N.cbdb + N.cddb) * ag0;
         T.gcsdb = N.cdsb * ag0;
@@ -1487,9 +1496,9 @@ __host__ __device__ constexpr int b4_col(int k) {
 }
 
 // `N` is the load's operating point; `T` its charge companions (all zero in an operating point).
-template <class Put>
-__host__ __device__ inline void stamp(const B4Par& P, const B4Op& N, const B4Tran& T, Put&& put) {
-    const double p = P.m.p;
+template <class PV, class Put>
+__host__ __device__ inline void stamp(const PV& P, const B4Op& N, const B4Tran& T, Put&& put) {
+    const double p = B4_M(p);
     double ceqdrn, ceqbd, ceqbs, Gm, Gmbs, FwdSum, RevSum;
     double gbbdp, gbbsp, gbdpg, gbdpdp, gbdpb, gbdpsp, gbspg, gbspdp, gbspb, gbspsp;
     if (N.mode >= 0) {

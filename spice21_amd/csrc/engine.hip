@@ -71,6 +71,7 @@ struct GroupHost {
     Topology topo;
     std::unique_ptr<DevMem> d_topo, d_par, d_state, d_out;
     std::unique_ptr<DevMem> d_b4;  // BSIM4 parameters and states
+    int b4_var_fields = 0;          // BSIM4 parameter fields stored per instance
     std::unique_ptr<DevMem> d_acop;  // Mos1 small-signal operating points captured for AC
     std::unique_ptr<DevMem> d_ac;    // AC: per-frequency records, runs' last iterates, runs' outcomes
     GroupGPU gg{};
@@ -424,7 +425,12 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const bool latency = total <= LATENCY_INST || (b->precision == S21_MIXED &&
                                                    total <= (int64_t)LATENCY_INST_PER_SIMD * 4 * std::max(cus, 1));
-    const bool every_shared = b->hub == 1 || (b->hub == 0 && latency);
+    // A group with BSIM4 devices, whose evaluation dominates an iteration, evaluates each class in one
+    // round: with colour rounds the devices of one node (drain and gate) would take several (four in an
+    // inverter chain), most lanes idle in each.
+    bool b4 = false;
+    for (const s21_comp& c : g.comps) b4 = b4 || c.type == S21_COMP_BSIM4;
+    const bool every_shared = b->hub == 1 || (b->hub == 0 && (latency || b4));
     host_prepare(b->opts, g, tr, hp, ac, every_shared && !ac ? 1 : HUB_CONTRIB);
     const Topology& t = g.topo;
     const int NI = g.n_inst;
@@ -530,23 +536,46 @@ static void prepare_group(s21_batch* b, GroupHost& g, const s21_tran_options* tr
     if (!par.empty()) HIPCHECK(hipMemcpy(g.d_par->p, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
     g.gg.par = (const double*)g.d_par->p;
 
-    // ---- BSIM4: parameters, then the last load's and the committed points and the limiting fields ----
+    // ---- BSIM4: the field map, shared and per-instance parameters, then the state (last) ----
     {
         const size_t nb = (size_t)hp.n_b4;
-        const size_t s_par = ((hp.b4.size() * sizeof(b4::B4Par) + 255) / 256) * 256,
-                     s_op = ((NI * nb * sizeof(b4::B4Op) + 255) / 256) * 256, s_prev = pad(5 * NI * nb);
+        std::vector<int> map(B4_PAR_D, -1);
+        int n_var = 0;
+        const size_t ni_b4 = hp.b4_per_inst ? (size_t)NI : 1;
+        auto field = [&](size_t i, size_t j, int k) { return ((const double*)&hp.b4[i * nb + j])[k]; };
+        for (int k = 0; k < B4_PAR_D; k++) {
+            bool same = true;
+            for (size_t i = 1; i < ni_b4 && same; i++)
+                for (size_t j = 0; j < nb && same; j++) {
+                    const double a = field(i, j, k), c = field(0, j, k);
+                    same = memcmp(&a, &c, sizeof(double)) == 0;
+                }
+            if (!same) map[k] = n_var++;
+        }
+        std::vector<double> sh((size_t)B4_PAR_D * nb), var((size_t)n_var * NI * nb);
+        for (int k = 0; k < B4_PAR_D; k++)
+            for (size_t j = 0; j < nb; j++) {
+                sh[(size_t)k * nb + j] = field(0, j, k);
+                if (map[k] >= 0)
+                    for (size_t i = 0; i < (size_t)NI; i++) var[((size_t)map[k] * NI + i) * nb + j] = field(i, j, k);
+            }
+        const size_t s_map = ((B4_PAR_D * sizeof(int) + 255) / 256) * 256, s_sh = pad(sh.size()),
+                     s_var = pad(var.size()), s_st = pad((size_t)B4_ST_F * NI * nb);
         g.d_b4.reset(new DevMem());
-        g.d_b4->alloc(nb ? s_par + 2 * s_op + s_prev : 0);
+        g.d_b4->alloc(nb ? s_map + s_sh + s_var + s_st : 0);
         char* p = (char*)g.d_b4->p;
         if (nb) {
-            HIPCHECK(hipMemcpy(p, hp.b4.data(), hp.b4.size() * sizeof(b4::B4Par), hipMemcpyHostToDevice));
-            HIPCHECK(hipMemsetAsync(p + s_par, 0, 2 * s_op + s_prev, b->stream));
+            HIPCHECK(hipMemcpy(p, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(p + s_map, sh.data(), sh.size() * sizeof(double), hipMemcpyHostToDevice));
+            if (!var.empty())
+                HIPCHECK(hipMemcpy(p + s_map + s_sh, var.data(), var.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemsetAsync(p + s_map + s_sh + s_var, 0, s_st, b->stream));
         }
-        g.gg.b4_par = nb ? (const b4::B4Par*)p : nullptr;
-        g.gg.b4_istr = hp.b4_per_inst ? (int)nb : 0;
-        g.gg.b4_guess = nb ? (b4::B4Op*)(p + s_par) : nullptr;
-        g.gg.b4_op = nb ? (b4::B4Op*)(p + s_par + s_op) : nullptr;
-        g.gg.b4_prev = nb ? (double*)(p + s_par + 2 * s_op) : nullptr;
+        g.gg.b4_map = nb ? (const int*)p : nullptr;
+        g.gg.b4_sh = nb ? (const double*)(p + s_map) : nullptr;
+        g.gg.b4_var = nb ? (const double*)(p + s_map + s_sh) : nullptr;
+        g.gg.b4_st = nb ? (double*)(p + s_map + s_sh + s_var) : nullptr;
+        g.b4_var_fields = n_var;
     }
 
     // ---- state (zero-initialised: Default op points, comps/mos.rs:524-546) ----
@@ -639,8 +668,8 @@ static void prepare_all(s21_batch* b, const s21_tran_options* tr) {
         if (g.prepared == key) {
             HIPCHECK(hipMemsetAsync(g.d_state->p, 0, g.d_state->bytes, b->stream));
             HIPCHECK(hipMemsetAsync(g.d_out->p, 0, g.d_out->bytes, b->stream));
-            if (g.gg.b4_guess) {  // the parameters stay; every state after them restarts from zero
-                char* p0 = (char*)g.gg.b4_guess;
+            if (g.gg.b4_st) {  // the parameters stay; the state after them restarts from zero
+                char* p0 = (char*)g.gg.b4_st;
                 HIPCHECK(hipMemsetAsync(p0, 0, g.d_b4->bytes - (size_t)(p0 - (char*)g.d_b4->p), b->stream));
             }
         } else {
@@ -909,7 +938,7 @@ struct InstState {
     // its committed point (the B4Op's doubles)
     std::vector<std::vector<double>> b4;
 };
-constexpr int B4_PREV_F = 5, B4_OP_D = (int)(sizeof(b4::B4Op) / sizeof(double));
+constexpr int B4_PREV_F = 5, B4_QC_F = 3;
 static void dev_maps(const Topology& t, std::vector<int>& m1, std::vector<int>& d, std::vector<int>& lin) {
     m1.assign(t.dev_comp.begin() + t.cls_start[CLS_MOS1], t.dev_comp.begin() + t.cls_start[CLS_MOS1T + 1]);
     d.assign(t.dev_comp.begin() + t.cls_start[CLS_DIODE], t.dev_comp.begin() + t.cls_start[CLS_DIODE + 1]);
@@ -929,15 +958,12 @@ static InstState read_state(const GroupHost& g, int i) {
     const Topology& t = g.topo;
     const size_t nb = (size_t)(t.cls_start[CLS_BSIM4 + 1] - t.cls_start[CLS_BSIM4]);
     if (nb) {
-        std::vector<double> prev((size_t)B4_PREV_F * NI * nb);
-        std::vector<b4::B4Op> op(nb);
-        HIPCHECK(hipMemcpy(prev.data(), g.gg.b4_prev, prev.size() * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(op.data(), g.gg.b4_op + (size_t)i * nb, nb * sizeof(b4::B4Op), hipMemcpyDeviceToHost));
+        std::vector<double> st((size_t)B4_ST_F * NI * nb);
+        HIPCHECK(hipMemcpy(st.data(), g.gg.b4_st, st.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < nb; k++) {
             std::vector<double>& v = S.b4[t.dev_comp[t.cls_start[CLS_BSIM4] + k]];
-            for (int f = 0; f < B4_PREV_F; f++) v.push_back(prev[f * NI * nb + i * nb + k]);
-            const double* od = (const double*)&op[k];
-            v.insert(v.end(), od, od + B4_OP_D);
+            for (int f = 0; f < B4_PREV_F; f++) v.push_back(st[(B4_ST_PREV + f) * NI * nb + i * nb + k]);
+            for (int f = 0; f < B4_QC_F; f++) v.push_back(st[(B4_ST_QC + f) * NI * nb + i * nb + k]);
         }
     }
     std::vector<double> tmp;
@@ -980,24 +1006,18 @@ static void write_state(GroupHost& h, const InstState& S) {
     wr(h.gg.d_state, D_STATE_F, nd, S.d, d);
     const Topology& t = h.topo;
     const size_t nb = (size_t)(t.cls_start[CLS_BSIM4 + 1] - t.cls_start[CLS_BSIM4]);
-    if (nb) {  // the load resumes from the limiting fields it started from, against the committed point
-        std::vector<double> prev((size_t)B4_PREV_F * nb, 0.0);
-        std::vector<b4::B4Op> guess(nb), op(nb);
+    if (nb) {  // the load resumes from the limiting fields it started from, against the committed charges
+        std::vector<double> st((size_t)B4_ST_F * nb, 0.0);
         for (size_t k = 0; k < nb; k++) {
             const std::vector<double>& v = S.b4[t.dev_comp[t.cls_start[CLS_BSIM4] + k]];
-            if (v.size() != (size_t)(B4_PREV_F + B4_OP_D)) fail(S21_ERR, "internal: BSIM4 state of the wrong size");
-            memset(&guess[k], 0, sizeof(b4::B4Op));
-            guess[k].vds = v[0];
-            guess[k].vgs = v[1];
-            guess[k].vbs = v[2];
-            guess[k].vbd = v[3];
-            guess[k].von = v[4];
-            for (int f = 0; f < B4_PREV_F; f++) prev[f * nb + k] = v[f];
-            memcpy(&op[k], v.data() + B4_PREV_F, sizeof(b4::B4Op));
+            if (v.size() != (size_t)(B4_PREV_F + B4_QC_F)) fail(S21_ERR, "internal: BSIM4 state of the wrong size");
+            for (int f = 0; f < B4_PREV_F; f++) {
+                st[(B4_ST_LIM + f) * nb + k] = v[f];
+                st[(B4_ST_PREV + f) * nb + k] = v[f];
+            }
+            for (int f = 0; f < B4_QC_F; f++) st[(B4_ST_QC + f) * nb + k] = v[B4_PREV_F + f];
         }
-        HIPCHECK(hipMemcpy(h.gg.b4_guess, guess.data(), nb * sizeof(b4::B4Op), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(h.gg.b4_op, op.data(), nb * sizeof(b4::B4Op), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(h.gg.b4_prev, prev.data(), prev.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(h.gg.b4_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     if (nl) {
         tmp.assign(nl, 0.0);
@@ -1715,8 +1735,8 @@ int s21_batch_op_noalloc(s21_batch* b, double* x, int32_t* status, int64_t* iter
         // call's contents, and the caller's buffers hold whatever they held.
         zero_async(base + st_off[0], off - st_off[0], b->stream);
         for (const GroupHost& g : b->groups)  // BSIM4 points stay in the batch's own buffer (no allocation)
-            if (g.gg.b4_guess) {
-                char* p0 = (char*)g.gg.b4_guess;
+            if (g.gg.b4_st) {
+                char* p0 = (char*)g.gg.b4_st;
                 zero_async(p0, g.d_b4->bytes - (size_t)(p0 - (char*)g.d_b4->p), b->stream);
             }
         zero_async(x, xo * sizeof(double), b->stream);

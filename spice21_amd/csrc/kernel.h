@@ -37,6 +37,12 @@ static_assert(b4::B4_NS == BSIM4_NE, "BSIM4 code stride");
 // Device-side descriptors
 // ============================================================================================
 constexpr int M1_CORE_F = 11, M1_CAPS_F = 4, M1_STATE_F = 7, D_PAR_F = 14, D_STATE_F = 4;
+// BSIM4: doubles of a parameter structure; state fields: the last load's limiting fields (vds, vgs, vbs,
+// vbd, von) and charges (qg, qd, qb), the committed charges, and the limiting fields the last load started
+// from (to resume an interrupted solve at that load)
+constexpr int B4_PAR_D = (int)(sizeof(b4::B4Par) / sizeof(double));
+static_assert(sizeof(b4::B4Par) % sizeof(double) == 0, "B4Par is doubles only");
+enum { B4_ST_LIM = 0, B4_ST_Q = 5, B4_ST_QC = 8, B4_ST_PREV = 11, B4_ST_F = 16 };
 
 struct TopoGPU {
     int n, nnz, n_hub, n_part;
@@ -83,15 +89,14 @@ struct GroupGPU {
     double* ac_end;         // AC: [n_inst][ac_runs][n] complex: each run's last iterate (first pass)
     int* ac_state;          // AC: [n_inst][ac_runs] outcome of each run's re-solve from the previous
                             // run's end (second pass); entry 0: run to stitch from in order (third pass)
-    // BSIM4 (class-local device j of instance i): parameters b4_par[i * b4_istr + j] (b4_istr 0 when no
-    // instance overrides a BSIM4 parameter); the last load's point and the committed one [n_inst][n_b4];
-    // the limiting fields the last load started from [5][n_inst][n_b4] (vds, vgs, vbs, vbd, von), to
-    // resume an interrupted solve at that load
-    const b4::B4Par* b4_par;
-    int b4_istr;
-    b4::B4Op* b4_guess;
-    b4::B4Op* b4_op;
-    double* b4_prev;
+    // BSIM4 (class-local device j of instance i).  Parameters by field k of the B4Par structure: a field
+    // that no instance changes is stored once per device, b4_sh[k][j]; the others (b4_map[k] = their row
+    // r, -1 for a shared field) per instance, b4_var[r][i][j], so that a wavefront's lanes (the devices of
+    // an instance) read consecutive addresses.  State b4_st[B4_ST_F][i][j] (see B4_ST_*).
+    const int* b4_map;
+    const double* b4_sh;
+    const double* b4_var;
+    double* b4_st;
 };
 
 enum { MODE_OP = 0, MODE_TRAN = 1, MODE_ASSEMBLE = 2, MODE_AC = 3 };
@@ -371,31 +376,62 @@ __device__ inline void eval_mos1(const Ctx& C, int j, LDS const u16* dv, LDS con
 // BSIM4 (bsim4solver.rs:134-145): Bsim4::op from the terminal voltages with limiting against the last
 // load's point, tran_op's companions against the committed point in a transient, the 42 stamps.  The new
 // point is written to global memory as it is computed (its 86 fields never occupy registers together).
+// The parameter view of one BSIM4 device of the batched kernel: each field the evaluation reads is loaded
+// where it is stored (the field's row is uniform across the wavefront, the devices of an instance are
+// consecutive), no copy of the whole structure is made.
+struct B4Gather {
+    GLB const int* map;
+    GLB const double* sh;
+    GLB const double* va;
+    size_t nb, row0, j;  // devices per instance; instance * nb; class-local device
+    size_t n_inst;
+    template <int K>
+    __device__ double get() const {
+        const int r = map[K];
+        return r < 0 ? sh[(size_t)K * nb + j] : va[(size_t)r * n_inst * nb + row0 + j];
+    }
+};
+
 __device__ inline void eval_b4(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran, const Args& A) {
     const GroupGPU& G = *C.G;
     const int nb = C.T.n_cls[CLS_BSIM4];
     const size_t s = (size_t)G.n_inst * nb, o = (size_t)C.inst * nb + j;
-    const b4::B4Par& P = G.b4_par[(size_t)C.inst * G.b4_istr + j];
-    b4::B4Op& N = G.b4_guess[o];
+    const B4Gather P{glb(G.b4_map), glb(G.b4_sh), glb(G.b4_var), (size_t)nb, (size_t)C.inst * nb, (size_t)j,
+                     (size_t)G.n_inst};
+    GLB double* st = glb(G.b4_st);
     b4::B4Op Gl;
-    Gl.vds = N.vds;
-    Gl.vgs = N.vgs;
-    Gl.vbs = N.vbs;
-    Gl.vbd = N.vbd;
-    Gl.von = N.von;
-    GLB double* pv = glb(G.b4_prev);
-    pv[0 * s + o] = Gl.vds;
-    pv[1 * s + o] = Gl.vgs;
-    pv[2 * s + o] = Gl.vbs;
-    pv[3 * s + o] = Gl.vbd;
-    pv[4 * s + o] = Gl.von;
+    Gl.vds = st[(B4_ST_LIM + 0) * s + o];
+    Gl.vgs = st[(B4_ST_LIM + 1) * s + o];
+    Gl.vbs = st[(B4_ST_LIM + 2) * s + o];
+    Gl.vbd = st[(B4_ST_LIM + 3) * s + o];
+    Gl.von = st[(B4_ST_LIM + 4) * s + o];
+    st[(B4_ST_PREV + 0) * s + o] = Gl.vds;
+    st[(B4_ST_PREV + 1) * s + o] = Gl.vgs;
+    st[(B4_ST_PREV + 2) * s + o] = Gl.vbs;
+    st[(B4_ST_PREV + 3) * s + o] = Gl.vbd;
+    st[(B4_ST_PREV + 4) * s + o] = Gl.von;
+    b4::B4Op N;
     b4::op_eval(P, Gl, xv(C.x, (int)dv[0] - 1), xv(C.x, (int)dv[1] - 1), xv(C.x, (int)dv[2] - 1),
                 xv(C.x, (int)dv[3] - 1), A.gmin, N);
     b4::B4Tran T{};
-    if (tran) b4::tran_eval(P, G.b4_op[o], N, A.tstep, T);
+    if (tran) {
+        b4::B4Op O;  // tran_op reads the committed point's charges only
+        O.qg = st[(B4_ST_QC + 0) * s + o];
+        O.qd = st[(B4_ST_QC + 1) * s + o];
+        O.qb = st[(B4_ST_QC + 2) * s + o];
+        b4::tran_eval(P, O, N, A.tstep, T);
+    }
     Stamps<BSIM4_NE> sv;
     b4::stamp(P, N, T, sv);
     apply<BSIM4_NE, false>(C, code, sv.v, false);
+    st[(B4_ST_LIM + 0) * s + o] = N.vds;
+    st[(B4_ST_LIM + 1) * s + o] = N.vgs;
+    st[(B4_ST_LIM + 2) * s + o] = N.vbs;
+    st[(B4_ST_LIM + 3) * s + o] = N.vbd;
+    st[(B4_ST_LIM + 4) * s + o] = N.von;
+    st[(B4_ST_Q + 0) * s + o] = N.qg;
+    st[(B4_ST_Q + 1) * s + o] = N.qd;
+    st[(B4_ST_Q + 2) * s + o] = N.qb;
 }
 
 __device__ inline void eval_diode(const Ctx& C, int j, LDS const u16* dv, LDS const u16* code, bool tran,
@@ -678,9 +714,11 @@ __device__ inline void commit(const Ctx& C, bool act) {
     }
     if constexpr (B4) {
         const int nb = T.n_cls[CLS_BSIM4];
+        const size_t s = (size_t)G.n_inst * nb;
+        GLB double* st = glb(G.b4_st);
         for (int d = act ? C.sl : nb; d < nb; d += C.L) {
             const size_t o = (size_t)C.inst * nb + d;
-            G.b4_op[o] = G.b4_guess[o];
+            for (int f = 0; f < 3; f++) st[(B4_ST_QC + f) * s + o] = st[(B4_ST_Q + f) * s + o];
         }
     }
 }

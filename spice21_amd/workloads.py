@@ -9,6 +9,8 @@ Random draws use splitmix64 -> 53-bit uniform -> Box-Muller, seeded per workload
   ring_batch() ring oscillators of uneven size: k = 3 + 2 U{0..49} stages, C ~ U[5, 50] fF, VDD ~ U[0.8, 1.2]
   amp_ladder() N instances of a two-stage common-source amplifier driving a 20-section RC ladder with
                R ~ U[1k, 10k] and C ~ U[10f, 100f] per section and instance (AC sweep workload)
+  bsim4_chain_mc() N instances of a CMOS inverter chain of BSIM4 devices (default model, W 1 / 2 um,
+               L 0.1 um), per-device DELVTO ~ N(0, 15 mV) and W ~ N(base, 3 %), input tied to 0.5 V
 """
 import numpy as np
 
@@ -192,3 +194,33 @@ def power_grid(size=200, n_inv=5000, n_diode=2000, pad_every=20, seed=0xC4):
     for j in range(n_diode):
         n.diode(f"d{j}", "", node(pick[2 * n_inv + j]), dm, di)
     return n
+
+
+def bsim4_chain(stages=16, vin=0.5, vdd=1.0):
+    """Inverter chain of BSIM4 devices (the default NMOS and PMOS models, W = 1 um / 2 um, L = 0.1 um)."""
+    n = Netlist()
+    nm, pm = n.bsim4_model(0), n.bsim4_model(1)
+    ni, pi = n.bsim4_inst(w=1e-6, l=0.1e-6), n.bsim4_inst(w=2e-6, l=0.1e-6)
+    n.v("vin", vin, "in", "")
+    n.v("vdd", vdd, "vdd", "")
+    mos = []
+    for k in range(1, stages + 1):
+        a = "in" if k == 1 else f"s{k-1}"
+        b = f"s{k}"
+        mos.append((n.bsim4(f"p{k}", b, a, "vdd", "vdd", pm, pi), "p"))
+        mos.append((n.bsim4(f"n{k}", b, a, "", "", nm, ni), "n"))
+    return n, mos
+
+
+def bsim4_chain_mc(n_inst=1024, stages=16, seed=0xC6, sigma_vt=0.015, sigma_w=0.03):
+    """Monte-Carlo BSIM4 inverter chains: (netlist, overrides) setting every device's own DELVTO and W
+    (instance fields) per instance."""
+    n, mos = bsim4_chain(stages)
+    nd = len(mos)
+    z = normal(seed, 2 * nd * n_inst).reshape(n_inst, nd, 2)
+    over = []
+    for j, (ci, kind) in enumerate(mos):
+        base_w = 1e-6 if kind == "n" else 2e-6
+        over.append((ci, "inst", "delvto", sigma_vt * z[:, j, 0]))
+        over.append((ci, "inst", "w", base_w * (1.0 + sigma_w * z[:, j, 1])))
+    return n, over

@@ -191,3 +191,19 @@ def test_bsim4_netlist_slots():
     assert p.v[1] == 1e-6 and p.v[12] == 2e-12 and p.v[17] == 1.0
     assert n.bsim4_models[n.bsim4_model(1)].v[1] == 1.0
     assert A.S21_COMP_BSIM4 == 7
+
+
+def test_oracle_batch_applies_bsim4_overrides():
+    """The oracle's batched operating point applies BSIM4 instance overrides exactly as the standalone
+    netlist of each instance does (the CPU side of the BSIM4 Monte Carlo workload)."""
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import Group
+    from tests.circuits import materialize
+
+    nl, over = W.bsim4_chain_mc(6, stages=4)
+    x, it, st = O.dcop_batch(Group(nl.to_c(), 6, over), threads=2)
+    assert (st == 0).all()
+    for i in range(6):
+        xi, iti, _ = O.dcop(materialize(nl, over, i).to_c())
+        assert np.array_equal(x[i], xi) and it[i] == iti
+    assert np.max(np.abs(x[0] - x[5])) > 1e-3
