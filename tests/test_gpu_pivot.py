@@ -89,3 +89,36 @@ def test_batch_resumes_only_the_instances_that_need_it():
         t_o, X_o, names, it_o = O.tran(ci_, 1e-8, 2e-6, ics, opts=no_gmin())
         assert wave_within(b.wave(0, i), X_o, names, kinds, f"instance {i}")
         assert abs(int(it[i]) - it_o) <= 0.01 * it_o, (i, it[i], it_o)
+
+
+
+def test_transient_resume_restores_bsim4_state():
+    """The switching diode beside a BSIM4 inverter (input at 0.3 V, a 1 fF load charged from 0 V) in one
+    netlist: when the diode's pivot fails mid-transient, the resumed iteration restarts the BSIM4 devices
+    from the limiting fields their interrupted load started from and their committed charges; the waveform
+    matches the oracle point for point, with the oracle's iteration count in FP64."""
+    nl = switching_diode()
+    nm, pm, ip = nl.bsim4_model(0), nl.bsim4_model(1), nl.bsim4_inst()
+    nl.v("v1", 1.0, "vdd", "")
+    nl.v("vin", 0.3, "in", "")
+    nl.bsim4("p", "o", "in", "vdd", "vdd", pm, ip)
+    nl.bsim4("n", "o", "in", "", "", nm, ip)
+    nl.c(1e-15, "o", "")
+    cn = nl.to_c()
+    ics = [("a", 0.0), ("o", 0.0)]
+    t_o, X_o, names, it_o = O.tran(cn, 1e-8, 3e-6, ics, opts=no_gmin())
+    kinds = list(nl.kinds) + [0, 1, 0, 1]
+    for prec in (FP64, MIXED):
+        log("switching diode + BSIM4 inverter tran", prec)
+        b = Batch(Group(cn), opts=no_gmin(), precision=prec).tran(1e-8, 3e-6, ics)
+        st, it = b.status()
+        log("  status", st, "iterations", it, "oracle", it_o, "resumptions", b.resumptions())
+        assert st[0] == A.S21_OK
+        assert b.resumptions() >= 1
+        assert np.array_equal(b.axis(), t_o)
+        if prec == FP64:
+            assert int(it[0]) == it_o
+            assert wave_within(b.wave(0, 0), X_o, names, kinds, "diode + bsim4")
+        else:
+            v = [k for k, kind in enumerate(kinds) if kind == 0]
+            assert np.abs(b.wave(0, 0)[:, v] - X_o[:, v]).max() <= 1e-6
