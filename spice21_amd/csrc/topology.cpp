@@ -528,9 +528,9 @@ static void topo_identity_pattern(Topology& t) {
 }
 
 template <class T>
-static void pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r = nullptr,
-                         const std::vector<int>* order_c = nullptr,
-                         const std::vector<std::vector<T>>* later_values = nullptr);
+static double pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r = nullptr,
+                           const std::vector<int>* order_c = nullptr,
+                           const std::vector<std::vector<T>>* later_values = nullptr);
 
 void topo_pivot(Topology& t, const std::vector<double>& first_values) {
     t.sym_status = S21_OK;
@@ -564,13 +564,33 @@ void topo_pivot_sequence(Topology& t, const std::vector<double>& first_values,
                          const std::vector<std::vector<double>>& later_values) {
     t.sym_status = S21_OK;
     t.sym_error.clear();
+    // The sequence stands in for the reference's searches at every factorisation by the first one and the
+    // given later ones, so its arrangement can differ from the reference's, which factored this last matrix.
+    // From that arrangement the search can meet a zero pivot (a column whose candidates are all zero after
+    // the earlier eliminations), or leave a last pivot -- never searched -- that is zero or subnormal, which
+    // the kernel's factorisation cannot keep either; the resumed solve would stop at the same iteration
+    // again.  The last matrix is then searched afresh instead, as a new solver's first factorisation would.
     try {
-        pivot_search<double>(t, first_values, nullptr, nullptr, &later_values);
+        const double last = pivot_search<double>(t, first_values, nullptr, nullptr, &later_values);
+        if (!later_values.empty() && std::fabs(last) < 2.2250738585072014e-308)
+            fail(S21_PIVOT, "Assert Neq Failed: zero pivot");
+        return;
     } catch (Error& e) {
         t.sym_status = e.code;
         t.sym_error = e.what();
-        topo_identity_pattern(t);
     }
+    if (!later_values.empty()) {
+        try {
+            pivot_search<double>(t, later_values.back());
+            t.sym_status = S21_OK;
+            t.sym_error.clear();
+            return;
+        } catch (Error& e) {
+            t.sym_status = e.code;
+            t.sym_error = e.what();
+        }
+    }
+    topo_identity_pattern(t);
 }
 
 void topo_pivot_forced(Topology& t, const std::vector<int>& row_i2e, const std::vector<int>& col_i2e) {
@@ -587,9 +607,11 @@ void topo_pivot_forced(Topology& t, const std::vector<int>& row_i2e, const std::
 // the arrangement and the elements (fill included) the previous one left, element values reset to zero
 // and reloaded (Matrix::reset / update / lu_factorize, sparse21/mod.rs:272-282, 647-674), Markowitz
 // counts restarting from each row's and column's element count (setup_factoring, :191-196).
+// Returns the magnitude of the last pivot after the elimination (1 for a given order, whose values are not
+// computed).
 template <class T>
-static void pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r,
-                         const std::vector<int>* order_c, const std::vector<std::vector<T>>* later_values) {
+static double pivot_search(Topology& t, const std::vector<T>& first_values, const std::vector<int>* order_r,
+                           const std::vector<int>* order_c, const std::vector<std::vector<T>>* later_values) {
     const int n = t.n;
     PivotState<T> s;
     s.n = n;
@@ -643,7 +665,9 @@ static void pivot_search(Topology& t, const std::vector<T>& first_values, const 
     t.row_e2i = s.r_e2i;
     t.col_i2e = s.c_i2e;
     t.col_e2i = s.c_e2i;
-    if (s.lookup(s.r_i2e[n - 1], s.c_i2e[n - 1]) < 0) fail(S21_SINGULAR, "Singular Matrix");
+    const int last = s.lookup(s.r_i2e[n - 1], s.c_i2e[n - 1]);
+    if (last < 0) fail(S21_SINGULAR, "Singular Matrix");
+    const double last_mag = forced ? 1.0 : (double)mag(s.val[last]);
 
     // Pattern in internal coordinates; the first erow.size() entries are the matrix elements.
     const int ne = (int)s.orow.size();
@@ -668,6 +692,7 @@ static void pivot_search(Topology& t, const std::vector<T>& first_values, const 
         if (t.diag[i] < 0) fail(S21_SINGULAR, "Singular Matrix");
     }
     t.nnz = (int)t.ci.size();
+    return last_mag;
 }
 
 void topo_schedule(Topology& t, const std::vector<uint8_t>& m1_caps, int hub_contrib) {

@@ -122,3 +122,37 @@ def test_transient_resume_restores_bsim4_state():
         else:
             v = [k for k, kind in enumerate(kinds) if kind == 0]
             assert np.abs(b.wave(0, 0)[:, v] - X_o[:, v]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["mos1", "bsim4"])
+def test_ring_beside_the_switching_diode_recovers(kind):
+    """The switching diode beside a three-stage ring oscillator (Mos1 or BSIM4) at 10 ns steps.  The
+    recovery's search sequence (the first search, then the interrupted iteration's matrix from the
+    arrangement it left) meets a zero pivot here, where the reference -- which has searched at every
+    factorisation -- does not; the interrupted matrix is then searched afresh, and one resumption carries
+    the run through, point for point with the oracle."""
+    nl = switching_diode()
+    if kind == "bsim4":
+        nm, pm = nl.bsim4_model(0), nl.bsim4_model(1)
+        ni = pi = nl.bsim4_inst()
+        dev, cap = nl.bsim4, 1e-15
+    else:
+        nm, pm = nl.mos1_model(vt0=0.4, kp=100e-6), nl.mos1_model(mos_type=1, vt0=-0.4, kp=50e-6)
+        ni, pi = nl.mos1_inst(w=1e-6, l=0.1e-6), nl.mos1_inst(w=2e-6, l=0.1e-6)
+        dev, cap = nl.mos1, 10e-15
+    nl.v("v1", 1.0, "vdd", "")
+    for x, (a, b) in enumerate([("r1", "r2"), ("r2", "r3"), ("r3", "r1")]):
+        dev(f"x{x}.p", b, a, "vdd", "vdd", pm, pi)
+        dev(f"x{x}.n", b, a, "", "", nm, ni)
+        nl.c(cap, b, "")
+    cn = nl.to_c()
+    ics = [("a", 0.0), ("r1", 0.0)]
+    t_o, X_o, names, it_o = O.tran(cn, 1e-8, 3e-6, ics, opts=no_gmin())
+    log("switching diode + ring", kind)
+    b = Batch(Group(cn), opts=no_gmin()).tran(1e-8, 3e-6, ics)
+    st, it = b.status()
+    log("  status", st, "iterations", it, "oracle", it_o, "resumptions", b.resumptions())
+    assert st[0] == A.S21_OK
+    assert b.resumptions() >= 1
+    assert abs(int(it[0]) - it_o) <= 0.01 * it_o
+    assert wave_within(b.wave(0, 0), X_o, names, list(nl.kinds) + [0, 1, 0, 1], "diode + ring")
