@@ -195,43 +195,69 @@ __global__ void __launch_bounds__(TPB) k_eval(EvalArgs E) {
 
 // BSIM4, one thread per device (bsim4solver.rs:134-145, 3757-3769): Bsim4::op from the port voltages with
 // limiting against the device's last guess, tran_op's Backward-Euler charge companions against the committed
-// point in a transient, then the 42 stamps into the device's contribution cells.  Parameters and states are
-// arrays of structures in HBM; the new guess is written in place once the old one's limiting fields are in
-// registers.
+// charges in a transient, then the 42 stamps into the device's contribution cells.  Parameters and state are
+// structures of arrays in HBM (field-major, so consecutive threads read consecutive addresses): parameter
+// field K of device j at par[K][j]; state [B4S_F][j]: the last load's limiting fields (vds vgs vbs vbd von)
+// and charges (qg qd qb), then the committed charges.
+constexpr int B4S_LIM = 0, B4S_Q = 5, B4S_QC = 8, B4S_F = 11;
 struct B4Args {
     int n, tran;
     double gmin, dt;
     const double* x;
-    const int* node;          // [n][4] d g s b (-1 = ground)
-    const b4::B4Par* par;     // [n]
-    b4::B4Op* guess;          // [n] the last load's point
-    const b4::B4Op* op;       // [n] the committed point
-    const int* cb;            // [n] first contribution cell
+    const int* node;   // [n][4] d g s b (-1 = ground)
+    const double* par; // [parameter doubles][n]
+    double* st;        // [B4S_F][n]
+    const int* cb;     // [n] first contribution cell
     double* cv;
+};
+// The parameter view of one device: field K read where it is stored.
+struct B4Fields {
+    const double* par;
+    int n, j;
+    template <int K>
+    __device__ double get() const {
+        return par[(size_t)K * n + j];
+    }
 };
 
 __global__ void __launch_bounds__(128) k_eval_b4(B4Args A) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.n) return;
     const int* nd = A.node + 4 * j;
-    const b4::B4Par& P = A.par[j];
-    b4::B4Op G;  // only the limiting fields of the old guess are read (vds, vgs, vbs, vbd, von)
-    G.vds = A.guess[j].vds;
-    G.vgs = A.guess[j].vgs;
-    G.vbs = A.guess[j].vbs;
-    G.vbd = A.guess[j].vbd;
-    G.von = A.guess[j].von;
-    b4::B4Op& N = A.guess[j];
+    const B4Fields P{A.par, A.n, j};
+    const size_t n = A.n;
+    b4::B4Op G;  // only the limiting fields of the last load's point are read
+    G.vds = A.st[(B4S_LIM + 0) * n + j];
+    G.vgs = A.st[(B4S_LIM + 1) * n + j];
+    G.vbs = A.st[(B4S_LIM + 2) * n + j];
+    G.vbd = A.st[(B4S_LIM + 3) * n + j];
+    G.von = A.st[(B4S_LIM + 4) * n + j];
+    b4::B4Op N;
     b4::op_eval(P, G, xval(A.x, nd[0]), xval(A.x, nd[1]), xval(A.x, nd[2]), xval(A.x, nd[3]), A.gmin, N);
     b4::B4Tran T{};
-    if (A.tran) b4::tran_eval(P, A.op[j], N, A.dt, T);
+    if (A.tran) {
+        b4::B4Op O;  // tran_op reads the committed point's charges only
+        O.qg = A.st[(B4S_QC + 0) * n + j];
+        O.qd = A.st[(B4S_QC + 1) * n + j];
+        O.qb = A.st[(B4S_QC + 2) * n + j];
+        b4::tran_eval(P, O, N, A.dt, T);
+    }
     double* cv = A.cv + A.cb[j];
     b4::stamp(P, N, T, [&](int k, double v) { cv[k] = v; });
+    A.st[(B4S_LIM + 0) * n + j] = N.vds;
+    A.st[(B4S_LIM + 1) * n + j] = N.vgs;
+    A.st[(B4S_LIM + 2) * n + j] = N.vbs;
+    A.st[(B4S_LIM + 3) * n + j] = N.vbd;
+    A.st[(B4S_LIM + 4) * n + j] = N.von;
+    A.st[(B4S_Q + 0) * n + j] = N.qg;
+    A.st[(B4S_Q + 1) * n + j] = N.qd;
+    A.st[(B4S_Q + 2) * n + j] = N.qb;
 }
 
-__global__ void k_commit_b4(int n, const b4::B4Op* __restrict__ guess, b4::B4Op* __restrict__ op) {
+__global__ void k_commit_b4(int n, double* __restrict__ st) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) op[j] = guess[j];
+    if (j < n)
+        for (int f = 0; f < 3; f++) st[(size_t)(B4S_QC + f) * n + j] = st[(size_t)(B4S_Q + f) * n + j];
 }
 
 // Each matrix element (then each RHS entry) sums its contributions in component-then-stamp order.
@@ -917,7 +943,7 @@ struct s21_large {
     s21::lg::Buf f_fo, f_first, f_rptr, f_ridx, f_chptr, f_ch, f_xptr, f_xmap, f_aptr, f_aelem, f_aoff, f_woff;
     s21::lg::Buf l_small, l_big, l_jobs, l_all, perm, prow, pool, yv, wv;
     s21::lg::Buf wave;
-    s21::lg::Buf b4_node, b4_par, b4_guess, b4_op, b4_cb;
+    s21::lg::Buf b4_node, b4_par, b4_st, b4_cb;
     // trapezoidal integration: per-charge currents, unknown kinds, accepted-point history
     s21::lg::Buf m1_icur, d_icur, c_icur, kinds_dev, hist[4];
     int trap = 0;
@@ -1253,10 +1279,15 @@ static void prepare(s21_large* L, const s21_tran_options* tr) {
     upload(L->l_val, lval);
     upload(L->l_cb, l_cb);
     upload(L->b4_node, b4_node);
-    upload(L->b4_par, h0.b4);
+    {  // parameters field-major: field K of device j at [K][j]
+        constexpr int D = (int)(sizeof(b4::B4Par) / sizeof(double));
+        std::vector<double> par((size_t)D * nb4);
+        for (int j = 0; j < nb4; j++)
+            for (int k = 0; k < D; k++) par[(size_t)k * nb4 + j] = ((const double*)&h0.b4[j])[k];
+        upload(L->b4_par, par);
+    }
     upload(L->b4_cb, b4_cb);
-    L->b4_guess.alloc(std::max<size_t>(nb4, 1) * sizeof(b4::B4Op));
-    L->b4_op.alloc(std::max<size_t>(nb4, 1) * sizeof(b4::B4Op));
+    L->b4_st.alloc(std::max<size_t>((size_t)B4S_F * nb4, 1) * sizeof(double));
     L->m1_state.alloc(std::max<size_t>((size_t)2 * M1S * nm, 1) * sizeof(double));
     L->d_state.alloc(std::max<size_t>((size_t)DSF * nd, 1) * sizeof(double));
     L->c_q.alloc(std::max<size_t>(nl, 1) * sizeof(double));
@@ -1410,7 +1441,7 @@ static void commit_state(s21_large* L, bool tran, double dt) {
                        L->c_q.as<double>());
     if (L->n_b4)
         hipLaunchKernelGGL(k_commit_b4, dim3(blocks(L->n_b4)), dim3(TPB), 0, L->stream, L->n_b4,
-                           L->b4_guess.as<b4::B4Op>(), L->b4_op.as<b4::B4Op>());
+                           L->b4_st.as<double>());
     LCHECK(hipGetLastError());
 }
 
@@ -1426,9 +1457,8 @@ static void assemble_residual(s21_large* L, bool tran, double dt) {
                  dt,
                  L->x.as<double>(),
                  L->b4_node.as<int>(),
-                 L->b4_par.as<b4::B4Par>(),
-                 L->b4_guess.as<b4::B4Op>(),
-                 L->b4_op.as<b4::B4Op>(),
+                 L->b4_par.as<double>(),
+                 L->b4_st.as<double>(),
                  L->b4_cb.as<int>(),
                  L->cv.as<double>()};
         hipLaunchKernelGGL(k_eval_b4, dim3((L->n_b4 + 127) / 128), dim3(128), 0, L->stream, A);
@@ -1507,8 +1537,7 @@ static int newton(s21_large* L, bool tran, double dt, bool commit = true) {
 }
 
 static void reset_state(s21_large* L) {
-    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x, &L->m1_icur, &L->d_icur, &L->c_icur, &L->b4_guess,
-                   &L->b4_op})
+    for (Buf* b : {&L->m1_state, &L->d_state, &L->c_q, &L->x, &L->m1_icur, &L->d_icur, &L->c_icur, &L->b4_st})
         LCHECK(hipMemsetAsync(b->p, 0, b->bytes, L->stream));
     L->iters = 0;
     L->factorisations = 0;

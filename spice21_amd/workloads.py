@@ -156,11 +156,13 @@ def amp_ladder(n_inst=256, sections=20, seed=0xC5, vin=0.6, vdd=1.0, rd1=5e3, rd
     return n, over
 
 
-def power_grid(size=200, n_inv=5000, n_diode=2000, pad_every=20, seed=0xC4):
+def power_grid(size=200, n_inv=5000, n_diode=2000, pad_every=20, seed=0xC4, device="mos1"):
     """One large instance: a size x size resistor mesh (edge R ~ U[0.5, 2] ohm, 1 pF from every mesh node
     to ground) with 1.0 V pads on every `pad_every`-th node of its boundary walk, `n_inv` CMOS inverters
     (input and PMOS source/bulk at random mesh nodes, NMOS to ground, each driving a new output node) and
-    `n_diode` reverse-biased diodes (IS 1e-14, N 1; anode at ground, cathode at a random mesh node)."""
+    `n_diode` reverse-biased diodes (IS 1e-14, N 1; anode at ground, cathode at a random mesh node).
+    device: "mos1" (the benchmark's Mos1 models) or "bsim4" (the default BSIM4 models, W 1 / 2 um,
+    L 0.1 um)."""
     n = Netlist()
     g = lambda r, c: f"g{r}_{c}"
     nedge = 2 * size * (size - 1)
@@ -183,12 +185,18 @@ def power_grid(size=200, n_inv=5000, n_diode=2000, pad_every=20, seed=0xC4):
             [(size - 1, c) for c in range(size - 2, -1, -1)] + [(r, 0) for r in range(size - 2, 0, -1)])
     for j, (r, c) in enumerate(walk[::pad_every]):
         n.v(f"vpad{j}", 1.0, g(r, c), "")
-    nm, pm, ni, pi = _models(n)
+    if device == "bsim4":
+        nm, pm = n.bsim4_model(0), n.bsim4_model(1)
+        ni, pi = n.bsim4_inst(w=1e-6, l=0.1e-6), n.bsim4_inst(w=2e-6, l=0.1e-6)
+        dev = n.bsim4
+    else:
+        nm, pm, ni, pi = _models(n)
+        dev = n.mos1
     node = lambda p: g(int(p) // size, int(p) % size)
     for j in range(n_inv):
         a, s = node(pick[2 * j]), node(pick[2 * j + 1])
-        n.mos1(f"inv{j}.p", f"o{j}", a, s, s, pm, pi)
-        n.mos1(f"inv{j}.n", f"o{j}", a, "", "", nm, ni)
+        dev(f"inv{j}.p", f"o{j}", a, s, s, pm, pi)
+        dev(f"inv{j}.n", f"o{j}", a, "", "", nm, ni)
     dm = n.diode_model(is_=1e-14, n=1.0)
     di = n.diode_inst()
     for j in range(n_diode):

@@ -13,6 +13,7 @@ from oracle import oracle_py as O
 from spice21_amd import _abi as A
 from spice21_amd import api
 from spice21_amd import protos as P
+from spice21_amd import workloads as W
 from spice21_amd._lib import S21Error
 from spice21_amd.batch import Batch, Group
 from spice21_amd.large import FP64, MIXED, Large
@@ -239,6 +240,75 @@ def test_bsim4_batch_and_single_instance_paths_agree():
     assert b.status()[0][0] == A.S21_OK and g.status()[0] == A.S21_OK
     assert b.var_names() == g.var_names()
     assert wave_within(b.wave(0, 0), g.wave(), g.var_names(), list(nl.kinds) + [0, 1], "batch vs large")
+
+
+def bsim4_grid(size):
+    return W.power_grid(size=size, n_inv=size * size // 8, n_diode=size * size // 20, pad_every=4, device="bsim4")
+
+
+@pytest.mark.parametrize("size", [8, 16])
+def test_bsim4_power_grid_dcop_matches_the_oracle(size):
+    """A resistor mesh with BSIM4 inverters and diodes on the single-instance path: FP64 node voltages
+    against the oracle to 1e-9 relative, the pads' branch currents to 1e-14 A (they carry ~1e-10 A of
+    leakage, the sum of mesh currents of ~1 A, so their round-off is ~1e-16 A whatever the pivot order),
+    mixed against FP64 to 1e-6 V."""
+    nl = bsim4_grid(size)
+    cn = nl.to_c()
+    log("bsim4 grid dcop", size, "unknowns", cn.n_vars)
+    g = Large(cn).op()
+    st, it = g.status()
+    xo, ito, _ = O.dcop(cn)
+    log("  status", st, "iterations", it, "oracle", ito)
+    assert st == A.S21_OK
+    v = np.asarray(nl.kinds) == A.S21_VAR_V
+    rel = np.abs(g.x() - xo)[v] / np.maximum(np.abs(xo[v]), 1e-9)
+    cur = np.abs(g.x() - xo)[~v]
+    log("  max relative node |x - oracle|", rel.max(), "max branch current difference", cur.max())
+    assert rel.max() < 1e-9 and cur.max() <= 1e-14
+    gm = Large(cn, precision=MIXED).op()
+    assert gm.status()[0] == A.S21_OK
+    err = np.abs(gm.x() - g.x())[v].max()
+    log("  mixed vs fp64 max node V", err)
+    assert err <= 1e-6
+
+
+def test_bsim4_power_grid_transient_matches_the_oracle():
+    nl = bsim4_grid(10)
+    cn = nl.to_c()
+    ics = [("g0_0", 0.0)]
+    log("bsim4 grid tran")
+    t_o, X_o, names_o, it_o = O.tran(cn, 1e-12, 30e-12, ics)
+    g = Large(cn).tran(1e-12, 30e-12, ics)
+    st, it = g.status()
+    log("  status", st, "iterations", it, "oracle", it_o)
+    assert st == A.S21_OK
+    assert np.array_equal(g.axis(), t_o) and g.var_names() == list(names_o)
+    assert wave_within(g.wave(), X_o, names_o, list(nl.kinds) + [0, 1], "bsim4 grid tran")
+
+
+def test_full_bsim4_power_grid_dcop_kcl():
+    """The 200 x 200 grid with 5,000 BSIM4 inverters (10,000 devices) and 2,000 diodes: the FP64
+    operating point's KCL residual in the oracle's own assembly is within 1e-10 A, and mixed precision
+    agrees with FP64 to 1e-6 V."""
+    nl = W.power_grid(device="bsim4")
+    cn = nl.to_c()
+    log("full bsim4 grid fp64 dcop")
+    g = Large(cn).op()
+    st, it = g.status()
+    log("  status", st, "iterations", it, "ms", g.timing()[0])
+    assert st == A.S21_OK
+    x = g.x()
+    r, c, v, rhs = O.assemble_op(cn, x, max_elems=1 << 22)
+    res = rhs.copy()
+    np.subtract.at(res, np.asarray(r), np.asarray(v) * x[np.asarray(c)])
+    log("  oracle-assembled KCL residual", np.abs(res).max())
+    assert np.abs(res).max() <= 1e-10
+    gm = Large(cn, precision=MIXED).op()
+    assert gm.status()[0] == A.S21_OK
+    vmask = np.asarray(nl.kinds) == A.S21_VAR_V
+    err = np.abs(gm.x() - x)[vmask].max()
+    log("  mixed vs fp64 max node V", err)
+    assert err <= 1e-6
 
 
 def test_bsim4_error_paths():
