@@ -632,13 +632,26 @@ __global__ void __launch_bounds__(TPB) k_backward(Fronts S, const int* __restric
     for (int j = tid; j < k; j += TPB) y[f + j] = w[j];
 }
 
+// acc - sum_q a[q F] x[q] over the panel's w columns, in q order; every load is issued before the first
+// use, so a thread waits for memory once per panel.
+template <typename T>
+__device__ inline T panel_dot(const T* __restrict__ a, int F, int w, const T* x, T acc) {
+    T av[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) av[q] = q < w ? a[(size_t)q * F] : (T)0;
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+        if (q < w) acc = acc - av[q] * x[q];
+    return acc;
+}
+
 // Sweeps of large fronts, the fronts of a level per launch (grid.y), panel by panel.
-// Forward: panel [c, c + w) solves L11 x = w[c, c + w) (unit lower, one wavefront), block 0 stores x in y,
+// Forward: panel [c, c + w) solves L11 x = w[c, c + w) (unit lower) in one wavefront, lane i holding x_i
+// and row i of L11 in registers, x_j passed from lane j through the scalar unit; block 0 stores x in y,
 // every block updates its rows below: w_i -= L(i, c..c+w) x.
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_fwd_panel(Fronts S, const int* __restrict__ list, int panel,
                                                    const T* __restrict__ pool, T* __restrict__ y, T* __restrict__ wbuf) {
-    __shared__ T Ls[NB][NB + 1];
     __shared__ T xp[NB];
     const int s = list[blockIdx.y];
     const int k = fk(S, s), F = fF(S, s), f = S.first[s];
@@ -650,27 +663,26 @@ __global__ void __launch_bounds__(TPB) k_fwd_panel(Fronts S, const int* __restri
     const T* A = pool + S.fo[s];
     T* wv = wbuf + S.w_off[s];
     const int tid = threadIdx.x;
-    for (int q = tid; q < w * w; q += TPB) {
-        const int j = q / w, i = q % w;
-        Ls[j][i] = A[(size_t)(c + j) * F + c + i];
-    }
-    if (tid < w) xp[tid] = wv[c + tid];
-    __syncthreads();
     if (tid < 64) {
-        for (int j = 0; j < w; j++) {
-            wave_sync();
-            const T xj = xp[j];
-            if (tid > j && tid < w) xp[tid] = xp[tid] - Ls[j][tid] * xj;
+        const bool own = tid < w;
+        T lrow[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) lrow[j] = (own && j < w) ? A[(size_t)(c + j) * F + c + tid] : (T)0;
+        T x = own ? wv[c + tid] : (T)0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j < w) {
+                const T xj = lane_value(x, j);
+                if (tid > j && own) x = x - lrow[j] * xj;
+            }
         }
-        wave_sync();
+        if (own) xp[tid] = x;
     }
     __syncthreads();
     if (blockIdx.x == 0 && tid < w) y[f + c + tid] = xp[tid];
     const int i = r0 + blockIdx.x * TPB + tid;
     if (i >= F) return;
-    T acc = wv[i];
-    for (int q = 0; q < w; q++) acc = acc - A[(size_t)(c + q) * F + i] * xp[q];
-    wv[i] = acc;
+    wv[i] = panel_dot(A + (size_t)c * F + i, F, w, xp, wv[i]);
 }
 
 // Forward of large fronts: the sweep vector from the own right-hand side and the children's updates.
@@ -708,8 +720,23 @@ __global__ void __launch_bounds__(1024) k_bwd_init(Fronts S, const int* __restri
     const T* A = pool + S.fo[s];
     const int* R = S.r_idx + S.r_ptr[s];
     T acc = (T)0;
-    if (j < k)
-        for (int c = k + wv; c < F; c += 16) acc = acc + A[(size_t)c * F + j] * y[R[c - k]];
+    if (j < k) {
+        int c = k + wv;
+        for (; c + 16 * 7 < F; c += 16 * 8) {  // eight columns' loads in flight, added in column order
+            int ri[8];
+            T a[8], yv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                ri[u] = R[c + 16 * u - k];
+                a[u] = A[(size_t)(c + 16 * u) * F + j];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) yv[u] = y[ri[u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = acc + a[u] * yv[u];
+        }
+        for (; c < F; c += 16) acc = acc + A[(size_t)c * F + j] * y[R[c - k]];
+    }
     part[wv][lane] = acc;
     __syncthreads();
     if (wv == 0 && j < k) {
@@ -719,12 +746,12 @@ __global__ void __launch_bounds__(1024) k_bwd_init(Fronts S, const int* __restri
     }
 }
 
-// Backward panel [c, c + w), panels from the last: U11 x = w[c, c + w) (one wavefront), block 0 stores x
-// in y, every block updates its rows above: w_i -= U(i, c..c+w) x.
+// Backward panel [c, c + w), panels from the last: U11 x = w[c, c + w) in one wavefront (lane i holding
+// x_i and row i of U11, as forward), block 0 stores x in y, every block updates its rows above:
+// w_i -= U(i, c..c+w) x.
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_bwd_panel(Fronts S, const int* __restrict__ list, int panel,
                                                    const T* __restrict__ pool, T* __restrict__ y, T* __restrict__ wbuf) {
-    __shared__ T Us[NB][NB + 1];
     __shared__ T xp[NB];
     const int s = list[blockIdx.y];
     const int k = fk(S, s), F = fF(S, s), f = S.first[s];
@@ -735,29 +762,27 @@ __global__ void __launch_bounds__(TPB) k_bwd_panel(Fronts S, const int* __restri
     const T* A = pool + S.fo[s];
     T* wv = wbuf + S.w_off[s];
     const int tid = threadIdx.x;
-    for (int q = tid; q < w * w; q += TPB) {
-        const int j = q / w, i = q % w;
-        Us[j][i] = A[(size_t)(c + j) * F + c + i];
-    }
-    if (tid < w) xp[tid] = wv[c + tid];
-    __syncthreads();
     if (tid < 64) {
-        for (int j = w - 1; j >= 0; j--) {
-            wave_sync();
-            if (tid == j) xp[j] = xp[j] / Us[j][j];
-            wave_sync();
-            const T xj = xp[j];
-            if (tid < j) xp[tid] = xp[tid] - Us[j][tid] * xj;
+        const bool own = tid < w;
+        T urow[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) urow[j] = (own && j < w) ? A[(size_t)(c + j) * F + c + tid] : (T)0;
+        T x = own ? wv[c + tid] : (T)0;
+#pragma unroll
+        for (int j = NB - 1; j >= 0; j--) {
+            if (j < w) {
+                if (tid == j) x = x / urow[j];
+                const T xj = lane_value(x, j);
+                if (tid < j) x = x - urow[j] * xj;
+            }
         }
-        wave_sync();
+        if (own) xp[tid] = x;
     }
     __syncthreads();
     if (blockIdx.x == 0 && tid < w) y[f + c + tid] = xp[tid];
     const int i = blockIdx.x * TPB + tid;
     if (i >= c) return;
-    T acc = wv[i];
-    for (int q = 0; q < w; q++) acc = acc - A[(size_t)(c + q) * F + i] * xp[q];
-    wv[i] = acc;
+    wv[i] = panel_dot(A + (size_t)c * F + i, F, w, xp, wv[i]);
 }
 
 // internal right-hand side from the FP64 residual (rows in pivot order), and dx back in external order
