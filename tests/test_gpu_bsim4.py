@@ -311,6 +311,33 @@ def test_full_bsim4_power_grid_dcop_kcl():
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("vs", [0.0, 0.05])
+def test_bsim4_junction_areas_on_the_gpu(vs):
+    """The junction-area closed form of the reference's equations (tests/test_oracle_bsim4.py::
+    reference_junction) through both GPU paths: a batch whose instances give AS or AD and drive the bulk at
+    0.1-0.4 V, and the single-instance path."""
+    from tests.test_oracle_bsim4 import reference_junction
+    a = 1e-10
+    for vb in (0.1, 0.25, 0.4):
+        for src_area in (True, False):
+            n = Netlist()
+            m = n.bsim4_model(0)
+            ip = n.bsim4_inst(as_=a if src_area else 0.0, ad=0.0 if src_area else a, ps=0.0, pd=0.0)
+            n.bsim4("m", "d", "", "s", "b", m, ip)
+            n.v("vb", vb, "b", "")
+            n.v("vs", vs, "s", "")
+            n.v("vd", 0.0, "d", "")
+            want = reference_junction(vb - vs if src_area else vb, a)
+            cn = n.to_c()
+            b = Batch(Group(cn, 2)).op()
+            g = Large(cn).op()
+            k = n.names.index("vb")
+            log("junction vb", vb, "vs", vs, "source" if src_area else "drain", b.x(0)[0][k], g.x()[k], want)
+            assert b.status()[0][0] == A.S21_OK and g.status()[0] == A.S21_OK
+            assert b.x(0)[0][k] == pytest.approx(want, rel=1e-9)
+            assert g.x()[k] == pytest.approx(want, rel=1e-9)
+
+
 def test_bsim4_error_paths():
     nl = ro3("bsim4_cmos")
     with pytest.raises(S21Error, match="profiling kernels"):

@@ -207,3 +207,57 @@ def test_oracle_batch_applies_bsim4_overrides():
         xi, iti, _ = O.dcop(materialize(nl, over, i).to_c())
         assert np.array_equal(x[i], xi) and it[i] == iti
     assert np.max(np.abs(x[0] - x[5])) > 1e-3
+
+
+@pytest.mark.parametrize("nf,sd", [(1.0, None), (2.0, 1e-6)])
+def test_bsim4_stress_parameters_are_inert_at_the_default_model(nf, sd):
+    """SA/SB/SD enter through the stress effect (bsim4inst.rs:784-824), whose every correction is a model
+    coefficient (KU0, KVTH0, STK2, STETA0, model/vals.rs:1170-1186, all zero by default) times an offset: at
+    the only model the proto API reaches they change nothing, bit for bit, with one finger or with two
+    (where the branch also needs SD > 0)."""
+    base = dict(w=10e-6, nf=nf)
+    stress = dict(base, sa=2e-6, sb=3e-6)
+    if sd is not None:
+        stress["sd"] = sd
+    assert bias(0.7, 0.0, **base) == bias(0.7, 0.0, **stress)
+
+
+def bulk_current(vb, vs=0.0, vd=0.0, **inst):
+    """An NMOS with gate, drain and source driven (gate at 0 V: no channel current reaches the bulk) and
+    its bulk held at vb: the current the bulk source supplies."""
+    n = Netlist()
+    m, ip = n.bsim4_model(0), n.bsim4_inst(ps=0.0, pd=0.0, **inst)
+    n.bsim4("m", "d", "", "s", "b", m, ip)
+    n.v("vb", vb, "b", "")
+    n.v("vs", vs, "s", "")
+    n.v("vd", vd, "d", "")
+    x, _, _ = O.dcop(n.to_c())
+    return dict(zip(n.names, x))["vb"]
+
+
+def reference_junction(v, area):
+    """The bulk current of one forward-biased junction as the reference solves it at the default
+    rbodymod = 0.  The junction current is Shockley's, cbs = Is (e^x - 1), gbs = Is e^x / Vt, x = v / Vt
+    (bsim4solver.rs:549-558, dioMod 1, below the explosion voltage), Is = JSS * AS (bsim4inst.rs:1124, the
+    sidewall densities being zero), Vt = KB / Q * 300.15 (bsim4derive.rs:38, 54; comps/mod.rs:26-28);
+    the bulk row's right-hand side carries gbs v - cbs (stamp.rs:295) while the Jacobian's bulk diagonal
+    holds the junction conductance only when rbodymod is on (stamp.rs:369, 485).  With the bulk voltage
+    forced, the bulk source's current is therefore gbs v - cbs = Is (e^x (x - 1) + 1), not cbs."""
+    vt = 1.3806226e-23 / 1.6021918e-19 * 300.15
+    x = v / vt
+    return 1e-4 * area * (np.exp(x) * (x - 1.0) + 1.0)
+
+
+@pytest.mark.parametrize("vb", [0.1, 0.25, 0.4])
+def test_bsim4_junction_areas_closed_form(vb):
+    """AS and AD given (perimeters zero): the bulk current of each junction against the closed form of
+    the reference's equations; AS reaches the source junction and AD the drain one (source or drain at
+    0.05 V changes only its own junction's voltage); no junction area, no bulk current."""
+    a = 1e-10
+    assert bulk_current(vb, as_=0.0, ad=0.0) == 0.0
+    i_s = bulk_current(vb, as_=a, ad=0.0)
+    assert i_s == pytest.approx(reference_junction(vb, a), rel=1e-9)
+    assert bulk_current(vb, as_=0.0, ad=a) == i_s
+    assert bulk_current(vb, as_=2 * a, ad=0.0) == pytest.approx(2 * i_s, rel=1e-14)
+    assert bulk_current(vb, vs=0.05, as_=a, ad=0.0) == pytest.approx(reference_junction(vb - 0.05, a), rel=1e-9)
+    assert bulk_current(vb, vs=0.05, as_=0.0, ad=a) == pytest.approx(reference_junction(vb, a), rel=1e-9)
