@@ -261,3 +261,16 @@ def test_bsim4_junction_areas_closed_form(vb):
     assert bulk_current(vb, as_=2 * a, ad=0.0) == pytest.approx(2 * i_s, rel=1e-14)
     assert bulk_current(vb, vs=0.05, as_=a, ad=0.0) == pytest.approx(reference_junction(vb - 0.05, a), rel=1e-9)
     assert bulk_current(vb, vs=0.05, as_=0.0, ad=a) == pytest.approx(reference_junction(vb, a), rel=1e-9)
+
+
+@pytest.mark.parametrize("vd", [0.3, 1.0])
+def test_bsim4_fingers_scale_the_one_finger_current(vd):
+    """NF fingers of total width NF * W are NF devices of width W (bsim4inst.rs:167-169 size one finger
+    from W / NF; the channel current is that finger's times NF) except for the drain junction's gmin
+    conductance, which is per device (with no junction area its current is gmin * vbd,
+    bsim4solver.rs:534-537): I(NF, NF W) - NF I(1, W) = -(NF - 1) gmin vbd = (NF - 1) gmin vd, with gmin the
+    options' 1e-12 S.  A lost or doubled NF, or a finger sized from W, fails this by orders of magnitude."""
+    one = bias(vd, 0.0, w=5e-6)["vd"]
+    for nf in (2, 3, 4):
+        many = bias(vd, 0.0, w=5e-6 * nf, nf=nf)["vd"]
+        assert many - nf * one == pytest.approx((nf - 1) * 1e-12 * vd, rel=1e-6)
