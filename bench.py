@@ -74,11 +74,12 @@ def summary_fields(path):
 def measured_traffic():
     """HBM bytes per launch of the mixed kernel on this workload, copied from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size; counters cannot be read inside
-    this run); (None, None) when absent."""
+    this run), and the kernel time the summary was taken at; (None, None, None) when absent."""
     f = summary_fields(TRAFFIC_FILE)
     if "traffic_bytes_per_launch" not in f:
-        return None, None
-    return float(f["traffic_bytes_per_launch"]), f"copied from profiles/{os.path.basename(TRAFFIC_FILE)}"
+        return None, None, None
+    at_ms = float(f["kernel_avg_ms_at_profile"]) if "kernel_avg_ms_at_profile" in f else None
+    return float(f["traffic_bytes_per_launch"]), f"copied from profiles/{os.path.basename(TRAFFIC_FILE)}", at_ms
 
 
 def binding_unit(kernel_avg_ms):
@@ -393,7 +394,7 @@ def main():
     kernel_avg_ms = float(np.mean(h["kernel_ms"]))
     b_iter = algorithmic_bytes(cn, h["structure"], 4 if head == "mixed" else 8)
     achieved = b_iter * inst_iters / (kernel_avg_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic() if head == "mixed" else (None, None)
+    traffic, traffic_src, traffic_at_ms = measured_traffic() if head == "mixed" else (None, None, None)
 
     line = {
         "metric": "batched Newton instance-iters/s & tran pts/s, mixed vs FP64 at \u22641e-6 V error",
@@ -417,6 +418,10 @@ def main():
         # below 1 % of them, and the unit that binds the kernel is named in "binding".
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     # the copied figure describes this kernel only while its time still matches the profiled one
+                     "traffic_profile_kernel_ms": traffic_at_ms,
+                     "traffic_kernel_time_matches": (abs(kernel_avg_ms / traffic_at_ms - 1.0) <= 0.1
+                                                     if traffic_at_ms else None),
                      "bytes_per_inst_iter": b_iter, "kernel_avg_ms": kernel_avg_ms,
                      "binding": binding_unit(kernel_avg_ms) if head == "mixed" else None},
     }
