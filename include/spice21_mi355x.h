@@ -369,7 +369,9 @@ S21_API int s21_large_host_check(int32_t n, int64_t ne, const int32_t* rows, con
 
 /* ---- byte ABI mirroring CallableProto::call_bytes (proto.rs:32-145) ------------------------ */
 /* Op -> OpResult, Tran -> TranResult, Ac -> AcResult, all protobuf-encoded.  *out is allocated
- * by the library and released with s21_free_bytes. */
+ * by the library and released with s21_free_bytes.  Op and Tran run the circuit as a batch of one, or
+ * on the single-instance path when it holds BSIM4 devices or the batched kernel refuses it as too
+ * large; Ac runs as a batch of one only. */
 S21_API int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
 S21_API int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);
 S21_API int s21_ac_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len);

@@ -771,6 +771,15 @@ static void engine_check(int rc) {
     }
 }
 
+// Whether a batch call failed only because the circuit does not fit one workgroup's LDS (or the batched
+// kernel's 16-bit record indices): such a circuit runs on the single-instance path instead.
+static bool too_large_for_batch(int rc) {
+    if (rc != S21_INVALID) return false;
+    char buf[1024];
+    s21_last_error(buf, sizeof buf);
+    return std::strstr(buf, "circuit too large for the batched kernel") != nullptr;
+}
+
 }  // namespace s21
 
 using namespace s21;
@@ -826,19 +835,27 @@ int s21_op_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len) 
         s21_netlist nl;
         view(e, nl);
         std::vector<double> x(nl.n_vars);
-        if (has_bsim4(nl)) {  // BSIM4 circuits run on the single-instance path
+        // BSIM4 circuits, and circuits too large for the batched kernel, run on the single-instance path
+        bool large = has_bsim4(nl);
+        if (!large) {
+            s21_group g{&nl, 1, 0, nullptr};
+            BatchPtr bp;
+            int rc = s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b);
+            if (rc == S21_OK) rc = s21_batch_op(bp.b);
+            if (too_large_for_batch(rc)) {
+                large = true;
+            } else {
+                engine_check(rc);
+                check_status(bp.b);
+                engine_check(s21_batch_get_x(bp.b, 0, x.data()));
+            }
+        }
+        if (large) {
             LargePtr lp;
             engine_check(s21_large_create(&nl, &c.opts, S21_FP64, &lp.l));
             engine_check(s21_large_op(lp.l));
             check_status(lp.l);
             engine_check(s21_large_get_x(lp.l, x.data()));
-        } else {
-            s21_group g{&nl, 1, 0, nullptr};
-            BatchPtr bp;
-            engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
-            engine_check(s21_batch_op(bp.b));
-            check_status(bp.b);
-            engine_check(s21_batch_get_x(bp.b, 0, x.data()));
         }
         std::map<std::string, double> vals;  // OpResult::from keeps the last value of a repeated name
         for (int k = 0; k < nl.n_vars; k++) vals[e.names[k]] = x[k];
@@ -882,7 +899,16 @@ int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len
         int nv;
         std::vector<double> t, wv;
         std::vector<std::string> vnames;
-        if (has_bsim4(nl)) {  // BSIM4 circuits run on the single-instance path
+        // BSIM4 circuits, and circuits too large for the batched kernel, run on the single-instance path
+        bool large = has_bsim4(nl);
+        if (!large) {
+            s21_group g{&nl, 1, 0, nullptr};
+            int rc = s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b);
+            if (rc == S21_OK) rc = s21_batch_tran(bp.b, &to);
+            large = too_large_for_batch(rc);
+            if (!large) engine_check(rc);
+        }
+        if (large) {
             engine_check(s21_large_create(&nl, &c.opts, S21_FP64, &lp.l));
             engine_check(s21_large_tran(lp.l, &to));
             check_status(lp.l);
@@ -896,9 +922,6 @@ int s21_tran_bytes(const uint8_t* in, size_t len, uint8_t** out, size_t* out_len
             engine_check(s21_large_get_wave(lp.l, nv, all.data(), wv.data()));
             for (int k = 0; k < nv; k++) vnames.push_back(s21_large_var_name(lp.l, k));
         } else {
-            s21_group g{&nl, 1, 0, nullptr};
-            engine_check(s21_batch_create(1, &g, &c.opts, S21_FP64, &bp.b));
-            engine_check(s21_batch_tran(bp.b, &to));
             check_status(bp.b);
             np = s21_batch_n_points(bp.b);
             nv = s21_batch_n_vars(bp.b, 0);

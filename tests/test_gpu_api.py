@@ -114,3 +114,30 @@ def test_mos1_ro_matches_oracle():
             worst = max(worst, float(np.max(np.abs(np.array(res[nm]) - X[:, k]))))
     print("mos1 ring vs oracle: max |dx|", worst, flush=True)
     assert worst <= 1e-9
+
+
+@pytest.mark.parametrize("size,why", [(24, "16-bit index"), (30, "slot index")])
+def test_circuit_too_large_for_the_batch_runs_on_the_single_instance_path(size, why):
+    """A power grid whose matrix does not fit the batched kernel (record offsets beyond 16 bits at
+    24 x 24, more slots than its slot codes address at 30 x 30) is refused by the batch and solved by the
+    byte ABI on the single-instance path: node voltages against the oracle to 1e-9 relative, branch
+    currents to 1e-12 A (sums of ~1 A mesh currents)."""
+    from spice21_amd import _abi as A
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import Batch, Group
+    from tests.test_elab_scale import power_grid_proto
+
+    kw = dict(size=size, n_inv=size * size // 8, n_diode=size * size // 20, pad_every=4)
+    nl = W.power_grid(**kw)
+    cn = nl.to_c()
+    with pytest.raises(S21Error, match="too large for the batched kernel.*" + why):
+        Batch(Group(cn)).op()
+    print("grid", size, "unknowns", cn.n_vars, flush=True)
+    res = api.dcop(protos.Op(ckt=power_grid_proto(**kw)))
+    xo, _, _ = O.dcop(cn)
+    v = np.asarray(nl.kinds) == A.S21_VAR_V
+    got = np.array([res[nm] for nm in nl.names])
+    rel = np.abs(got - xo)[v] / np.maximum(np.abs(xo[v]), 1e-9)
+    cur = np.abs(got - xo)[~v]
+    print("  max relative node difference", rel.max(), "max branch current difference", cur.max(), flush=True)
+    assert rel.max() < 1e-9 and cur.max() <= 1e-12
