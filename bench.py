@@ -132,7 +132,40 @@ def extra_measurements(which=("tran", "ac", "ring5", "grid", "bsim4")):
         out.update(_power_grid_single(tran=False))
     if "bsim4" in which:
         out.update(_bsim4_chain_mc())
+        out.update(_bsim4_ring_batch())
     return out
+
+
+def _bsim4_ring_batch(n_inst=1024, stages=5, tstop=0.5e-9):
+    """Batched BSIM4 transient: 1024 five-stage BSIM4 ring oscillators whose stage capacitors are drawn per
+    instance (U[1, 10] fF), 0-0.5 ns at 1 ps from node 1 at 0 V, both precisions (parity of such batches
+    with the oracle is tests/test_gpu_bsim4.py's).  Seven of the 1024 instances end in "Convergence
+    Failed", as the oracle's plain Newton iteration does on them (tests/test_gpu_bsim4.py::
+    test_bsim4_ring_batch_workload_matches_the_oracle_per_instance checks two)."""
+    import numpy as np
+
+    from spice21_amd import workloads as W
+    from spice21_amd.batch import FP64, MIXED, Batch, Group
+
+    nl, over = W.bsim4_ring_batch(n_inst, stages)
+    g = Group(nl.to_c(), n_inst, over)
+    rec, ws = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        b = Batch(g, precision=prec)
+        b.tran(1e-12, tstop, [("1", 0.0)])  # untimed: structure and parameters
+        b.tran(1e-12, tstop, [("1", 0.0)])
+        ms = b.timing()[0]
+        st, it = b.status()
+        npts = len(b.axis())
+        iters = int(it.sum())
+        rec[name] = {"tran_pts_per_s": n_inst * npts / (ms * 1e-3), "instance_iters_per_s": iters / (ms * 1e-3),
+                     "bsim4_evals_per_s": iters * 2 * stages / (ms * 1e-3), "kernel_ms": ms, "points": npts,
+                     "newton_iters": iters, "failed_instances": int(np.sum(st != 0)), "launch": list(b.config())}
+        ws[name] = b.wave(0, 0)
+    v = np.asarray(list(nl.kinds) + [0, 1]) == 0
+    rec["mixed_vs_fp64_max_node_v_instance0"] = float(np.abs(ws["mixed"] - ws["fp64"])[:, v[:ws["fp64"].shape[1]]].max())
+    return {"bsim4_ring_batch": {"workload": f"{n_inst} {stages}-stage BSIM4 ring oscillators (C per instance), "
+                                             f"0-{tstop * 1e9:g} ns at 1 ps", **rec}}
 
 
 def _bsim4_chain_mc(n_inst=4096, stages=16):

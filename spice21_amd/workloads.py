@@ -220,6 +220,30 @@ def bsim4_chain(stages=16, vin=0.5, vdd=1.0):
     return n, mos
 
 
+def bsim4_ring(stages=5, cap=1e-15, vdd=1.0):
+    """Ring oscillator of BSIM4 inverters (the default NMOS and PMOS models, W = 1 um, L = 0.1 um) with a
+    capacitor from every stage node to ground; the transient kicks node "1" through an initial condition
+    (the ring's DC point alone is an unstable equilibrium)."""
+    n = Netlist(signals=[str(k + 1) for k in range(stages)] + ["vdd"])
+    nm, pm, ip = n.bsim4_model(0), n.bsim4_model(1), n.bsim4_inst(w=1e-6, l=0.1e-6)
+    n.v("vdd", vdd, "vdd", "")
+    for k in range(stages):
+        a, b = str(k + 1), str((k + 1) % stages + 1)
+        n.bsim4(f"x{k}.p", b, a, "vdd", "vdd", pm, ip)
+        n.bsim4(f"x{k}.n", b, a, "", "", nm, ip)
+        n.c(cap, b, "", name=f"x{k}.c")
+    return n
+
+
+def bsim4_ring_batch(n_inst=1024, stages=5, seed=0xC7):
+    """Batch of BSIM4 rings whose stage capacitors are drawn per instance, U[1, 10] fF: (netlist,
+    overrides)."""
+    n = bsim4_ring(stages)
+    cap = 1e-15 + 9e-15 * uniform(seed, n_inst)
+    caps = [ci for ci, c in enumerate(n.comps) if c[0] == 1]  # S21_COMP_C
+    return n, [(ci, "value", 0, cap) for ci in caps]
+
+
 def bsim4_chain_mc(n_inst=1024, stages=16, seed=0xC6, sigma_vt=0.015, sigma_w=0.03):
     """Monte-Carlo BSIM4 inverter chains: (netlist, overrides) setting every device's own DELVTO and W
     (instance fields) per instance."""

@@ -338,6 +338,31 @@ def test_bsim4_junction_areas_on_the_gpu(vs):
             assert g.x()[k] == pytest.approx(want, rel=1e-9)
 
 
+def test_bsim4_ring_batch_workload_matches_the_oracle_per_instance():
+    """Instances of the benchmark's BSIM4 ring batch (workloads.bsim4_ring_batch: stage capacitors drawn per
+    instance), among them two on which the reference's plain Newton iteration fails (the oracle reports
+    "Convergence Failed" there; the batch must report the same status), the others' waveforms within 1e-9
+    of each signal's swing."""
+    from tests.circuits import materialize
+    nl, over = W.bsim4_ring_batch(1024, 5)
+    ids = [0, 1, 2, 194, 337]
+    sub = [(ci, f, k, np.asarray(v)[ids]) for ci, f, k, v in over]
+    b = Batch(Group(nl.to_c(), len(ids), sub)).tran(1e-12, 0.5e-9, [("1", 0.0)])
+    st = b.status()[0]
+    kinds = list(nl.kinds) + [0, 1]
+    for j, i in enumerate(ids):
+        try:
+            t, X, names, _ = O.tran(materialize(nl, over, i).to_c(), 1e-12, 0.5e-9, [("1", 0.0)])
+        except O.OracleError as e:
+            log("instance", i, "oracle:", e, "batch status", st[j])
+            assert "Convergence Failed" in str(e) and st[j] == A.S21_CONVERGENCE
+            continue
+        log("instance", i, "batch status", st[j])
+        assert st[j] == A.S21_OK
+        assert np.array_equal(b.axis(), t) and b.var_names() == list(names)
+        assert wave_within(b.wave(0, j), X, names, kinds, f"bsim4 ring {i}")
+
+
 def test_bsim4_error_paths():
     nl = ro3("bsim4_cmos")
     with pytest.raises(S21Error, match="profiling kernels"):
