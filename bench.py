@@ -479,7 +479,7 @@ def main():
             allowed = len(os.sched_getaffinity(0))
         except AttributeError:
             allowed = nproc
-        # every core this process may use, up to the host-thread share of one GPU (16 on the GPU boxes)
+        # every core this process may use, up to the host-thread share of one GPU (--cpu-threads, 16)
         threads = max(1, min(args.cpu_threads, allowed))
         walls = []
         for _ in range(args.cpu_reps):  # the median of a few repetitions: a rate steadier than one pass
