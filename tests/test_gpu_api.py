@@ -141,3 +141,23 @@ def test_circuit_too_large_for_the_batch_runs_on_the_single_instance_path(size, 
     cur = np.abs(got - xo)[~v]
     print("  max relative node difference", rel.max(), "max branch current difference", cur.max(), flush=True)
     assert rel.max() < 1e-9 and cur.max() <= 1e-12
+
+
+def test_transient_too_large_for_the_batch_runs_on_the_single_instance_path():
+    """The transient byte call on the 24 x 24 grid (refused by the batched kernel): the single-instance
+    path's waveform against the oracle's, point for point."""
+    from spice21_amd import _abi as A
+    from spice21_amd import workloads as W
+    from tests.test_elab_scale import power_grid_proto
+    from tests.test_gpu_parity import wave_within
+
+    kw = dict(size=24, n_inv=72, n_diode=28, pad_every=4)
+    nl = W.power_grid(**kw)
+    args = protos.TranOptions(tstep=1e-12, tstop=4e-12)
+    args.ic["g0_0"] = 0.0
+    res = api.tran(protos.Tran(ckt=power_grid_proto(**kw), args=args))
+    t, X, names, _ = O.tran(nl.to_c(), 1e-12, 4e-12, [("g0_0", 0.0)])
+    assert np.array_equal(np.array(res["time"]), t)
+    got = np.stack([np.array(res[nm]) for nm in names], axis=1)
+    assert len(names) == len(nl.kinds) + 2  # the initial condition's source and its current
+    assert wave_within(got, X, names, list(nl.kinds) + [A.S21_VAR_V, A.S21_VAR_I], "grid tran bytes")
