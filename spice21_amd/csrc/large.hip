@@ -615,10 +615,25 @@ __global__ void __launch_bounds__(TPB) k_backward(Fronts S, const int* __restric
     T* w = wbuf + S.w_off[s];
     const int* R = S.r_idx + S.r_ptr[s];
     const int tid = threadIdx.x;
-    // w = y - U12 x_R (the ancestors' solution is final)
+    // w = y - U12 x_R (the ancestors' solution is final), eight columns' loads in flight at a time,
+    // subtracted in column order
     for (int j = tid; j < k; j += TPB) {
         T acc = y[f + j];
-        for (int c = k; c < F; c++) acc = acc - A[(size_t)c * F + j] * y[R[c - k]];
+        int c = k;
+        for (; c + 7 < F; c += 8) {
+            int ri[8];
+            T a[8], yv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                ri[u] = R[c + u - k];
+                a[u] = A[(size_t)(c + u) * F + j];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) yv[u] = y[ri[u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = acc - a[u] * yv[u];
+        }
+        for (; c < F; c++) acc = acc - A[(size_t)c * F + j] * y[R[c - k]];
         w[j] = acc;
     }
     __syncthreads();
