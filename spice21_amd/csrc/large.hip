@@ -58,6 +58,8 @@ template <typename T>
 constexpr int small_max() {
     return sizeof(T) == 8 ? 128 : 181;  // F * F * sizeof(T) <= SMALL_LDS
 }
+static_assert(small_max<double>() <= TPB && small_max<float>() <= TPB,
+              "the small-front sweeps give every row of a front a thread of its own");
 
 // ---------------------------------------------------------------------------------------------------
 // Device evaluation
@@ -598,10 +600,24 @@ __global__ void __launch_bounds__(TPB) k_forward(Fronts S, const int* __restrict
         for (int q = tid; q < mc; q += TPB) w[mp[q]] += wc[q];
         __syncthreads();
     }
-    for (int j = 0; j < k; j++) {
-        const T wj = w[j];
-        for (int i = j + 1 + tid; i < F; i += TPB) w[i] = w[i] - A[(size_t)j * F + i] * wj;
-        __syncthreads();
+    // thread i owns row i (a small front has at most TPB rows) and loads its entries of the next eight
+    // pivots' columns at once
+    for (int j0 = 0; j0 < k; j0 += 8) {
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 + u;
+            a[u] = (j < k && tid > j && tid < F) ? A[(size_t)j * F + tid] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 + u;
+            if (j < k) {
+                const T wj = w[j];
+                if (tid > j && tid < F) w[tid] = w[tid] - a[u] * wj;
+                __syncthreads();
+            }
+        }
     }
     for (int j = tid; j < k; j += TPB) y[f + j] = w[j];
 }
@@ -637,12 +653,26 @@ __global__ void __launch_bounds__(TPB) k_backward(Fronts S, const int* __restric
         w[j] = acc;
     }
     __syncthreads();
-    for (int j = k - 1; j >= 0; j--) {
-        if (tid == 0) w[j] = w[j] / A[(size_t)j * F + j];
-        __syncthreads();
-        const T xj = w[j];
-        for (int i = tid; i < j; i += TPB) w[i] = w[i] - A[(size_t)j * F + i] * xj;
-        __syncthreads();
+    // thread i owns row i and loads its entries of the next eight pivots' columns at once (thread j
+    // the pivot itself)
+    for (int j0 = k - 1; j0 >= 0; j0 -= 8) {
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 - u;
+            a[u] = (j >= 0 && tid <= j) ? A[(size_t)j * F + tid] : (T)1;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 - u;
+            if (j >= 0) {
+                if (tid == j) w[j] = w[j] / a[u];
+                __syncthreads();
+                const T xj = w[j];
+                if (tid < j) w[tid] = w[tid] - a[u] * xj;
+                __syncthreads();
+            }
+        }
     }
     for (int j = tid; j < k; j += TPB) y[f + j] = w[j];
 }
