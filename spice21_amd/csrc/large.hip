@@ -434,13 +434,13 @@ __global__ void __launch_bounds__(64) k_diag(Fronts S, const int* __restrict__ l
         if (j < w) {
             const T piv = lane_value(r[j], j);
             if (piv == (T)0 && lane == 0) atomicOr(flags, 2);
-            const bool below = lane > j && own;
-            if (below) r[j] = r[j] / piv;
-            const T l = r[j];
+            // the rows below update under the execution mask (readlane reads the pivot lane whatever the
+            // mask), not through a select per entry
+            if (lane > j && own) {
+                r[j] = r[j] / piv;
+                const T l = r[j];
 #pragma unroll
-            for (int q = j + 1; q < NB; q++) {
-                const T u = lane_value(r[q], j);
-                if (below) r[q] = r[q] - l * u;
+                for (int q = j + 1; q < NB; q++) r[q] = r[q] - l * lane_value(r[q], j);
             }
         }
     }
